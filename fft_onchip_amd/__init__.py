@@ -1,0 +1,158 @@
+"""fft_onchip_amd -- batched small complex FFTs (N = 8..4096) for MI355X.
+
+The product is the C-ABI library ``libfftoc.so`` (include/fftoc.h) with
+hand-written gfx950 HIP kernels.  This module is its host-side mirror: the
+same entry points as the reference's size-templated functors
+(``tensor_fft_N``, ``legacy_fft`` = radix 8, ``legacy16_fft`` = radix 16;
+includes/tensor_fft_*.cuh, includes/legacy8_fft.cuh, includes/legacy16_fft.cuh)
+taking device buffers, plus a host-array convenience wrapper.
+
+Data layout is the reference's: interleaved complex<double>, each signal
+contiguous, signals back to back (includes/config.hpp:6).  Transforms run in
+place when ``out`` is omitted, like the reference's in-shared-memory functors.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _native
+from .hip import DeviceBuffer, Event, Stream, device_count, synchronize  # noqa: F401
+
+FORWARD = 0x0
+INVERSE = 0x1
+NORMALIZE = 0x2
+PREC_FP64 = 0x00
+PREC_FP32 = 0x10
+PREC_BF16X3 = 0x20
+_PREC = {"fp64": PREC_FP64, "fp32": PREC_FP32, "bf16x3": PREC_BF16X3}
+
+STATUS = {
+    0: "SUCCESS", 1: "INVALID_SIZE", 2: "INVALID_BATCH", 3: "NULL_POINTER", 4: "INVALID_FLAGS",
+    5: "OVERLAP", 6: "UNSUPPORTED", 7: "DEVICE", 8: "LAUNCH", 9: "ALLOC", 10: "MISALIGNED",
+    11: "BUDGET", 12: "QUEUE_FULL", 13: "INVALID_HANDLE",
+}
+TENSOR_SIZES = _native.TENSOR_SIZES
+LEGACY8_SIZES = (8, 64, 512, 4096)
+LEGACY16_SIZES = (16, 256, 4096)
+ALL_SIZES = tuple(1 << k for k in range(3, 13))
+
+
+class FftocError(RuntimeError):
+    def __init__(self, status: int, where: str = ""):
+        self.status = status
+        self.name = STATUS.get(status, str(status))
+        msg = _native.load().fftoc_strerror(status).decode()
+        super().__init__(f"{where}: FFTOC_ERROR_{self.name} ({msg})" if where else msg)
+
+
+def lib():
+    return _native.load()
+
+
+def _ptr(x) -> int | None:
+    if x is None:
+        return None
+    if isinstance(x, DeviceBuffer):
+        return x.ptr
+    if isinstance(x, int):
+        return x
+    if hasattr(x, "data_ptr"):
+        return int(x.data_ptr())
+    raise TypeError(f"cannot take a device pointer from {type(x)!r}")
+
+
+def _stream(s) -> int | None:
+    if s is None:
+        return None
+    if isinstance(s, Stream):
+        return s.handle
+    if isinstance(s, int):
+        return s
+    if hasattr(s, "cuda_stream"):
+        return int(s.cuda_stream)
+    raise TypeError(f"cannot take a stream handle from {type(s)!r}")
+
+
+def make_flags(inverse: bool = False, normalize: bool = False, precision: str = "fp64",
+               tile: int = 0) -> int:
+    f = (INVERSE if inverse else 0) | (NORMALIZE if normalize else 0) | _PREC[precision]
+    return f | ((int(tile) & 0xFF) << 16)
+
+
+def _call(fn_name: str, x, out, batch: int, n: int | None, flags: int, stream) -> None:
+    f = getattr(lib(), fn_name)
+    ip = _ptr(x)
+    op = _ptr(out) if out is not None else ip
+    if n is None:
+        rc = f(ip, op, int(batch), int(flags), _stream(stream))
+    else:
+        rc = f(ip, op, int(batch), int(n), int(flags), _stream(stream))
+    if rc != 0:
+        raise FftocError(rc, fn_name)
+
+
+def tensor_fft(x, batch: int, n: int, out=None, flags: int = 0, stream=None) -> None:
+    """FP64-MFMA Karatsuba path (the reference's tensor_fft_N functors)."""
+    _call("fftoc_tensor_fft", x, out, batch, n, flags, stream)
+
+
+def _make_sized(n):
+    def f(x, batch: int, out=None, flags: int = 0, stream=None) -> None:
+        _call(f"fftoc_tensor_fft_{n}", x, out, batch, None, flags, stream)
+    f.__name__ = f"tensor_fft_{n}"
+    f.__doc__ = f"{n}-point FP64 MFMA FFT (includes/tensor_fft_{n}.cuh)."
+    return f
+
+
+for _n in TENSOR_SIZES:
+    globals()[f"tensor_fft_{_n}"] = _make_sized(_n)
+
+
+def legacy_fft(x, batch: int, n: int, out=None, flags: int = 0, stream=None) -> None:
+    """Radix-8 register-butterfly path (includes/legacy8_fft.cuh), n = 8^k."""
+    _call("fftoc_legacy8_fft", x, out, batch, n, flags, stream)
+
+
+def legacy16_fft(x, batch: int, n: int, out=None, flags: int = 0, stream=None) -> None:
+    """Radix-16 register-butterfly path (includes/legacy16_fft.cuh), n = 16^k."""
+    _call("fftoc_legacy16_fft", x, out, batch, n, flags, stream)
+
+
+def fft(x, batch: int, n: int, out=None, flags: int = 0, stream=None) -> None:
+    """Fastest FP64 kernel for n (any power of two 8..4096)."""
+    _call("fftoc_fft", x, out, batch, n, flags, stream)
+
+
+_PATHS = {"tensor": tensor_fft, "legacy8": legacy_fft, "legacy16": legacy16_fft, "best": fft}
+
+
+def fft_host(x: np.ndarray, path: str = "best", inverse: bool = False, normalize: bool = False,
+             precision: str = "fp64", in_place: bool = False) -> np.ndarray:
+    """Uploads complex128 signals (batch, n), transforms them on the GPU and
+    returns the result as a new host array."""
+    x = np.ascontiguousarray(x, dtype=np.complex128)
+    if x.ndim == 1:
+        x = x.reshape(1, -1)
+    batch, n = x.shape
+    flags = make_flags(inverse, normalize, precision)
+    src = DeviceBuffer.from_array(x)
+    if in_place:
+        _PATHS[path](src, batch, n, None, flags)
+        res = src
+    else:
+        res = DeviceBuffer(x.nbytes)
+        _PATHS[path](src, batch, n, res, flags)
+    synchronize()
+    return res.download(x.shape)
+
+
+def init() -> None:
+    rc = lib().fftoc_init()
+    if rc != 0:
+        raise FftocError(rc, "fftoc_init")
+
+
+def version() -> int:
+    return int(lib().fftoc_version())

@@ -1,0 +1,54 @@
+"""Loads libfftoc.so (built in-tree by ``make -C fft_onchip_amd/csrc``) and
+declares the C ABI of include/fftoc.h.  There is no fallback: if the library
+is missing the import of the transform entry points fails loudly."""
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libfftoc.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "fftoc.h")
+
+_lib = None
+
+_vp = ctypes.c_void_p
+_i64 = ctypes.c_int64
+_u32 = ctypes.c_uint32
+_int = ctypes.c_int
+
+# name -> (argtypes, restype)
+SIGNATURES = {
+    "fftoc_init": ([], _int),
+    "fftoc_shutdown": ([], _int),
+    "fftoc_strerror": ([_int], ctypes.c_char_p),
+    "fftoc_version": ([], _int),
+    "fftoc_tensor_fft": ([_vp, _vp, _i64, _int, _u32, _vp], _int),
+    "fftoc_legacy8_fft": ([_vp, _vp, _i64, _int, _u32, _vp], _int),
+    "fftoc_legacy16_fft": ([_vp, _vp, _i64, _int, _u32, _vp], _int),
+    "fftoc_fft": ([_vp, _vp, _i64, _int, _u32, _vp], _int),
+}
+TENSOR_SIZES = (8, 16, 64, 128, 256, 512, 4096)
+for _n in TENSOR_SIZES:
+    SIGNATURES[f"fftoc_tensor_fft_{_n}"] = ([_vp, _vp, _i64, _u32, _vp], _int)
+
+
+class LibraryMissing(ImportError):
+    pass
+
+
+def load():
+    """Returns the loaded ctypes library (raises LibraryMissing if not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise LibraryMissing(
+                f"{LIB_PATH} is not built; run `make -C fft_onchip_amd/csrc` "
+                "(or __graft_entry__.build())")
+        lib = ctypes.CDLL(LIB_PATH, mode=ctypes.RTLD_GLOBAL)
+        for name, (args, res) in SIGNATURES.items():
+            f = getattr(lib, name)
+            f.argtypes = args
+            f.restype = res
+        _lib = lib
+    return _lib

@@ -1,0 +1,134 @@
+// Device-side building blocks shared by the gfx950 FFT kernels: complex FP64
+// arithmetic, fixed-root multiplications, and in-register DFT butterflies.
+//
+// Conventions: a complex value is a double2 {x = re, y = im}; every butterfly
+// computes a forward DFT (exp(-2 pi i nk/R)) in place with natural-order
+// output.  Inverse transforms reuse the forward kernels by swapping re/im on
+// load and on store (swap(fwd(swap(x))) = inverse DFT of x).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace fftoc {
+
+using c2 = double2;
+
+// Correctly rounded FP64 constants of the 16th roots of unity.
+constexpr double kSqrtHalf = 0.70710678118654752440;  // cos(pi/4)
+constexpr double kCos8 = 0.92387953251128675613;      // cos(pi/8)
+constexpr double kSin8 = 0.38268343236508977173;      // sin(pi/8)
+
+// Twiddle table length: W_4096^j, j = 0..4095, covers every N <= 4096.
+constexpr int kTwN = 4096;
+constexpr int kTwLog = 12;
+
+__device__ __forceinline__ c2 cadd(c2 a, c2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ c2 csub(c2 a, c2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ c2 cmul(c2 a, c2 w) {
+  return make_double2(__fma_rn(a.x, w.x, -a.y * w.y), __fma_rn(a.x, w.y, a.y * w.x));
+}
+__device__ __forceinline__ c2 cswap(c2 a) { return make_double2(a.y, a.x); }
+__device__ __forceinline__ c2 cscale(c2 a, double s) { return make_double2(a.x * s, a.y * s); }
+
+// Multiplication by fixed roots W_16^j = exp(-2 pi i j / 16).
+__device__ __forceinline__ c2 mul_mi(c2 a) { return make_double2(a.y, -a.x); }  // * -i = W16^4
+__device__ __forceinline__ c2 mul_w8_1(c2 a) {  // * (1 - i)/sqrt2 = W16^2
+  return make_double2(kSqrtHalf * (a.x + a.y), kSqrtHalf * (a.y - a.x));
+}
+__device__ __forceinline__ c2 mul_w8_3(c2 a) {  // * (-1 - i)/sqrt2 = W16^6
+  return make_double2(kSqrtHalf * (a.y - a.x), -kSqrtHalf * (a.x + a.y));
+}
+__device__ __forceinline__ c2 mul_w16_1(c2 a) {  // * (c - i s)
+  return make_double2(__fma_rn(a.x, kCos8, a.y * kSin8), __fma_rn(a.y, kCos8, -a.x * kSin8));
+}
+__device__ __forceinline__ c2 mul_w16_3(c2 a) {  // * (s - i c)
+  return make_double2(__fma_rn(a.x, kSin8, a.y * kCos8), __fma_rn(a.y, kSin8, -a.x * kCos8));
+}
+__device__ __forceinline__ c2 mul_w16_9(c2 a) {  // * (-c + i s)
+  return make_double2(-__fma_rn(a.x, kCos8, a.y * kSin8), __fma_rn(a.x, kSin8, -a.y * kCos8));
+}
+
+// ---- in-register DFTs (natural order in, natural order out) ---------------
+__device__ __forceinline__ void dft2(c2 &a0, c2 &a1) {
+  c2 t = csub(a0, a1);
+  a0 = cadd(a0, a1);
+  a1 = t;
+}
+
+__device__ __forceinline__ void dft4(c2 &a0, c2 &a1, c2 &a2, c2 &a3) {
+  c2 t0 = cadd(a0, a2), t1 = csub(a0, a2);
+  c2 t2 = cadd(a1, a3), t3 = mul_mi(csub(a1, a3));
+  a0 = cadd(t0, t2);
+  a2 = csub(t0, t2);
+  a1 = cadd(t1, t3);
+  a3 = csub(t1, t3);
+}
+
+// DFT8 = radix-2 DIT over two DFT4s: X[k] = E[k] + W8^k O[k].
+__device__ __forceinline__ void dft8(c2 *a) {
+  c2 e0 = a[0], e1 = a[2], e2 = a[4], e3 = a[6];
+  c2 o0 = a[1], o1 = a[3], o2 = a[5], o3 = a[7];
+  dft4(e0, e1, e2, e3);
+  dft4(o0, o1, o2, o3);
+  o1 = mul_w8_1(o1);
+  o2 = mul_mi(o2);
+  o3 = mul_w8_3(o3);
+  a[0] = cadd(e0, o0); a[4] = csub(e0, o0);
+  a[1] = cadd(e1, o1); a[5] = csub(e1, o1);
+  a[2] = cadd(e2, o2); a[6] = csub(e2, o2);
+  a[3] = cadd(e3, o3); a[7] = csub(e3, o3);
+}
+
+// DFT16 = 4 x 4: x[4 n1 + n2]; DFT4 over n1, twiddle W16^{n2 k1}, DFT4 over n2;
+// X[k1 + 4 k2].
+__device__ __forceinline__ void dft16(c2 *a) {
+  c2 y[16];
+#pragma unroll
+  for (int n2 = 0; n2 < 4; ++n2) {
+    c2 b0 = a[n2], b1 = a[4 + n2], b2 = a[8 + n2], b3 = a[12 + n2];
+    dft4(b0, b1, b2, b3);
+    y[4 * n2 + 0] = b0; y[4 * n2 + 1] = b1; y[4 * n2 + 2] = b2; y[4 * n2 + 3] = b3;
+  }
+  // y[4 n2 + k1] *= W16^{n2 k1}
+  y[5] = mul_w16_1(y[5]);   // n2=1,k1=1 -> W^1
+  y[6] = mul_w8_1(y[6]);    // W^2
+  y[7] = mul_w16_3(y[7]);   // W^3
+  y[9] = mul_w8_1(y[9]);    // n2=2,k1=1 -> W^2
+  y[10] = mul_mi(y[10]);    // W^4
+  y[11] = mul_w8_3(y[11]);  // W^6
+  y[13] = mul_w16_3(y[13]); // n2=3,k1=1 -> W^3
+  y[14] = mul_w8_3(y[14]);  // W^6
+  y[15] = mul_w16_9(y[15]); // W^9
+#pragma unroll
+  for (int k1 = 0; k1 < 4; ++k1) {
+    c2 b0 = y[k1], b1 = y[4 + k1], b2 = y[8 + k1], b3 = y[12 + k1];
+    dft4(b0, b1, b2, b3);
+    a[k1] = b0; a[k1 + 4] = b1; a[k1 + 8] = b2; a[k1 + 12] = b3;
+  }
+}
+
+template <int R>
+__device__ __forceinline__ void dft(c2 *a) {
+  if constexpr (R == 2) {
+    dft2(a[0], a[1]);
+  } else if constexpr (R == 4) {
+    dft4(a[0], a[1], a[2], a[3]);
+  } else if constexpr (R == 8) {
+    dft8(a);
+  } else {
+    static_assert(R == 16, "radix must be 2, 4, 8 or 16");
+    dft16(a);
+  }
+}
+
+// Wave-scope ordering point for LDS exchanges among the lanes of one wave:
+// LDS operations of a wave execute in issue order, so only compiler
+// reordering has to be prevented.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+}  // namespace fftoc

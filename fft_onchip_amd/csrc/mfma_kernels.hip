@@ -1,0 +1,483 @@
+// FP64 MFMA ("tensor") FFT kernels for gfx950: DFT stages computed as dense
+// 16x16 complex matrix products on v_mfma_f64_16x16x4_f64 with the
+// three-real-product (Karatsuba) complex GEMM of the reference
+// (includes/tensor_utils.cuh:25-36: s1 = Ar Br, s2 = Ai Bi,
+// s3 = (Ar+Ai)(Br+Bi); Re = s1 - s2, Im = s3 - s1 - s2) and the
+// DFT -> twiddle -> DFT structure of includes/c64_fft64.cuh:8-27.
+//
+// CDNA4 operand layout of v_mfma_f64_16x16x4_f64 (wave64, lane l, g = l/16,
+// j = l%16):  A[i][k]: lane l holds A[j][g];  B[k][col]: lane l holds B[g][j];
+// D[i][col]: lane l register r holds D[4g + r][j].
+//
+// Design points:
+//  * The signal goes into the A operand (rows = independent sub-transforms,
+//    K = the 16 inputs of one DFT16, 4 K-chunks), the DFT matrix into B.
+//  * For two-stage transforms (256 = 16 x 16 and the sizes built on it) the
+//    four D registers of stage 1 ARE the A operands of the four K-chunks of
+//    stage 2 once the K order of stage 1 is taken as n1 = 4g + c: the data
+//    transpose between the stages costs no data movement (the role of the
+//    fragment reuse in includes/c64_fft64.cuh:24-26).
+//  * Both orders use the same per-lane B fragment (F16 is symmetric), so a
+//    wave keeps 12 doubles of DFT matrix (Br, Bi, Br+Bi per K-chunk).
+//  * Twiddles and DFT-matrix entries come from the FP64 table W_4096^j built
+//    on the host; only additions are done in-kernel (Br + Bi).
+#include "fftoc_device.hpp"
+#include "fftoc_internal.hpp"
+
+namespace fftoc {
+namespace {
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ v4d mfma16(double a, double b, v4d c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ v4d vzero() { return v4d{0.0, 0.0, 0.0, 0.0}; }
+
+// B fragment of a 16x16 DFT-type matrix M[k][col] = W_16^{k*col} in the
+// Karatsuba form.  K order: chunk c, slot g -> k = 4c + g (ORDER_CG) or
+// k = 4g + c (ORDER_GC).
+struct Frag {
+  double r[4], i[4], s[4];
+};
+
+enum KOrder { ORDER_CG = 0, ORDER_GC = 1 };
+
+template <KOrder ORD>
+__device__ __forceinline__ Frag dft16_frag(const c2 *__restrict__ tw, int lane) {
+  const int g = lane >> 4, j = lane & 15;
+  Frag f;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const int k = (ORD == ORDER_CG) ? 4 * c + g : 4 * g + c;
+    const c2 w = tw[((k * j) & 15) << 8];  // W_16^{kj} = W_4096^{256 kj}
+    f.r[c] = w.x;
+    f.i[c] = w.y;
+    f.s[c] = w.x + w.y;
+  }
+  return f;
+}
+
+// Block-diagonal diag(F8, F8) with K order k = 4c + g.
+__device__ __forceinline__ Frag dft8x2_frag(const c2 *__restrict__ tw, int lane) {
+  const int g = lane >> 4, j = lane & 15;
+  Frag f;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const int k = 4 * c + g;
+    if ((k < 8) == (j < 8)) {
+      const c2 w = tw[(((k & 7) * (j & 7)) & 7) << 9];  // W_8^{..} = W_4096^{512 ..}
+      f.r[c] = w.x;
+      f.i[c] = w.y;
+      f.s[c] = w.x + w.y;
+    } else {
+      f.r[c] = 0.0;
+      f.i[c] = 0.0;
+      f.s[c] = 0.0;
+    }
+  }
+  return f;
+}
+
+// Complex 16x16x16 GEMM Y = X * M with 3 real MFMA chains (Karatsuba).
+// a[c] = A operand of K-chunk c as (re, im).
+__device__ __forceinline__ void kgemm(const c2 (&a)[4], const Frag &f, v4d &yr, v4d &yi) {
+  v4d p1 = vzero(), p2 = vzero(), p3 = vzero();
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    p1 = mfma16(a[c].x, f.r[c], p1);
+    p2 = mfma16(a[c].y, f.i[c], p2);
+    p3 = mfma16(a[c].x + a[c].y, f.s[c], p3);
+  }
+  yr = p1 - p2;
+  yi = p3 - p1 - p2;
+}
+
+template <bool INV>
+__device__ __forceinline__ c2 ld_in(const c2 *p) {
+  c2 v = *p;
+  if constexpr (INV) v = cswap(v);
+  return v;
+}
+
+template <bool INV>
+__device__ __forceinline__ void st_out(c2 *p, c2 v, double scale) {
+  if (scale != 1.0) v = cscale(v, scale);
+  if constexpr (INV) v = cswap(v);
+  *p = v;
+}
+
+__device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
+__device__ __forceinline__ int64_t wave_global() {
+  return (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+}
+
+constexpr int kWaves = 4;  // waves per workgroup for the one-unit-per-wave kernels
+
+// ---------------------------------------------------------------------------
+// N = 8: 32 signals per wave.  A = [x_a | x_b] (16 rows, K = 16: chunks 0-1
+// hold signal i's 8 inputs, chunks 2-3 those of signal 16+i), B = diag(F8,F8).
+template <bool INV>
+__global__ __launch_bounds__(64 * kWaves) void tensor8_kernel(const c2 *in, c2 *out, int64_t batch,
+                                                              const c2 *__restrict__ tw,
+                                                              double scale) {
+  const int lane = lane_id(), g = lane >> 4, j = lane & 15;
+  const int64_t s0 = wave_global() * 32;
+  if (s0 >= batch) return;
+  const Frag f = dft8x2_frag(tw, lane);
+  c2 a[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const int64_t s = s0 + (c < 2 ? 0 : 16) + j;
+    const int e = 4 * (c & 1) + g;
+    a[c] = (s < batch) ? ld_in<INV>(in + s * 8 + e) : make_double2(0.0, 0.0);
+  }
+  v4d yr, yi;
+  kgemm(a, f, yr, yi);
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int64_t s = s0 + (j < 8 ? 0 : 16) + 4 * g + r;
+    if (s < batch) st_out<INV>(out + s * 8 + (j & 7), make_double2(yr[r], yi[r]), scale);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// N = 16: 16 signals per wave, one DFT16 per row.
+template <bool INV>
+__global__ __launch_bounds__(64 * kWaves) void tensor16_kernel(const c2 *in, c2 *out, int64_t batch,
+                                                               const c2 *__restrict__ tw,
+                                                               double scale) {
+  const int lane = lane_id(), g = lane >> 4, j = lane & 15;
+  const int64_t s0 = wave_global() * 16;
+  if (s0 >= batch) return;
+  const Frag f = dft16_frag<ORDER_CG>(tw, lane);
+  c2 a[4];
+  const int64_t sa = s0 + j;
+#pragma unroll
+  for (int c = 0; c < 4; ++c)
+    a[c] = (sa < batch) ? ld_in<INV>(in + sa * 16 + 4 * c + g) : make_double2(0.0, 0.0);
+  v4d yr, yi;
+  kgemm(a, f, yr, yi);
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int64_t s = s0 + 4 * g + r;
+    if (s < batch) st_out<INV>(out + s * 16 + j, make_double2(yr[r], yi[r]), scale);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// N = 16 * M with M in {2, 4, 8}: n = M n1 + n2.  Rows of the MFMA are
+// (signal, n2) pairs, 16/M signals per wave.  Stage 1: DFT16 over n1 on
+// MFMA; then twiddle W_N^{n2 k1} and a DFT_M over n2 on the VALU.  For M = 2
+// and M = 4 a lane's four D registers hold every n2 of its (signal, k1); for
+// M = 8 the other half sits in lane l ^ 16 and is fetched with one exchange.
+template <int M, bool INV>
+__global__ __launch_bounds__(64 * kWaves) void tensor16xm_kernel(const c2 *in, c2 *out,
+                                                                 int64_t batch,
+                                                                 const c2 *__restrict__ tw,
+                                                                 double scale) {
+  constexpr int N = 16 * M;
+  constexpr int SPW = 16 / M;  // signals per wave
+  const int lane = lane_id(), g = lane >> 4, j = lane & 15;
+  const int64_t s0 = wave_global() * SPW;
+  if (s0 >= batch) return;
+  const Frag f = dft16_frag<ORDER_CG>(tw, lane);
+  // A operand: row i = j -> (signal j / M, n2 = j % M); k = n1 = 4c + g
+  c2 a[4];
+  {
+    const int64_t s = s0 + j / M;
+    const int n2 = j % M;
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+      a[c] = (s < batch) ? ld_in<INV>(in + s * N + M * (4 * c + g) + n2) : make_double2(0.0, 0.0);
+  }
+  v4d yr, yi;
+  kgemm(a, f, yr, yi);
+  // D: register r of lane l holds row i = 4g + r -> (signal i / M, n2 = i % M), k1 = j
+  c2 y[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int n2 = (4 * g + r) % M;
+    c2 v = make_double2(yr[r], yi[r]);
+    if (n2 != 0) v = cmul(v, tw[(n2 * j) * (kTwN / N)]);
+    y[r] = v;
+  }
+  if constexpr (M == 2) {
+    // registers (0,1) -> signal 2g, (2,3) -> signal 2g+1; DFT2 over n2
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int64_t s = s0 + 2 * g + h;
+      c2 u0 = y[2 * h], u1 = y[2 * h + 1];
+      dft2(u0, u1);
+      if (s < batch) {
+        st_out<INV>(out + s * N + j, u0, scale);
+        st_out<INV>(out + s * N + j + 16, u1, scale);
+      }
+    }
+  } else if constexpr (M == 4) {
+    // all four n2 of signal g in this lane: DFT4 over r
+    dft4(y[0], y[1], y[2], y[3]);
+    const int64_t s = s0 + g;
+    if (s < batch) {
+#pragma unroll
+      for (int k2 = 0; k2 < 4; ++k2) st_out<INV>(out + s * N + j + 16 * k2, y[k2], scale);
+    }
+  } else {
+    static_assert(M == 8, "M must be 2, 4 or 8");
+    // lane holds n2 = 4h + r (h = g & 1) of signal g >> 1; partner lane l^16
+    // holds the other h.  DFT8 over n2 = 4h + r with k2 = k2a + 2 k2b:
+    // radix-2 over h (lane pair), twiddle W8^{r k2a}, DFT4 over r.
+    const int h = g & 1;
+    c2 p[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      p[r].x = __shfl_xor(y[r].x, 16);
+      p[r].y = __shfl_xor(y[r].y, 16);
+    }
+    // lane with h = 0 computes k2a = 0 (sum), h = 1 computes k2a = 1 (difference)
+    c2 u[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) u[r] = (h == 0) ? cadd(y[r], p[r]) : csub(p[r], y[r]);
+    if (h == 1) {
+      u[1] = mul_w8_1(u[1]);
+      u[2] = mul_mi(u[2]);
+      u[3] = mul_w8_3(u[3]);
+    }
+    dft4(u[0], u[1], u[2], u[3]);
+    const int64_t s = s0 + (g >> 1);
+    if (s < batch) {
+#pragma unroll
+      for (int k2b = 0; k2b < 4; ++k2b)
+        st_out<INV>(out + s * N + j + 16 * (h + 2 * k2b), u[k2b], scale);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 256-point core on one wave: input a[c] = x[64g + 16c + j] (ORDER_GC,
+// n = 16 n1 + n2 with n1 = 4g + c, n2 = j); output z[r] = X[4g + r + 16 j].
+// tw256 = W_256 step in the 4096 table (16), used for the inter-stage
+// twiddle W_256^{n2 k1}.
+__device__ __forceinline__ void core256(c2 (&a)[4], const Frag &f, const c2 *__restrict__ tw,
+                                        int lane, c2 (&z)[4]) {
+  const int g = lane >> 4, j = lane & 15;
+  v4d yr, yi;
+  kgemm(a, f, yr, yi);
+  // D: reg r = Y[n2 = 4g + r][k1 = j]; twiddle W_256^{n2 k1}
+  c2 b[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int n2 = 4 * g + r;
+    b[r] = cmul(make_double2(yr[r], yi[r]), tw[(n2 * j) << 4]);
+  }
+  // stage 2: A chunk r = b[r] (rows = k1 = j, K = n2 = 4g + r)
+  kgemm(b, f, yr, yi);
+#pragma unroll
+  for (int r = 0; r < 4; ++r) z[r] = make_double2(yr[r], yi[r]);
+}
+
+// LDS element swizzle for the output transposes: XOR the low 4 bits of the
+// element index with bits 4..7 (16 x 16-byte slots per 256 B bank window).
+__device__ __forceinline__ int swz16(int e) { return e ^ ((e >> 4) & 15); }
+
+// N = 256: one signal per wave, two MFMA stages, output re-ordered through
+// 4 KiB of LDS per wave so every store instruction writes 1 KiB contiguously.
+template <bool INV>
+__global__ __launch_bounds__(64 * kWaves) void tensor256_kernel(const c2 *in, c2 *out,
+                                                                int64_t batch,
+                                                                const c2 *__restrict__ tw,
+                                                                double scale) {
+  __shared__ c2 lds[kWaves * 256];
+  const int lane = lane_id(), g = lane >> 4, j = lane & 15;
+  const int64_t s = wave_global();
+  if (s >= batch) return;
+  c2 *buf = lds + (threadIdx.x >> 6) * 256;
+  const Frag f = dft16_frag<ORDER_GC>(tw, lane);
+  const c2 *x = in + s * 256;
+  c2 a[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) a[c] = ld_in<INV>(x + 64 * g + 16 * c + j);
+  c2 z[4];
+  core256(a, f, tw, lane, z);
+#pragma unroll
+  for (int r = 0; r < 4; ++r) buf[swz16(4 * g + r + 16 * j)] = z[r];
+  wave_lds_sync();
+  c2 *y = out + s * 256;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) st_out<INV>(y + 64 * c + lane, buf[swz16(64 * c + lane)], scale);
+}
+
+// ---------------------------------------------------------------------------
+// N = 512 = 16 x 32, one signal per wave.  n = 32 n1 + n2, n2 = 16 h + n2'.
+// Stage 1 (two MFMA groups h = 0, 1): DFT16 over n1 -> Y_h[n2'][k1];
+// twiddle W_512^{n2 k1}; radix-2 over h in registers with W_32^{n2' k2a};
+// stage 2 (two groups k2a = 0, 1): DFT16 over n2' with the free transpose.
+// X[k1 + 16 k2a + 32 k2b]: lane l register r of group k2a holds
+// k1 = 4g + r, k2b = j.
+template <bool INV>
+__global__ __launch_bounds__(64 * kWaves) void tensor512_kernel(const c2 *in, c2 *out,
+                                                                int64_t batch,
+                                                                const c2 *__restrict__ tw,
+                                                                double scale) {
+  __shared__ c2 lds[kWaves * 512];
+  const int lane = lane_id(), g = lane >> 4, j = lane & 15;
+  const int64_t s = wave_global();
+  if (s >= batch) return;
+  c2 *buf = lds + (threadIdx.x >> 6) * 512;
+  const Frag f = dft16_frag<ORDER_GC>(tw, lane);
+  const c2 *x = in + s * 512;
+  c2 yh[2][4];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    c2 a[4];
+    // n1 = 4g + c, n2 = 16h + j
+#pragma unroll
+    for (int c = 0; c < 4; ++c) a[c] = ld_in<INV>(x + 32 * (4 * g + c) + 16 * h + j);
+    v4d yr, yi;
+    kgemm(a, f, yr, yi);
+    // reg r: n2' = 4g + r (n2 = 16h + n2'), k1 = j
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int n2 = 16 * h + 4 * g + r;
+      yh[h][r] = cmul(make_double2(yr[r], yi[r]), tw[(n2 * j) << 3]);  // W_512
+    }
+  }
+  // radix-2 over h, then W_32^{n2' k2a}
+  c2 b[2][4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int n2p = 4 * g + r;
+    b[0][r] = cadd(yh[0][r], yh[1][r]);
+    b[1][r] = cmul(csub(yh[0][r], yh[1][r]), tw[n2p << 7]);  // W_32^{n2'}
+  }
+#pragma unroll
+  for (int k2a = 0; k2a < 2; ++k2a) {
+    v4d zr, zi;
+    kgemm(b[k2a], f, zr, zi);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int e = 4 * g + r + 16 * k2a + 32 * j;
+      buf[e ^ ((e >> 5) & 15)] = make_double2(zr[r], zi[r]);
+    }
+  }
+  wave_lds_sync();
+  c2 *y = out + s * 512;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    const int e = 64 * c + lane;
+    st_out<INV>(y + e, buf[e ^ ((e >> 5) & 15)], scale);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// N = 4096 = 16 x 256 (16 x 16 x 16), one signal per 512-thread workgroup.
+// n = 256 n1 + m, k = k1 + 16 k2.
+//  Stage A (each wave: 2 groups of 16 columns m): DFT16 over n1 on MFMA,
+//   twiddle W_4096^{m k1}, write Z[k1][m] to LDS.
+//  Stage B (each wave: 2 rows k1): 256-point core over m -> X[k1 + 16 k2],
+//   results kept in VGPRs until every wave has read Z, then written to LDS
+//   in natural order and streamed out with contiguous 1 KiB stores.
+constexpr int k4096Waves = 8;
+
+__device__ __forceinline__ int swz_z(int k1, int m) { return k1 * 256 + (m ^ k1); }
+__device__ __forceinline__ int swz_x(int e) { return e ^ ((e >> 8) & 15); }
+
+template <bool INV>
+__global__ __launch_bounds__(64 * k4096Waves) void tensor4096_kernel(const c2 *in, c2 *out,
+                                                                     int64_t batch,
+                                                                     const c2 *__restrict__ tw,
+                                                                     double scale) {
+  __shared__ c2 lds[4096];
+  const int lane = lane_id(), g = lane >> 4, j = lane & 15;
+  const int w = threadIdx.x >> 6;
+  const int64_t s = blockIdx.x;
+  if (s >= batch) return;  // uniform over the workgroup
+  const c2 *x = in + s * 4096;
+  const Frag fcg = dft16_frag<ORDER_CG>(tw, lane);
+  const Frag fgc = dft16_frag<ORDER_GC>(tw, lane);
+
+  // ---- stage A
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    const int m0 = 16 * (2 * w + q);
+    c2 a[4];
+    // row i = j -> m = m0 + j; k = n1 = 4c + g
+#pragma unroll
+    for (int c = 0; c < 4; ++c) a[c] = ld_in<INV>(x + 256 * (4 * c + g) + m0 + j);
+    v4d yr, yi;
+    kgemm(a, fcg, yr, yi);
+    // reg r: m = m0 + 4g + r, k1 = j
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int m = m0 + 4 * g + r;
+      lds[swz_z(j, m)] = cmul(make_double2(yr[r], yi[r]), tw[m * j]);
+    }
+  }
+  __syncthreads();
+
+  // ---- stage B: rows k1 = 2w + q
+  c2 z[2][4];
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    const int k1 = 2 * w + q;
+    c2 a[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) a[c] = lds[swz_z(k1, 64 * g + 16 * c + j)];
+    core256(a, fgc, tw, lane, z[q]);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    const int k1 = 2 * w + q;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int k2 = 4 * g + r + 16 * j;
+      lds[swz_x(k1 + 16 * k2)] = z[q][r];
+    }
+  }
+  __syncthreads();
+  c2 *y = out + s * 4096;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    const int e = 512 * c + threadIdx.x;
+    st_out<INV>(y + e, lds[swz_x(e)], scale);
+  }
+}
+
+int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+#define FFTOC_LAUNCH_INV(KERNEL, GRID, BLOCK, ...)                                                 \
+  do {                                                                                             \
+    if ((GRID) > 0x7fffffffLL) return FFTOC_ERROR_INVALID_BATCH;                                   \
+    if (a.inverse)                                                                                 \
+      hipLaunchKernelGGL((KERNEL<__VA_ARGS__ __VA_OPT__(, ) true>), dim3((unsigned)(GRID)),        \
+                         dim3(BLOCK), 0, a.stream, in, out, a.batch, tw, a.scale);                 \
+    else                                                                                           \
+      hipLaunchKernelGGL((KERNEL<__VA_ARGS__ __VA_OPT__(, ) false>), dim3((unsigned)(GRID)),       \
+                         dim3(BLOCK), 0, a.stream, in, out, a.batch, tw, a.scale);                 \
+  } while (0)
+
+}  // namespace
+
+int launch_tensor(int n, const LaunchArgs &a) {
+  if (a.precision != FFTOC_PREC_FP64) return FFTOC_ERROR_UNSUPPORTED;
+  const c2 *in = static_cast<const c2 *>(a.in);
+  c2 *out = static_cast<c2 *>(a.out);
+  const c2 *tw = static_cast<const c2 *>(a.twiddles);
+  const int64_t b = a.batch;
+  switch (n) {
+    case 8: FFTOC_LAUNCH_INV(tensor8_kernel, ceil_div(b, 32 * kWaves), 64 * kWaves); break;
+    case 16: FFTOC_LAUNCH_INV(tensor16_kernel, ceil_div(b, 16 * kWaves), 64 * kWaves); break;
+    case 32: FFTOC_LAUNCH_INV(tensor16xm_kernel, ceil_div(b, 8 * kWaves), 64 * kWaves, 2); break;
+    case 64: FFTOC_LAUNCH_INV(tensor16xm_kernel, ceil_div(b, 4 * kWaves), 64 * kWaves, 4); break;
+    case 128: FFTOC_LAUNCH_INV(tensor16xm_kernel, ceil_div(b, 2 * kWaves), 64 * kWaves, 8); break;
+    case 256: FFTOC_LAUNCH_INV(tensor256_kernel, ceil_div(b, kWaves), 64 * kWaves); break;
+    case 512: FFTOC_LAUNCH_INV(tensor512_kernel, ceil_div(b, kWaves), 64 * kWaves); break;
+    case 4096: FFTOC_LAUNCH_INV(tensor4096_kernel, b, 64 * k4096Waves); break;
+    default: return FFTOC_ERROR_INVALID_SIZE;
+  }
+  return hipGetLastError() == hipSuccess ? FFTOC_SUCCESS : FFTOC_ERROR_LAUNCH;
+}
+
+}  // namespace fftoc

@@ -1,0 +1,123 @@
+/*
+ * fftoc.h -- C ABI of the MI355X (gfx950) batched small complex FFT library
+ * (libfftoc.so).
+ *
+ * The reference (gprabowski/fft_onchip) exposes its FFTs as compile-time
+ * device functors run in place on shared memory by a test kernel:
+ *   template <typename CT, int Size, int UPB, int FPU> struct tensor_fft_N
+ *     { __device__ tensor_fft_N(CT *smem); __device__ void operator()(); }
+ *   (includes/tensor_fft_{8,64,128,256,512,4096}.cuh, launched by
+ *    testing::fft_tester / run_fft_kernel, includes/testing.cuh:16-92),
+ *   template <typename CT, int Size> struct legacy_fft      (radix 8,
+ *    includes/legacy8_fft.cuh:11-120) and legacy16_fft      (radix 16,
+ *    includes/legacy16_fft.cuh:17-173).
+ * A caller binding that path over an FFI would bind one launch per size
+ * family: "run the size-N FFT over `batch` contiguous signals in this device
+ * buffer".  The entry points below are exactly that, with the reference's
+ * data layout (interleaved complex<double>, signal-contiguous, batch-major;
+ * includes/config.hpp:6) and its in-place mode (in == out) kept.
+ *
+ * Conventions for every transform entry point:
+ *   in, out   device pointers (hipMalloc'd or equivalent), 16-byte aligned;
+ *             in == out is allowed (in-place, as the reference runs); other
+ *             overlaps are rejected with FFTOC_ERROR_OVERLAP.
+ *   batch     number of signals; 0 is a no-op, < 0 is an error.
+ *   n         transform length (power of two; per-family limits below).
+ *   flags     OR of the FFTOC_* flag bits below; 0 = forward, FP64, C128.
+ *   stream    a hipStream_t (NULL = the legacy default stream); the call is
+ *             asynchronous on that stream and never synchronises the device.
+ * Errors are returned as fftoc_status codes (the reference's gpuErrchk
+ * printed and exit()ed, includes/common.cuh:3-13); nothing here exits.
+ * Calls are thread-safe; twiddle tables are built once per device (lazily or
+ * by fftoc_init) and are read-only afterwards.
+ */
+#ifndef FFTOC_H
+#define FFTOC_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum fftoc_status {
+  FFTOC_SUCCESS = 0,
+  FFTOC_ERROR_INVALID_SIZE = 1,   /* n not supported by this entry point   */
+  FFTOC_ERROR_INVALID_BATCH = 2,  /* batch < 0                              */
+  FFTOC_ERROR_NULL_POINTER = 3,   /* in or out is NULL with batch > 0       */
+  FFTOC_ERROR_INVALID_FLAGS = 4,  /* unknown or contradictory flag bits     */
+  FFTOC_ERROR_OVERLAP = 5,        /* in and out overlap but are not equal   */
+  FFTOC_ERROR_UNSUPPORTED = 6,    /* valid request this path does not offer */
+  FFTOC_ERROR_DEVICE = 7,         /* HIP runtime error / no gfx950 device   */
+  FFTOC_ERROR_LAUNCH = 8,         /* kernel launch failed                   */
+  FFTOC_ERROR_ALLOC = 9,          /* device or host allocation failed       */
+  FFTOC_ERROR_MISALIGNED = 10,    /* in/out not 16-byte aligned             */
+  FFTOC_ERROR_BUDGET = 11,        /* request exceeds a memory budget        */
+  FFTOC_ERROR_QUEUE_FULL = 12,    /* no free request slot                   */
+  FFTOC_ERROR_INVALID_HANDLE = 13 /* unknown plan / server / request id     */
+} fftoc_status;
+
+/* ---- flag bits ---------------------------------------------------------- */
+#define FFTOC_FORWARD 0x0u        /* exp(-2 pi i nk/N) (reference direction) */
+#define FFTOC_INVERSE 0x1u        /* exp(+2 pi i nk/N), unnormalised         */
+#define FFTOC_NORMALIZE 0x2u      /* multiply the result by 1/N (exact)      */
+/* arithmetic of the DFT stages (tensor path); twiddles stay FP64 */
+#define FFTOC_PREC_FP64 0x00u     /* FP64 everywhere (default)               */
+#define FFTOC_PREC_FP32 0x10u     /* FP32 MFMA stage GEMMs, FP32 accumulate  */
+#define FFTOC_PREC_BF16X3 0x20u   /* 3-term split-BF16 MFMA, FP32 accumulate */
+#define FFTOC_PREC_MASK 0xF0u
+/* signals per workgroup override (0 = library default / autotuned choice) */
+#define FFTOC_TILE_SHIFT 16
+#define FFTOC_TILE_MASK (0xFFu << FFTOC_TILE_SHIFT)
+#define FFTOC_TILE(s) (((uint32_t)(s) & 0xFFu) << FFTOC_TILE_SHIFT)
+
+/* ---- library ------------------------------------------------------------ */
+/* Builds the FP64 twiddle / DFT-matrix tables on the current device.
+ * Optional: every entry point initialises its device on first use. */
+int fftoc_init(void);
+/* Frees the tables of every device.  Not thread-safe against running calls. */
+int fftoc_shutdown(void);
+/* Static description of a status code (never NULL). */
+const char *fftoc_strerror(int status);
+/* Library version as MAJOR*10000 + MINOR*100 + PATCH. */
+int fftoc_version(void);
+
+/* ---- tensor path: FP64 MFMA Karatsuba DFT stages ------------------------ */
+/* Replaces fft::tensor_fft_N<CT,N,UPB,FPU> (includes/tensor_fft_8.cuh:12-66,
+ * tensor_fft_16.cuh:9-63, tensor_fft_64.cuh:12-77, tensor_fft_128.cuh:12-101,
+ * tensor_fft_256.cuh:10-123, tensor_fft_512.cuh:12-122,
+ * tensor_fft_4096.cuh:12-98) as launched by testing::run_fft_kernel
+ * (includes/testing.cuh:59-92).  n in {8,16,32,...,4096}.  The UPB/FPU tile
+ * template parameters become FFTOC_TILE(s) in flags. */
+int fftoc_tensor_fft(const void *in, void *out, int64_t batch, int n,
+                     uint32_t flags, void *stream);
+int fftoc_tensor_fft_8(const void *in, void *out, int64_t batch, uint32_t flags, void *stream);
+int fftoc_tensor_fft_16(const void *in, void *out, int64_t batch, uint32_t flags, void *stream);
+int fftoc_tensor_fft_64(const void *in, void *out, int64_t batch, uint32_t flags, void *stream);
+int fftoc_tensor_fft_128(const void *in, void *out, int64_t batch, uint32_t flags, void *stream);
+int fftoc_tensor_fft_256(const void *in, void *out, int64_t batch, uint32_t flags, void *stream);
+int fftoc_tensor_fft_512(const void *in, void *out, int64_t batch, uint32_t flags, void *stream);
+int fftoc_tensor_fft_4096(const void *in, void *out, int64_t batch, uint32_t flags, void *stream);
+
+/* ---- legacy radix paths: register butterflies on the vector ALUs -------- */
+/* Replaces fft::legacy_fft<CT,N> (radix-8 Stockham, includes/legacy8_fft.cuh:
+ * 11-120): n in {8, 64, 512, 4096}.  FP64 only. */
+int fftoc_legacy8_fft(const void *in, void *out, int64_t batch, int n,
+                      uint32_t flags, void *stream);
+/* Replaces fft::legacy16_fft<CT,N> (radix-16 Stockham,
+ * includes/legacy16_fft.cuh:17-173): n in {16, 256, 4096}.  FP64 only. */
+int fftoc_legacy16_fft(const void *in, void *out, int64_t batch, int n,
+                       uint32_t flags, void *stream);
+
+/* ---- best path ----------------------------------------------------------- */
+/* Any power of two 8..4096; picks the fastest FP64 kernel for n from the
+ * tuning table (the role of the reference's UPB/FPU sweep,
+ * scripts/generate_bench.py:93-107). */
+int fftoc_fft(const void *in, void *out, int64_t batch, int n, uint32_t flags,
+              void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FFTOC_H */

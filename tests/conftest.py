@@ -9,3 +9,14 @@ if ROOT not in sys.path:
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (gfx950) GPU")
     config.addinivalue_line("markers", "slow: long-running CPU check")
+
+
+import numpy as _np
+import pytest as _pytest
+
+
+@_pytest.fixture(scope="session")
+def golden():
+    d = dict(_np.load(os.path.join(ROOT, "tests", "golden", "golden_fft.npz")))
+    d["sizes"] = {int(k[2:]) for k in d if k.startswith("x_")}
+    return d

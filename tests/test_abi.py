@@ -1,0 +1,79 @@
+"""CPU-side checks of the C ABI: the library loads, exports every symbol the
+header declares, and rejects bad arguments before touching a device."""
+import ctypes
+import os
+import re
+
+import pytest
+
+import fft_onchip_amd as fo
+from fft_onchip_amd import _native
+
+HEADER = _native.HEADER
+
+
+def declared_symbols():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(fftoc_\w+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(_native.LIB_PATH)
+    names = declared_symbols()
+    assert len(names) == len(_native.SIGNATURES)
+    for name in names:
+        assert hasattr(lib, name), name
+    assert set(names) <= set(_native.SIGNATURES), set(names) - set(_native.SIGNATURES)
+
+
+def test_strerror_covers_every_status():
+    for code, name in fo.STATUS.items():
+        s = fo.lib().fftoc_strerror(code)
+        assert s and s != b"unknown status", name
+    assert fo.lib().fftoc_strerror(999) == b"unknown status"
+
+
+def test_status_codes_match_header():
+    text = open(HEADER).read()
+    for name, code in re.findall(r"FFTOC_(?:ERROR_)?(\w+) = (\d+)", text):
+        if name in ("SUCCESS",) or name in fo.STATUS.values():
+            assert fo.STATUS[int(code)] == name
+
+
+def _rc(fn, *args):
+    return getattr(fo.lib(), fn)(*args)
+
+
+FAKE_IN = 0x10000
+FAKE_OUT = 0x20000000
+
+
+@pytest.mark.parametrize("fn,n", [("fftoc_fft", 4), ("fftoc_fft", 8192), ("fftoc_fft", 100),
+                                  ("fftoc_legacy8_fft", 16), ("fftoc_legacy8_fft", 128),
+                                  ("fftoc_legacy16_fft", 64), ("fftoc_legacy16_fft", 512),
+                                  ("fftoc_tensor_fft", 1024), ("fftoc_tensor_fft", 12)])
+def test_invalid_sizes(fn, n):
+    assert _rc(fn, FAKE_IN, FAKE_OUT, 1, n, 0, None) == 1
+
+
+def test_argument_errors_need_no_device():
+    lib = fo.lib()
+    assert lib.fftoc_fft(FAKE_IN, FAKE_OUT, -1, 64, 0, None) == 2          # batch < 0
+    assert lib.fftoc_fft(None, FAKE_OUT, 1, 64, 0, None) == 3              # null
+    assert lib.fftoc_fft(FAKE_IN, None, 1, 64, 0, None) == 3
+    assert lib.fftoc_fft(FAKE_IN, FAKE_OUT, 1, 64, 0x8, None) == 4         # unknown flag bit
+    assert lib.fftoc_fft(FAKE_IN, FAKE_OUT, 1, 64, 0x30, None) == 4        # unknown precision
+    assert lib.fftoc_fft(FAKE_IN, FAKE_IN + 16, 4, 64, 0, None) == 5       # partial overlap
+    assert lib.fftoc_fft(FAKE_IN + 8, FAKE_OUT, 1, 64, 0, None) == 10      # misaligned
+    assert lib.fftoc_legacy8_fft(FAKE_IN, FAKE_OUT, 1, 64, 0x10, None) == 6  # fp32 on legacy
+    assert lib.fftoc_tensor_fft_64(None, None, 0, 0, None) == 0            # empty batch: no-op
+    assert lib.fftoc_fft(None, None, 0, 4096, 0, None) == 0
+
+
+def test_python_mirror_raises():
+    with pytest.raises(fo.FftocError) as e:
+        fo.legacy16_fft(FAKE_IN, 1, 64, out=FAKE_OUT)
+    assert e.value.name == "INVALID_SIZE"
+    for n in fo.TENSOR_SIZES:
+        assert callable(getattr(fo, f"tensor_fft_{n}"))
