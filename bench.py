@@ -1,0 +1,269 @@
+#!/usr/bin/env python3
+"""Benchmark of the batched small complex FFT on one MI355X.
+
+Prints ONE JSON line.  The headline workload is the FP64 forward transform at
+N = 4096 over 16384 signals (1 GiB of complex128 in, 1 GiB out), inputs
+N(0,1) re/im from Philox seed 7, run by the library's best FP64 path
+(``fftoc_fft``).  A step is one launch over the whole batch; inputs are
+resident in HBM before the timed region starts.  ``extra`` carries the same
+measurement for the other configurations (N = 64 x 65536, the N = 8..512
+sweep at 256 MiB per run, and the MFMA / radix paths side by side).
+
+Roofline: algorithmic bytes per launch = 32 * N * batch (16 B read + 16 B
+written per point) over the average launch duration from HIP events recorded
+on the launch stream around every timed launch; peak = 8 TB/s (MI355X HBM3E).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import statistics
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBPS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+MALL_BYTES = 256 << 20
+
+
+def gen_signals(n: int, batch: int, seed: int, kind: str) -> np.ndarray:
+    g = np.random.Generator(np.random.Philox(seed))
+    x = np.empty((batch, n), dtype=np.complex128)
+    v = x.view(np.float64)
+    if kind == "normal":
+        v[...] = g.standard_normal(v.shape)
+    else:
+        v[...] = g.uniform(-1.0, 1.0, v.shape)
+    return x
+
+
+class Workload:
+    def __init__(self, name, n, batch, seed, kind, path):
+        self.name, self.n, self.batch, self.seed, self.kind, self.path = name, n, batch, seed, kind, path
+
+
+def run_gpu(fo, wl: Workload, steps: int, warmup: int, host_x: np.ndarray | None = None):
+    """Times `steps` launches of one path over the resident batch."""
+    from fft_onchip_amd import hip
+
+    n, batch = wl.n, wl.batch
+    nbytes = n * batch * 16
+    x = gen_signals(n, batch, wl.seed, wl.kind) if host_x is None else host_x
+    # enough in/out pairs that the rotated footprint is >= 4x the 256 MB MALL
+    pairs = max(1, min(8, math.ceil(4 * MALL_BYTES / (2 * nbytes))))
+    ins = [hip.DeviceBuffer.from_array(x) for _ in range(pairs)]
+    outs = [hip.DeviceBuffer(nbytes) for _ in range(pairs)]
+    stream = hip.Stream()
+    fn = {"best": fo.fft, "tensor": fo.tensor_fft, "legacy8": fo.legacy_fft,
+          "legacy16": fo.legacy16_fft}[wl.path]
+    for i in range(warmup):
+        fn(ins[i % pairs], batch, n, out=outs[i % pairs], stream=stream)
+    stream.synchronize()
+    ev = [(hip.Event(), hip.Event()) for _ in range(steps)]
+    t_start, t_end = hip.Event(), hip.Event()
+    hip.synchronize()
+    wall0 = time.perf_counter()
+    t_start.record(stream)
+    for i in range(steps):
+        ev[i][0].record(stream)
+        fn(ins[i % pairs], batch, n, out=outs[i % pairs], stream=stream)
+        ev[i][1].record(stream)
+    t_end.record(stream)
+    stream.synchronize()
+    hip.synchronize()
+    wall = time.perf_counter() - wall0
+    total_ms = t_start.elapsed_ms(t_end)
+    launch_ms = [a.elapsed_ms(b) for a, b in ev]
+    result = outs[(steps - 1) % pairs]
+    sample = result.download((min(batch, 256), n))
+    for b in ins + outs:
+        b.free()
+    mean_launch_ms = statistics.fmean(launch_ms)
+    ffts_per_s = batch * steps / (total_ms * 1e-3)
+    alg_bytes = 32.0 * n * batch
+    achieved = alg_bytes / (mean_launch_ms * 1e-3) / 1e9
+    flops = 5.0 * n * math.log2(n) * batch
+    return {
+        "workload": wl.name, "n": n, "batch": batch, "path": wl.path,
+        "ffts_per_s": ffts_per_s, "ms_per_step": total_ms / steps,
+        "launch_ms_mean": mean_launch_ms, "launch_ms_median": statistics.median(launch_ms),
+        "launch_ms_min": min(launch_ms), "wall_s": wall,
+        "GBps": achieved, "hbm_frac": achieved / HBM_PEAK_GBPS,
+        "GFLOPps": flops / (mean_launch_ms * 1e-3) / 1e9,
+        "rotation_pairs": pairs,
+    }, x, sample
+
+
+def copy_bandwidth(nbytes: int = 1 << 30, reps: int = 10) -> float:
+    """Device-to-device hipMemcpy of `nbytes` (read + write) in GB/s."""
+    import ctypes
+
+    from fft_onchip_amd import hip
+    src, dst = hip.DeviceBuffer(nbytes), hip.DeviceBuffer(nbytes)
+    stream = hip.Stream()
+    h = hip.lib()
+    for _ in range(2):
+        hip.check(h.hipMemcpyAsync(ctypes.c_void_p(dst.ptr), ctypes.c_void_p(src.ptr), nbytes, 3,
+                                   ctypes.c_void_p(stream.handle)))
+    a, b = hip.Event(), hip.Event()
+    a.record(stream)
+    for _ in range(reps):
+        hip.check(h.hipMemcpyAsync(ctypes.c_void_p(dst.ptr), ctypes.c_void_p(src.ptr), nbytes, 3,
+                                   ctypes.c_void_p(stream.handle)))
+    b.record(stream)
+    stream.synchronize()
+    ms = a.elapsed_ms(b) / reps
+    src.free()
+    dst.free()
+    return 2.0 * nbytes / (ms * 1e-3) / 1e9
+
+
+def cpu_baseline(x: np.ndarray, gpu_sample: np.ndarray, threads: int, budget_s: float = 12.0):
+    """FP64 FFT of the same signals on the host cores (the oracle's Stockham
+    transform, OpenMP over the batch); median of >= 5 timed runs."""
+    import oracle  # the CPU leg: timed baseline and checker of the GPU sample
+
+    batch, n = x.shape
+    # bounded sample: as many signals as fit the time budget over >= 5 runs
+    probe = x[: min(batch, 256)]
+    t0 = time.perf_counter()
+    oracle.fft_f64(probe, threads=threads)
+    per_sig = max((time.perf_counter() - t0) / probe.shape[0], 1e-9)
+    count = int(min(batch, max(256, budget_s / 6.0 / per_sig)))
+    sample = np.ascontiguousarray(x[:count])
+    times = []
+    for _ in range(6):
+        t0 = time.perf_counter()
+        out = oracle.fft_f64(sample, threads=threads)
+        times.append(time.perf_counter() - t0)
+    med = statistics.median(times[1:])
+    m = gpu_sample.shape[0]
+    ref = oracle.reference(x[:m])
+    err = float(np.max(oracle.rel_l2(gpu_sample, ref, axis=-1)))
+    cpu_err = float(np.max(oracle.rel_l2(out[:m], ref, axis=-1)))
+    return {
+        "value": count / med, "unit": "FFTs/s", "cores": threads, "kind": "port",
+        "sample": f"first {count} of the {batch} N={n} signals, median of 5 timed runs after 1 warm-up",
+        "GFLOPps": 5.0 * n * math.log2(n) * count / med / 1e9,
+        "cpu_model": _cpu_model(), "cpu_max_rel_l2_vs_oracle": cpu_err,
+        "gpu_max_rel_l2_vs_oracle": err, "gpu_checked_signals": m,
+    }
+
+
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def load_traffic(name: str):
+    """HBM bytes per launch from the committed rocprofv3 --pmc summaries."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            return json.load(f).get(name)
+    except (OSError, ValueError):
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--quick", action="store_true", help="headline workload only")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build measures one GPU; the batched FFT shards by contiguous "
+              "signal ranges with no exchange, so N>1 is not run here.", file=sys.stderr)
+        sys.exit(2)
+
+    import fft_onchip_amd as fo
+    from fft_onchip_amd import hip
+    if hip.device_count() < 1:
+        print("bench.py: no HIP device visible", file=sys.stderr)
+        sys.exit(1)
+    fo.init()
+
+    head = Workload("fp64_c2c_fwd_n4096_b16384", 4096, 16384, 7, "normal", "best")
+    res, x, sample = run_gpu(fo, head, args.steps, args.warmup)
+    print(f"[bench] {head.name}: {res['ffts_per_s']:.4g} FFT/s, {res['GBps']:.0f} GB/s", file=sys.stderr, flush=True)
+
+    extra = {}
+    if not args.quick:
+        others = [Workload("fp64_n4096_b16384_tensor", 4096, 16384, 7, "normal", "tensor"),
+                  Workload("fp64_n4096_b16384_legacy16", 4096, 16384, 7, "normal", "legacy16"),
+                  Workload("fp64_n4096_b16384_legacy8", 4096, 16384, 7, "normal", "legacy8")]
+        for path in ("best", "tensor", "legacy8"):
+            others.append(Workload(f"fp64_n64_b65536_{path}", 64, 65536, 0, "uniform", path))
+        for i, n in enumerate((8, 16, 128, 256, 512)):
+            for path in ("best", "tensor"):
+                others.append(Workload(f"fp64_n{n}_b{(1 << 24) // n}_{path}", n, (1 << 24) // n,
+                                       1 + i, "uniform", path))
+            if n in (8, 512):
+                others.append(Workload(f"fp64_n{n}_b{(1 << 24) // n}_legacy8", n, (1 << 24) // n,
+                                       1 + i, "uniform", "legacy8"))
+            if n in (16, 256):
+                others.append(Workload(f"fp64_n{n}_b{(1 << 24) // n}_legacy16", n, (1 << 24) // n,
+                                       1 + i, "uniform", "legacy16"))
+        cache = {}
+        for wl in others:
+            key = (wl.n, wl.batch, wl.seed, wl.kind)
+            if key not in cache:
+                cache.clear()
+                cache[key] = x if key == (head.n, head.batch, head.seed, head.kind) else \
+                    gen_signals(wl.n, wl.batch, wl.seed, wl.kind)
+            r, _, _ = run_gpu(fo, wl, args.steps, args.warmup, host_x=cache[key])
+            extra[wl.name] = {k: (round(v, 6) if isinstance(v, float) else v) for k, v in r.items()}
+            print(f"[bench] {wl.name}: {r['ffts_per_s']:.4g} FFT/s, {r['GBps']:.0f} GB/s", file=sys.stderr, flush=True)
+        extra["hip_d2d_copy_GBps"] = round(copy_bandwidth(), 1)
+
+    cpu = None
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        cpu = cpu_baseline(x, sample, threads)
+        print(f"[bench] cpu baseline: {cpu['value']:.4g} FFT/s on {threads} threads", file=sys.stderr, flush=True)
+
+    roof = {"bound": "hbm", "achieved": round(res["GBps"], 2), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+            "frac": round(res["hbm_frac"], 4), "traffic": load_traffic(head.name),
+            "kernel": "fftoc_fft N=4096 (radix-16 Stockham, 3 passes)",
+            "algorithmic_bytes_per_launch": 32 * head.n * head.batch,
+            "launch_ms_mean": round(res["launch_ms_mean"], 5)}
+    line = {
+        "metric": "FFTs/s (FP64 batched c2c forward FFT, N=4096)",
+        "value": round(res["ffts_per_s"], 1),
+        "unit": "FFTs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(res["ms_per_step"], 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (Philox seed 7, N(0,1) re/im), resident in HBM",
+        "config": {"workload": head.name, "n": head.n, "batch": head.batch, "path": "fftoc_fft",
+                   "layout": "interleaved complex128, signal-contiguous", "direction": "forward",
+                   "GFLOPps": round(res["GFLOPps"], 1)},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "extra": extra,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -6,18 +6,20 @@
 // DFT -> twiddle -> DFT structure of includes/c64_fft64.cuh:8-27.
 //
 // CDNA4 operand layout of v_mfma_f64_16x16x4_f64 (wave64, lane l, g = l/16,
-// j = l%16):  A[i][k]: lane l holds A[j][g];  B[k][col]: lane l holds B[g][j];
-// D[i][col]: lane l register r holds D[4g + r][j].
+// j = l%16), measured on MI355X:  A[i][k]: lane l holds A[j][g];
+// B[k][col]: lane l holds B[g][j];  D[i][col]: lane l register r holds
+// D[4r + g][j].
 //
 // Design points:
 //  * The signal goes into the A operand (rows = independent sub-transforms,
-//    K = the 16 inputs of one DFT16, 4 K-chunks), the DFT matrix into B.
+//    K = the 16 inputs of one DFT16 in 4 K-chunks, k = 4c + g), the DFT
+//    matrix into B.
 //  * For two-stage transforms (256 = 16 x 16 and the sizes built on it) the
 //    four D registers of stage 1 ARE the A operands of the four K-chunks of
-//    stage 2 once the K order of stage 1 is taken as n1 = 4g + c: the data
+//    stage 2 (register r = chunk r, row 4r + g = K index 4r + g): the data
 //    transpose between the stages costs no data movement (the role of the
 //    fragment reuse in includes/c64_fft64.cuh:24-26).
-//  * Both orders use the same per-lane B fragment (F16 is symmetric), so a
+//  * Both stages use the same per-lane B fragment (F16 is symmetric), so a
 //    wave keeps 12 doubles of DFT matrix (Br, Bi, Br+Bi per K-chunk).
 //  * Twiddles and DFT-matrix entries come from the FP64 table W_4096^j built
 //    on the host; only additions are done in-kernel (Br + Bi).
@@ -35,22 +37,18 @@ __device__ __forceinline__ v4d mfma16(double a, double b, v4d c) {
 
 __device__ __forceinline__ v4d vzero() { return v4d{0.0, 0.0, 0.0, 0.0}; }
 
-// B fragment of a 16x16 DFT-type matrix M[k][col] = W_16^{k*col} in the
-// Karatsuba form.  K order: chunk c, slot g -> k = 4c + g (ORDER_CG) or
-// k = 4g + c (ORDER_GC).
+// B fragment of the 16x16 DFT matrix M[k][col] = W_16^{k*col} in the
+// Karatsuba form; K-chunk c, slot g -> k = 4c + g.
 struct Frag {
   double r[4], i[4], s[4];
 };
 
-enum KOrder { ORDER_CG = 0, ORDER_GC = 1 };
-
-template <KOrder ORD>
 __device__ __forceinline__ Frag dft16_frag(const c2 *__restrict__ tw, int lane) {
   const int g = lane >> 4, j = lane & 15;
   Frag f;
 #pragma unroll
   for (int c = 0; c < 4; ++c) {
-    const int k = (ORD == ORDER_CG) ? 4 * c + g : 4 * g + c;
+    const int k = 4 * c + g;
     const c2 w = tw[((k * j) & 15) << 8];  // W_16^{kj} = W_4096^{256 kj}
     f.r[c] = w.x;
     f.i[c] = w.y;
@@ -137,7 +135,7 @@ __global__ __launch_bounds__(64 * kWaves) void tensor8_kernel(const c2 *in, c2 *
   kgemm(a, f, yr, yi);
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
-    const int64_t s = s0 + (j < 8 ? 0 : 16) + 4 * g + r;
+    const int64_t s = s0 + (j < 8 ? 0 : 16) + 4 * r + g;
     if (s < batch) st_out<INV>(out + s * 8 + (j & 7), make_double2(yr[r], yi[r]), scale);
   }
 }
@@ -151,7 +149,7 @@ __global__ __launch_bounds__(64 * kWaves) void tensor16_kernel(const c2 *in, c2 
   const int lane = lane_id(), g = lane >> 4, j = lane & 15;
   const int64_t s0 = wave_global() * 16;
   if (s0 >= batch) return;
-  const Frag f = dft16_frag<ORDER_CG>(tw, lane);
+  const Frag f = dft16_frag(tw, lane);
   c2 a[4];
   const int64_t sa = s0 + j;
 #pragma unroll
@@ -161,17 +159,20 @@ __global__ __launch_bounds__(64 * kWaves) void tensor16_kernel(const c2 *in, c2 
   kgemm(a, f, yr, yi);
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
-    const int64_t s = s0 + 4 * g + r;
+    const int64_t s = s0 + 4 * r + g;
     if (s < batch) st_out<INV>(out + s * 16 + j, make_double2(yr[r], yi[r]), scale);
   }
 }
 
 // ---------------------------------------------------------------------------
 // N = 16 * M with M in {2, 4, 8}: n = M n1 + n2.  Rows of the MFMA are
-// (signal, n2) pairs, 16/M signals per wave.  Stage 1: DFT16 over n1 on
-// MFMA; then twiddle W_N^{n2 k1} and a DFT_M over n2 on the VALU.  For M = 2
-// and M = 4 a lane's four D registers hold every n2 of its (signal, k1); for
-// M = 8 the other half sits in lane l ^ 16 and is fetched with one exchange.
+// (signal, n2) pairs, SPW = 16/M signals per wave, row i <-> (signal i % SPW,
+// n2 = i / SPW).  Stage 1: DFT16 over n1 on MFMA; then twiddle W_N^{n2 k1}
+// and a DFT_M over n2 on the VALU.  D register r of lane l is row 4r + g:
+//   M = 2: signal (4r + g) % 8, n2 = r / 2  -> both n2 of two signals in-lane
+//   M = 4: signal g, n2 = r                 -> all four n2 in-lane
+//   M = 8: signal g % 2, n2 = 2r + g / 2    -> even n2 here, odd n2 in lane
+//          l ^ 32; DFT4 in-lane, then one radix-2 exchange across the pair.
 template <int M, bool INV>
 __global__ __launch_bounds__(64 * kWaves) void tensor16xm_kernel(const c2 *in, c2 *out,
                                                                  int64_t batch,
@@ -182,33 +183,31 @@ __global__ __launch_bounds__(64 * kWaves) void tensor16xm_kernel(const c2 *in, c
   const int lane = lane_id(), g = lane >> 4, j = lane & 15;
   const int64_t s0 = wave_global() * SPW;
   if (s0 >= batch) return;
-  const Frag f = dft16_frag<ORDER_CG>(tw, lane);
-  // A operand: row i = j -> (signal j / M, n2 = j % M); k = n1 = 4c + g
+  const Frag f = dft16_frag(tw, lane);
   c2 a[4];
   {
-    const int64_t s = s0 + j / M;
-    const int n2 = j % M;
+    const int64_t s = s0 + j % SPW;
+    const int n2 = j / SPW;
 #pragma unroll
     for (int c = 0; c < 4; ++c)
       a[c] = (s < batch) ? ld_in<INV>(in + s * N + M * (4 * c + g) + n2) : make_double2(0.0, 0.0);
   }
   v4d yr, yi;
   kgemm(a, f, yr, yi);
-  // D: register r of lane l holds row i = 4g + r -> (signal i / M, n2 = i % M), k1 = j
   c2 y[4];
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
-    const int n2 = (4 * g + r) % M;
+    const int n2 = (4 * r + g) / SPW;
     c2 v = make_double2(yr[r], yi[r]);
     if (n2 != 0) v = cmul(v, tw[(n2 * j) * (kTwN / N)]);
     y[r] = v;
   }
   if constexpr (M == 2) {
-    // registers (0,1) -> signal 2g, (2,3) -> signal 2g+1; DFT2 over n2
+    // registers (0, 2) -> signal g, (1, 3) -> signal 4 + g; DFT2 over n2
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
-      const int64_t s = s0 + 2 * g + h;
-      c2 u0 = y[2 * h], u1 = y[2 * h + 1];
+      const int64_t s = s0 + 4 * h + g;
+      c2 u0 = y[h], u1 = y[h + 2];
       dft2(u0, u1);
       if (s < batch) {
         st_out<INV>(out + s * N + j, u0, scale);
@@ -216,7 +215,6 @@ __global__ __launch_bounds__(64 * kWaves) void tensor16xm_kernel(const c2 *in, c
       }
     }
   } else if constexpr (M == 4) {
-    // all four n2 of signal g in this lane: DFT4 over r
     dft4(y[0], y[1], y[2], y[3]);
     const int64_t s = s0 + g;
     if (s < batch) {
@@ -225,53 +223,48 @@ __global__ __launch_bounds__(64 * kWaves) void tensor16xm_kernel(const c2 *in, c
     }
   } else {
     static_assert(M == 8, "M must be 2, 4 or 8");
-    // lane holds n2 = 4h + r (h = g & 1) of signal g >> 1; partner lane l^16
-    // holds the other h.  DFT8 over n2 = 4h + r with k2 = k2a + 2 k2b:
-    // radix-2 over h (lane pair), twiddle W8^{r k2a}, DFT4 over r.
-    const int h = g & 1;
+    // DIT over n2 = 2m + h (h = g / 2, m = r): E/O = DFT4 over m of the
+    // even / odd n2, X[k] = E[k] + W8^k O[k], X[k + 4] = E[k] - W8^k O[k].
+    const int h = g >> 1;
+    dft4(y[0], y[1], y[2], y[3]);
+    if (h == 1) {
+      y[1] = mul_w8_1(y[1]);
+      y[2] = mul_mi(y[2]);
+      y[3] = mul_w8_3(y[3]);
+    }
     c2 p[4];
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      p[r].x = __shfl_xor(y[r].x, 16);
-      p[r].y = __shfl_xor(y[r].y, 16);
+      p[r].x = __shfl_xor(y[r].x, 32);
+      p[r].y = __shfl_xor(y[r].y, 32);
     }
-    // lane with h = 0 computes k2a = 0 (sum), h = 1 computes k2a = 1 (difference)
-    c2 u[4];
-#pragma unroll
-    for (int r = 0; r < 4; ++r) u[r] = (h == 0) ? cadd(y[r], p[r]) : csub(p[r], y[r]);
-    if (h == 1) {
-      u[1] = mul_w8_1(u[1]);
-      u[2] = mul_mi(u[2]);
-      u[3] = mul_w8_3(u[3]);
-    }
-    dft4(u[0], u[1], u[2], u[3]);
-    const int64_t s = s0 + (g >> 1);
+    const int64_t s = s0 + (g & 1);
     if (s < batch) {
 #pragma unroll
-      for (int k2b = 0; k2b < 4; ++k2b)
-        st_out<INV>(out + s * N + j + 16 * (h + 2 * k2b), u[k2b], scale);
+      for (int k = 0; k < 4; ++k) {
+        const c2 v = (h == 0) ? cadd(y[k], p[k]) : csub(p[k], y[k]);
+        st_out<INV>(out + s * N + j + 16 * (4 * h + k), v, scale);
+      }
     }
   }
 }
 
 // ---------------------------------------------------------------------------
-// 256-point core on one wave: input a[c] = x[64g + 16c + j] (ORDER_GC,
-// n = 16 n1 + n2 with n1 = 4g + c, n2 = j); output z[r] = X[4g + r + 16 j].
-// tw256 = W_256 step in the 4096 table (16), used for the inter-stage
-// twiddle W_256^{n2 k1}.
+// 256-point core on one wave: input a[c] = x[64c + l] (n = 16 n1 + n2 with
+// n1 = 4c + g, n2 = j); output z[r] = X[4r + g + 16 j].
 __device__ __forceinline__ void core256(c2 (&a)[4], const Frag &f, const c2 *__restrict__ tw,
                                         int lane, c2 (&z)[4]) {
   const int g = lane >> 4, j = lane & 15;
   v4d yr, yi;
   kgemm(a, f, yr, yi);
-  // D: reg r = Y[n2 = 4g + r][k1 = j]; twiddle W_256^{n2 k1}
+  // D: reg r = Y[n2 = 4r + g][k1 = j]; twiddle W_256^{n2 k1}
   c2 b[4];
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
-    const int n2 = 4 * g + r;
+    const int n2 = 4 * r + g;
     b[r] = cmul(make_double2(yr[r], yi[r]), tw[(n2 * j) << 4]);
   }
-  // stage 2: A chunk r = b[r] (rows = k1 = j, K = n2 = 4g + r)
+  // stage 2: A chunk r = b[r] (rows = k1 = j, K = n2 = 4r + g)
   kgemm(b, f, yr, yi);
 #pragma unroll
   for (int r = 0; r < 4; ++r) z[r] = make_double2(yr[r], yi[r]);
@@ -293,15 +286,15 @@ __global__ __launch_bounds__(64 * kWaves) void tensor256_kernel(const c2 *in, c2
   const int64_t s = wave_global();
   if (s >= batch) return;
   c2 *buf = lds + (threadIdx.x >> 6) * 256;
-  const Frag f = dft16_frag<ORDER_GC>(tw, lane);
+  const Frag f = dft16_frag(tw, lane);
   const c2 *x = in + s * 256;
   c2 a[4];
 #pragma unroll
-  for (int c = 0; c < 4; ++c) a[c] = ld_in<INV>(x + 64 * g + 16 * c + j);
+  for (int c = 0; c < 4; ++c) a[c] = ld_in<INV>(x + 64 * c + lane);
   c2 z[4];
   core256(a, f, tw, lane, z);
 #pragma unroll
-  for (int r = 0; r < 4; ++r) buf[swz16(4 * g + r + 16 * j)] = z[r];
+  for (int r = 0; r < 4; ++r) buf[swz16(4 * r + g + 16 * j)] = z[r];
   wave_lds_sync();
   c2 *y = out + s * 256;
 #pragma unroll
@@ -314,7 +307,7 @@ __global__ __launch_bounds__(64 * kWaves) void tensor256_kernel(const c2 *in, c2
 // twiddle W_512^{n2 k1}; radix-2 over h in registers with W_32^{n2' k2a};
 // stage 2 (two groups k2a = 0, 1): DFT16 over n2' with the free transpose.
 // X[k1 + 16 k2a + 32 k2b]: lane l register r of group k2a holds
-// k1 = 4g + r, k2b = j.
+// k1 = 4r + g, k2b = j.
 template <bool INV>
 __global__ __launch_bounds__(64 * kWaves) void tensor512_kernel(const c2 *in, c2 *out,
                                                                 int64_t batch,
@@ -325,21 +318,21 @@ __global__ __launch_bounds__(64 * kWaves) void tensor512_kernel(const c2 *in, c2
   const int64_t s = wave_global();
   if (s >= batch) return;
   c2 *buf = lds + (threadIdx.x >> 6) * 512;
-  const Frag f = dft16_frag<ORDER_GC>(tw, lane);
+  const Frag f = dft16_frag(tw, lane);
   const c2 *x = in + s * 512;
   c2 yh[2][4];
 #pragma unroll
   for (int h = 0; h < 2; ++h) {
     c2 a[4];
-    // n1 = 4g + c, n2 = 16h + j
+    // n1 = 4c + g, n2 = 16h + j
 #pragma unroll
-    for (int c = 0; c < 4; ++c) a[c] = ld_in<INV>(x + 32 * (4 * g + c) + 16 * h + j);
+    for (int c = 0; c < 4; ++c) a[c] = ld_in<INV>(x + 32 * (4 * c + g) + 16 * h + j);
     v4d yr, yi;
     kgemm(a, f, yr, yi);
-    // reg r: n2' = 4g + r (n2 = 16h + n2'), k1 = j
+    // reg r: n2' = 4r + g (n2 = 16h + n2'), k1 = j
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const int n2 = 16 * h + 4 * g + r;
+      const int n2 = 16 * h + 4 * r + g;
       yh[h][r] = cmul(make_double2(yr[r], yi[r]), tw[(n2 * j) << 3]);  // W_512
     }
   }
@@ -347,7 +340,7 @@ __global__ __launch_bounds__(64 * kWaves) void tensor512_kernel(const c2 *in, c2
   c2 b[2][4];
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
-    const int n2p = 4 * g + r;
+    const int n2p = 4 * r + g;
     b[0][r] = cadd(yh[0][r], yh[1][r]);
     b[1][r] = cmul(csub(yh[0][r], yh[1][r]), tw[n2p << 7]);  // W_32^{n2'}
   }
@@ -357,7 +350,7 @@ __global__ __launch_bounds__(64 * kWaves) void tensor512_kernel(const c2 *in, c2
     kgemm(b[k2a], f, zr, zi);
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const int e = 4 * g + r + 16 * k2a + 32 * j;
+      const int e = 4 * r + g + 16 * k2a + 32 * j;
       buf[e ^ ((e >> 5) & 15)] = make_double2(zr[r], zi[r]);
     }
   }
@@ -394,8 +387,7 @@ __global__ __launch_bounds__(64 * k4096Waves) void tensor4096_kernel(const c2 *i
   const int64_t s = blockIdx.x;
   if (s >= batch) return;  // uniform over the workgroup
   const c2 *x = in + s * 4096;
-  const Frag fcg = dft16_frag<ORDER_CG>(tw, lane);
-  const Frag fgc = dft16_frag<ORDER_GC>(tw, lane);
+  const Frag f = dft16_frag(tw, lane);
 
   // ---- stage A
 #pragma unroll
@@ -406,11 +398,11 @@ __global__ __launch_bounds__(64 * k4096Waves) void tensor4096_kernel(const c2 *i
 #pragma unroll
     for (int c = 0; c < 4; ++c) a[c] = ld_in<INV>(x + 256 * (4 * c + g) + m0 + j);
     v4d yr, yi;
-    kgemm(a, fcg, yr, yi);
-    // reg r: m = m0 + 4g + r, k1 = j
+    kgemm(a, f, yr, yi);
+    // reg r: m = m0 + 4r + g, k1 = j
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const int m = m0 + 4 * g + r;
+      const int m = m0 + 4 * r + g;
       lds[swz_z(j, m)] = cmul(make_double2(yr[r], yi[r]), tw[m * j]);
     }
   }
@@ -423,8 +415,8 @@ __global__ __launch_bounds__(64 * k4096Waves) void tensor4096_kernel(const c2 *i
     const int k1 = 2 * w + q;
     c2 a[4];
 #pragma unroll
-    for (int c = 0; c < 4; ++c) a[c] = lds[swz_z(k1, 64 * g + 16 * c + j)];
-    core256(a, fgc, tw, lane, z[q]);
+    for (int c = 0; c < 4; ++c) a[c] = lds[swz_z(k1, 64 * c + lane)];
+    core256(a, f, tw, lane, z[q]);
   }
   __syncthreads();
 #pragma unroll
@@ -432,7 +424,7 @@ __global__ __launch_bounds__(64 * k4096Waves) void tensor4096_kernel(const c2 *i
     const int k1 = 2 * w + q;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const int k2 = 4 * g + r + 16 * j;
+      const int k2 = 4 * r + g + 16 * j;
       lds[swz_x(k1 + 16 * k2)] = z[q][r];
     }
   }
