@@ -127,30 +127,31 @@ def copy_bandwidth(nbytes: int = 1 << 30, reps: int = 10) -> float:
 
 def cpu_baseline(x: np.ndarray, gpu_sample: np.ndarray, threads: int, budget_s: float = 12.0):
     """FP64 FFT of the same signals on the host cores (the oracle's Stockham
-    transform, OpenMP over the batch); median of >= 5 timed runs."""
+    transform, OpenMP over the batch); median of 5 timed runs."""
     import oracle  # the CPU leg: timed baseline and checker of the GPU sample
 
     batch, n = x.shape
-    # bounded sample: as many signals as fit the time budget over >= 5 runs
-    probe = x[: min(batch, 256)]
+    out = oracle.fft_f64(x, threads=threads)          # warm-up over the whole batch
     t0 = time.perf_counter()
-    oracle.fft_f64(probe, threads=threads)
-    per_sig = max((time.perf_counter() - t0) / probe.shape[0], 1e-9)
-    count = int(min(batch, max(256, budget_s / 6.0 / per_sig)))
-    sample = np.ascontiguousarray(x[:count])
+    oracle.fft_f64(x, threads=threads)
+    one = max(time.perf_counter() - t0, 1e-6)
+    reps = max(1, int(budget_s / 5.0 / one))           # each timed run ~budget/5 s
     times = []
-    for _ in range(6):
+    for _ in range(5):
         t0 = time.perf_counter()
-        out = oracle.fft_f64(sample, threads=threads)
+        for _ in range(reps):
+            oracle.fft_f64(x, threads=threads)
         times.append(time.perf_counter() - t0)
-    med = statistics.median(times[1:])
+    med = statistics.median(times)
     m = gpu_sample.shape[0]
     ref = oracle.reference(x[:m])
     err = float(np.max(oracle.rel_l2(gpu_sample, ref, axis=-1)))
     cpu_err = float(np.max(oracle.rel_l2(out[:m], ref, axis=-1)))
+    count = batch * reps
     return {
         "value": count / med, "unit": "FFTs/s", "cores": threads, "kind": "port",
-        "sample": f"first {count} of the {batch} N={n} signals, median of 5 timed runs after 1 warm-up",
+        "sample": f"all {batch} N={n} signals of the workload, {reps}x per timed run; "
+                  f"median of 5 runs ({sum(times):.1f} s of CPU work)",
         "GFLOPps": 5.0 * n * math.log2(n) * count / med / 1e9,
         "cpu_model": _cpu_model(), "cpu_max_rel_l2_vs_oracle": cpu_err,
         "gpu_max_rel_l2_vs_oracle": err, "gpu_checked_signals": m,
@@ -168,14 +169,64 @@ def _cpu_model() -> str:
     return "unknown"
 
 
-def load_traffic(name: str):
-    """HBM bytes per launch from the committed rocprofv3 --pmc summaries."""
-    path = os.path.join(ROOT, "profiles", "traffic.json")
+TRAFFIC_JSON = os.path.join(ROOT, "profiles", "traffic.json")
+COPY_KERNEL = "__amd_rocclr_copyBuffer"
+
+
+HEADLINE = "fp64_c2c_fwd_n4096_b16384"
+
+
+def load_traffic(workload: str):
+    """HBM bytes per launch of the headline kernel, from the committed
+    rocprofv3 --pmc summary (profiles/traffic.json, written by --traffic from
+    counter runs of `bench.py --quick`, which launches only that kernel and
+    the calibration copy)."""
     try:
-        with open(path) as f:
-            return json.load(f).get(name)
-    except (OSError, ValueError):
+        with open(TRAFFIC_JSON) as f:
+            t = json.load(f)
+        if t.get("workload") != workload or len(t["kernels"]) != 1:
+            return None
+        (name, v), = t["kernels"].items()
+        return dict(v, kernel=name, fetch_scale=t["fetch_scale"], write_scale=t["write_scale"])
+    except (OSError, ValueError, KeyError):
         return None
+
+
+def traffic_from_pmc(fetch_csv: str, write_csv: str) -> dict:
+    """Scales FETCH_SIZE / WRITE_SIZE (KiB per dispatch) by what the same
+    counters read for the 1 GiB device-to-device copy this script runs
+    (known: 1 GiB read, 1 GiB written) and writes profiles/traffic.json."""
+    import csv
+    import collections
+
+    def mean_by_kernel(path, counter):
+        acc = collections.defaultdict(list)
+        for r in csv.DictReader(open(path)):
+            if r["Counter_Name"] == counter:
+                acc[r["Kernel_Name"]].append(float(r["Counter_Value"]))
+        return {k: statistics.fmean(v) for k, v in acc.items()}
+
+    fetch = mean_by_kernel(fetch_csv, "FETCH_SIZE")
+    write = mean_by_kernel(write_csv, "WRITE_SIZE")
+    copy_f = [v for k, v in fetch.items() if COPY_KERNEL in k]
+    copy_w = [v for k, v in write.items() if COPY_KERNEL in k]
+    if not copy_f or not copy_w:
+        raise SystemExit("no copy kernel in the counter files")
+    kib = (1 << 30) / 1024.0
+    f_scale, w_scale = kib / max(copy_f), kib / max(copy_w)
+    out = {"fetch_scale": f_scale, "write_scale": w_scale,
+           "calibration": "1 GiB hipMemcpy device-to-device (read 1 GiB + write 1 GiB)",
+           "workload": HEADLINE, "kernels": {}}
+    for k in fetch:
+        if COPY_KERNEL in k or k not in write:
+            continue
+        key = k.split("(")[0]
+        out["kernels"][key] = {"fetch_bytes": fetch[k] * 1024 * f_scale,
+                               "write_bytes": write[k] * 1024 * w_scale,
+                               "bytes": (fetch[k] * f_scale + write[k] * w_scale) * 1024}
+    with open(TRAFFIC_JSON, "w") as f:
+        json.dump(out, f, indent=1)
+    return out
 
 
 def main():
@@ -185,7 +236,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--quick", action="store_true", help="headline workload only")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--traffic", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="write profiles/traffic.json from two rocprofv3 --pmc runs and exit")
     args = ap.parse_args()
+    if args.traffic:
+        print(json.dumps(traffic_from_pmc(*args.traffic), indent=1))
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one GPU; the batched FFT shards by contiguous "
               "signal ranges with no exchange, so N>1 is not run here.", file=sys.stderr)
@@ -198,7 +254,7 @@ def main():
         sys.exit(1)
     fo.init()
 
-    head = Workload("fp64_c2c_fwd_n4096_b16384", 4096, 16384, 7, "normal", "best")
+    head = Workload(HEADLINE, 4096, 16384, 7, "normal", "best")
     res, x, sample = run_gpu(fo, head, args.steps, args.warmup)
     print(f"[bench] {head.name}: {res['ffts_per_s']:.4g} FFT/s, {res['GBps']:.0f} GB/s", file=sys.stderr, flush=True)
 
@@ -229,7 +285,7 @@ def main():
             r, _, _ = run_gpu(fo, wl, args.steps, args.warmup, host_x=cache[key])
             extra[wl.name] = {k: (round(v, 6) if isinstance(v, float) else v) for k, v in r.items()}
             print(f"[bench] {wl.name}: {r['ffts_per_s']:.4g} FFT/s, {r['GBps']:.0f} GB/s", file=sys.stderr, flush=True)
-        extra["hip_d2d_copy_GBps"] = round(copy_bandwidth(), 1)
+    extra["hip_d2d_copy_GBps"] = round(copy_bandwidth(), 1)
 
     cpu = None
     if not args.no_cpu:
@@ -237,9 +293,12 @@ def main():
         cpu = cpu_baseline(x, sample, threads)
         print(f"[bench] cpu baseline: {cpu['value']:.4g} FFT/s on {threads} threads", file=sys.stderr, flush=True)
 
+    traffic = load_traffic(head.name)
     roof = {"bound": "hbm", "achieved": round(res["GBps"], 2), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-            "frac": round(res["hbm_frac"], 4), "traffic": load_traffic(head.name),
-            "kernel": "fftoc_fft N=4096 (radix-16 Stockham, 3 passes)",
+            "frac": round(res["hbm_frac"], 4),
+            "traffic": round(traffic["bytes"]) if traffic else None,
+            "traffic_detail": traffic,
+            "kernel": "fftoc_fft N=4096 = radix:16x16x16/tile/S1 (3 radix-16 Stockham passes, one signal per 256-thread workgroup)",
             "algorithmic_bytes_per_launch": 32 * head.n * head.batch,
             "launch_ms_mean": round(res["launch_ms_mean"], 5)}
     line = {

@@ -28,6 +28,17 @@ SIGNATURES = {
     "fftoc_legacy16_fft": ([_vp, _vp, _i64, _int, _u32, _vp], _int),
     "fftoc_fft": ([_vp, _vp, _i64, _int, _u32, _vp], _int),
 }
+SIGNATURES.update({
+    "fftoc_variant_count": ([_int], _int),
+    "fftoc_variant_name": ([_int, _int], ctypes.c_char_p),
+    "fftoc_fft_variant": ([_vp, _vp, _i64, _int, _u32, _int, _vp], _int),
+    "fftoc_get_variant": ([_int], _int),
+    "fftoc_set_variant": ([_int, _int], _int),
+    "fftoc_autotune": ([_int, _i64, _int, _vp], _int),
+    "fftoc_variant_time_us": ([_int, _int], ctypes.c_double),
+    "fftoc_tuning_save": ([ctypes.c_char_p], _int),
+    "fftoc_tuning_load": ([ctypes.c_char_p], _int),
+})
 TENSOR_SIZES = (8, 16, 64, 128, 256, 512, 4096)
 for _n in TENSOR_SIZES:
     SIGNATURES[f"fftoc_tensor_fft_{_n}"] = ([_vp, _vp, _i64, _u32, _vp], _int)

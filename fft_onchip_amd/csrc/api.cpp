@@ -2,10 +2,14 @@
 // dispatch to the gfx950 kernels.  Declarations and semantics: include/fftoc.h.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdint>
+#include <cstdio>
 #include <cstring>
 #include <mutex>
+#include <string>
 #include <vector>
 
 #include "fftoc_internal.hpp"
@@ -91,8 +95,81 @@ int init_current_device(int *dev_out) {
 }
 
 bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
+int log2i(int n) { return 31 - __builtin_clz((unsigned)n); }
 
 enum class Family { Tensor, Radix8, Radix16, Best };
+
+unsigned family_bit(Family f) {
+  switch (f) {
+    case Family::Tensor: return FAM_TENSOR;
+    case Family::Radix8: return FAM_RADIX8;
+    case Family::Radix16: return FAM_RADIX16;
+    default: return FAM_ANY;
+  }
+}
+int family_index(Family f) { return (int)f; }
+
+// ---- variant registry and tuning table --------------------------------------
+// Variants of length n = 2^k (k = 3..12): tensor variants first, then radix.
+struct SizeRegistry {
+  std::vector<Variant> v;
+  std::atomic<int> choice[4];  // per Family: index into v, -1 = none
+  std::vector<double> tuned_us;  // last autotune time per variant (<0: not run)
+};
+
+SizeRegistry g_reg[13];
+std::once_flag g_reg_once;
+
+// Default choice of fftoc_fft per length, from MI355X measurements of the
+// variants at 256 MiB - 1 GiB batches (replaced by fftoc_autotune /
+// fftoc_tuning_load).
+const char *default_best(int n) {
+  switch (n) {
+    case 8: return "radix:4x2/staged/S128";
+    case 16: return "radix:4x4/staged/S64";
+    case 32: return "tensor:16x2/8 per wave";
+    case 64: return "radix:8x8/tile/S16";
+    case 128: return "radix:16x8/tile/S8";
+    case 256: return "radix:16x16/tile/S4";
+    case 512: return "radix:8x8x8/tile/S1";
+    case 1024: return "radix:8x8x16/tile/S2";
+    case 2048: return "radix:8x16x16/tile/S1";
+    default: return "radix:16x16x16/tile/S1";
+  }
+}
+
+void build_registry() {
+  for (int k = 3; k <= 12; ++k) {
+    const int n = 1 << k;
+    SizeRegistry &r = g_reg[k];
+    const Variant *t = nullptr, *x = nullptr;
+    const int nt = tensor_variants(n, &t), nx = radix_variants(n, &x);
+    for (int i = 0; i < nt; ++i) r.v.push_back(t[i]);
+    for (int i = 0; i < nx; ++i) r.v.push_back(x[i]);
+    r.tuned_us.assign(r.v.size(), -1.0);
+    const Family fams[4] = {Family::Tensor, Family::Radix8, Family::Radix16, Family::Best};
+    for (Family f : fams) {
+      int pick = -1;
+      for (size_t i = 0; i < r.v.size(); ++i) {
+        if (r.v[i].families & family_bit(f)) {
+          pick = (int)i;
+          break;
+        }
+      }
+      if (f == Family::Best) {
+        for (size_t i = 0; i < r.v.size(); ++i)
+          if (std::strcmp(r.v[i].name, default_best(n)) == 0) pick = (int)i;
+      }
+      r.choice[family_index(f)].store(pick);
+    }
+  }
+}
+
+SizeRegistry *registry(int n) {
+  if (!is_pow2(n) || n < 8 || n > 4096) return nullptr;
+  std::call_once(g_reg_once, build_registry);
+  return &g_reg[log2i(n)];
+}
 
 int check_and_fill(const void *in, void *out, int64_t batch, int n, uint32_t flags,
                    void *stream, LaunchArgs &a) {
@@ -125,7 +202,7 @@ int check_and_fill(const void *in, void *out, int64_t batch, int n, uint32_t fla
 }
 
 int run(Family fam, const void *in, void *out, int64_t batch, int n, uint32_t flags,
-        void *stream) {
+        void *stream, int variant = -1) {
   LaunchArgs a;
   int rc = check_and_fill(in, out, batch, n, flags, stream, a);
   if (rc != FFTOC_SUCCESS) return rc;
@@ -147,19 +224,79 @@ int run(Family fam, const void *in, void *out, int64_t batch, int n, uint32_t fl
       if (a.precision != FFTOC_PREC_FP64) return FFTOC_ERROR_UNSUPPORTED;
       break;
   }
+  SizeRegistry *reg = registry(n);
+  if (!reg) return FFTOC_ERROR_INVALID_SIZE;
+  const int pick = variant >= 0 ? variant : reg->choice[family_index(fam)].load();
+  if (pick < 0 || pick >= (int)reg->v.size()) return FFTOC_ERROR_INVALID_SIZE;
   if (batch == 0) return FFTOC_SUCCESS;
   int dev = 0;
   rc = init_current_device(&dev);
   if (rc != FFTOC_SUCCESS) return rc;
   a.twiddles = g_dev[dev].twiddles;
-  switch (fam) {
-    case Family::Radix8: return launch_radix8(n, a);
-    case Family::Radix16: return launch_radix16(n, a);
-    case Family::Tensor: return launch_tensor(n, a);
-    case Family::Best: return launch_radix_mixed(n, a);
-  }
-  return FFTOC_ERROR_UNSUPPORTED;
+  return reg->v[pick].launch(a);
 }
+
+// Times every variant of length n on zero-filled scratch buffers (FFT
+// timing does not depend on the data) and makes the fastest variant of each
+// family the choice of that family's entry point.
+int autotune(int n, int64_t batch, int reps, void *stream_p) {
+  SizeRegistry *reg = registry(n);
+  if (!reg) return FFTOC_ERROR_INVALID_SIZE;
+  if (batch <= 0 || reps <= 0) return FFTOC_ERROR_INVALID_BATCH;
+  int dev = 0;
+  int rc = init_current_device(&dev);
+  if (rc != FFTOC_SUCCESS) return rc;
+  hipStream_t stream = static_cast<hipStream_t>(stream_p);
+  const size_t bytes = (size_t)batch * (size_t)n * 16u;
+  void *in = nullptr, *out = nullptr;
+  if (hipMalloc(&in, bytes) != hipSuccess) return FFTOC_ERROR_ALLOC;
+  if (hipMalloc(&out, bytes) != hipSuccess) {
+    (void)hipFree(in);
+    return FFTOC_ERROR_ALLOC;
+  }
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  rc = FFTOC_SUCCESS;
+  if (hipMemsetAsync(in, 0, bytes, stream) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+      hipEventCreate(&e1) != hipSuccess)
+    rc = FFTOC_ERROR_DEVICE;
+  std::vector<double> us(reg->v.size(), -1.0);
+  for (size_t i = 0; i < reg->v.size() && rc == FFTOC_SUCCESS; ++i) {
+    LaunchArgs a{in, out, batch, g_dev[dev].twiddles, stream, false, 1.0, 0, FFTOC_PREC_FP64};
+    if (reg->v[i].launch(a) != FFTOC_SUCCESS) continue;  // warm-up
+    std::vector<float> t;
+    for (int r = 0; r < reps; ++r) {
+      float ms = 0.f;
+      if (hipEventRecord(e0, stream) != hipSuccess || reg->v[i].launch(a) != FFTOC_SUCCESS ||
+          hipEventRecord(e1, stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+          hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+        rc = FFTOC_ERROR_DEVICE;
+        break;
+      }
+      t.push_back(ms);
+    }
+    if (rc != FFTOC_SUCCESS || t.empty()) break;
+    std::sort(t.begin(), t.end());
+    us[i] = 1000.0 * t[t.size() / 2];
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  (void)hipStreamSynchronize(stream);
+  (void)hipFree(in);
+  (void)hipFree(out);
+  if (rc != FFTOC_SUCCESS) return rc;
+  reg->tuned_us = us;
+  const Family fams[4] = {Family::Tensor, Family::Radix8, Family::Radix16, Family::Best};
+  for (Family f : fams) {
+    int best = -1;
+    for (size_t i = 0; i < us.size(); ++i)
+      if ((reg->v[i].families & family_bit(f)) && us[i] >= 0 && (best < 0 || us[i] < us[best]))
+        best = (int)i;
+    if (best >= 0) reg->choice[family_index(f)].store(best);
+  }
+  return FFTOC_SUCCESS;
+}
+
+const char *kFamilyNames[4] = {"tensor", "legacy8", "legacy16", "fft"};
 
 }  // namespace
 }  // namespace fftoc
@@ -242,6 +379,109 @@ int fftoc_legacy16_fft(const void *in, void *out, int64_t batch, int n, uint32_t
 
 int fftoc_fft(const void *in, void *out, int64_t batch, int n, uint32_t flags, void *stream) {
   return fftoc::run(Family::Best, in, out, batch, n, flags, stream);
+}
+
+int fftoc_variant_count(int n) {
+  fftoc::SizeRegistry *r = fftoc::registry(n);
+  return r ? (int)r->v.size() : 0;
+}
+
+const char *fftoc_variant_name(int n, int variant) {
+  fftoc::SizeRegistry *r = fftoc::registry(n);
+  if (!r || variant < 0 || variant >= (int)r->v.size()) return nullptr;
+  return r->v[variant].name;
+}
+
+int fftoc_fft_variant(const void *in, void *out, int64_t batch, int n, uint32_t flags,
+                      int variant, void *stream) {
+  fftoc::SizeRegistry *r = fftoc::registry(n);
+  if (!r) return FFTOC_ERROR_INVALID_SIZE;
+  if (variant < 0 || variant >= (int)r->v.size()) return FFTOC_ERROR_INVALID_HANDLE;
+  const Family fam = (r->v[variant].families & fftoc::FAM_TENSOR) ? Family::Tensor : Family::Best;
+  return fftoc::run(fam, in, out, batch, n, flags, stream, variant);
+}
+
+int fftoc_get_variant(int n) {
+  fftoc::SizeRegistry *r = fftoc::registry(n);
+  return r ? r->choice[fftoc::family_index(Family::Best)].load() : -1;
+}
+
+int fftoc_set_variant(int n, int variant) {
+  fftoc::SizeRegistry *r = fftoc::registry(n);
+  if (!r) return FFTOC_ERROR_INVALID_SIZE;
+  if (variant < 0 || variant >= (int)r->v.size()) return FFTOC_ERROR_INVALID_HANDLE;
+  r->choice[fftoc::family_index(Family::Best)].store(variant);
+  return FFTOC_SUCCESS;
+}
+
+int fftoc_autotune(int n, int64_t batch, int reps, void *stream) {
+  return fftoc::autotune(n, batch, reps, stream);
+}
+
+double fftoc_variant_time_us(int n, int variant) {
+  fftoc::SizeRegistry *r = fftoc::registry(n);
+  if (!r || variant < 0 || variant >= (int)r->v.size()) return -1.0;
+  return r->tuned_us[variant];
+}
+
+// Text format, one line per (entry point, length):
+//   <family> <n> <variant name up to end of line>
+int fftoc_tuning_save(const char *path) {
+  if (!path) return FFTOC_ERROR_NULL_POINTER;
+  FILE *f = std::fopen(path, "w");
+  if (!f) return FFTOC_ERROR_INVALID_HANDLE;
+  std::fprintf(f, "# fftoc tuning table: family n variant\n");
+  for (int k = 3; k <= 12; ++k) {
+    fftoc::SizeRegistry *r = fftoc::registry(1 << k);
+    for (int fi = 0; fi < 4; ++fi) {
+      const int c = r->choice[fi].load();
+      if (c >= 0) std::fprintf(f, "%s %d %s\n", fftoc::kFamilyNames[fi], 1 << k, r->v[c].name);
+    }
+  }
+  const int err = std::ferror(f);
+  std::fclose(f);
+  return err ? FFTOC_ERROR_INVALID_HANDLE : FFTOC_SUCCESS;
+}
+
+int fftoc_tuning_load(const char *path) {
+  if (!path) return FFTOC_ERROR_NULL_POINTER;
+  FILE *f = std::fopen(path, "r");
+  if (!f) return FFTOC_ERROR_INVALID_HANDLE;
+  char line[512];
+  int rc = FFTOC_SUCCESS;
+  while (std::fgets(line, sizeof line, f)) {
+    if (line[0] == '#' || line[0] == '\n') continue;
+    char fam[32];
+    int n = 0, used = 0;
+    if (std::sscanf(line, "%31s %d %n", fam, &n, &used) != 2) {
+      rc = FFTOC_ERROR_INVALID_HANDLE;
+      continue;
+    }
+    std::string name(line + used);
+    while (!name.empty() && (name.back() == '\n' || name.back() == '\r')) name.pop_back();
+    fftoc::SizeRegistry *r = fftoc::registry(n);
+    int fi = -1;
+    for (int i = 0; i < 4; ++i)
+      if (std::strcmp(fam, fftoc::kFamilyNames[i]) == 0) fi = i;
+    if (!r || fi < 0) {
+      rc = FFTOC_ERROR_INVALID_HANDLE;
+      continue;
+    }
+    int match = -1;
+    for (size_t i = 0; i < r->v.size(); ++i)
+      if (name == r->v[i].name) match = (int)i;
+    const unsigned need = fi == 0   ? fftoc::FAM_TENSOR
+                          : fi == 1 ? fftoc::FAM_RADIX8
+                          : fi == 2 ? fftoc::FAM_RADIX16
+                                    : fftoc::FAM_ANY;
+    if (match < 0 || !(r->v[match].families & need)) {
+      rc = FFTOC_ERROR_INVALID_HANDLE;
+      continue;
+    }
+    r->choice[fi].store(match);
+  }
+  std::fclose(f);
+  return rc;
 }
 
 }  // extern "C"

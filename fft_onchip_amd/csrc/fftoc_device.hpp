@@ -122,6 +122,54 @@ __device__ __forceinline__ void dft(c2 *a) {
   }
 }
 
+// Multiplies a[m] (m = 1..R-1) by W^{m k}, where tw[idx * m] = W^{m k}: only
+// W^k, W^2k, W^4k, W^8k are read from the table, the other powers are
+// products of at most four table values (error a few FP64 ulp).
+template <int R>
+__device__ __forceinline__ void apply_twiddles(c2 *a, const c2 *__restrict__ tw, int idx) {
+  if constexpr (R == 2) {
+    a[1] = cmul(a[1], tw[idx]);
+  } else {
+    const c2 w1 = tw[idx], w2 = tw[2 * idx];
+    const c2 w3 = cmul(w2, w1);
+    a[1] = cmul(a[1], w1);
+    a[2] = cmul(a[2], w2);
+    a[3] = cmul(a[3], w3);
+    if constexpr (R >= 8) {
+      const c2 w4 = tw[4 * idx];
+      a[4] = cmul(a[4], w4);
+      a[5] = cmul(a[5], cmul(w4, w1));
+      a[6] = cmul(a[6], cmul(w4, w2));
+      const c2 w7 = cmul(w4, w3);
+      a[7] = cmul(a[7], w7);
+      if constexpr (R == 16) {
+        const c2 w8 = tw[8 * idx];
+        a[8] = cmul(a[8], w8);
+        a[9] = cmul(a[9], cmul(w8, w1));
+        a[10] = cmul(a[10], cmul(w8, w2));
+        a[11] = cmul(a[11], cmul(w8, w3));
+        a[12] = cmul(a[12], cmul(w8, w4));
+        a[13] = cmul(a[13], cmul(w8, cmul(w4, w1)));
+        a[14] = cmul(a[14], cmul(w8, cmul(w4, w2)));
+        a[15] = cmul(a[15], cmul(w8, w7));
+      }
+    }
+  }
+}
+
+// Streaming (non-temporal) global access for data touched exactly once:
+// keeps the input/output stream from evicting the twiddle table and other
+// reused lines from L2 / MALL.
+typedef double dvec2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ c2 ld_stream(const c2 *p) {
+  const dvec2 v = __builtin_nontemporal_load(reinterpret_cast<const dvec2 *>(p));
+  return make_double2(v.x, v.y);
+}
+__device__ __forceinline__ void st_stream(c2 *p, c2 v) {
+  const dvec2 t = {v.x, v.y};
+  __builtin_nontemporal_store(t, reinterpret_cast<dvec2 *>(p));
+}
+
 // Wave-scope ordering point for LDS exchanges among the lanes of one wave:
 // LDS operations of a wave execute in issue order, so only compiler
 // reordering has to be prevented.

@@ -20,12 +20,24 @@ struct LaunchArgs {
   uint32_t precision;    // FFTOC_PREC_* bits
 };
 
-// radix_kernels.hip
-int launch_radix8(int n, const LaunchArgs &a);
-int launch_radix16(int n, const LaunchArgs &a);
-int launch_radix_mixed(int n, const LaunchArgs &a);
+// A kernel variant for one transform length: name, the entry-point families
+// it may serve, and its launcher.
+enum FamilyBits : unsigned {
+  FAM_RADIX8 = 1u,   // fftoc_legacy8_fft  (radix-8 passes only)
+  FAM_RADIX16 = 2u,  // fftoc_legacy16_fft (radix-16 passes only)
+  FAM_TENSOR = 4u,   // fftoc_tensor_fft   (FP64 MFMA stages)
+  FAM_ANY = 8u,      // fftoc_fft          (every variant)
+};
 
-// mfma_kernels.hip
-int launch_tensor(int n, const LaunchArgs &a);
+struct Variant {
+  const char *name;
+  unsigned families;
+  int (*launch)(const LaunchArgs &);
+};
+
+// radix_kernels.hip / mfma_kernels.hip: variants available for length n,
+// preferred (default) variant of each family first.
+int radix_variants(int n, const Variant **out);
+int tensor_variants(int n, const Variant **out);
 
 }  // namespace fftoc

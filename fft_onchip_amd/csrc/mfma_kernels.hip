@@ -94,7 +94,7 @@ __device__ __forceinline__ void kgemm(const c2 (&a)[4], const Frag &f, v4d &yr, 
 
 template <bool INV>
 __device__ __forceinline__ c2 ld_in(const c2 *p) {
-  c2 v = *p;
+  c2 v = ld_stream(p);
   if constexpr (INV) v = cswap(v);
   return v;
 }
@@ -103,7 +103,7 @@ template <bool INV>
 __device__ __forceinline__ void st_out(c2 *p, c2 v, double scale) {
   if (scale != 1.0) v = cscale(v, scale);
   if constexpr (INV) v = cswap(v);
-  *p = v;
+  st_stream(p, v);
 }
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
@@ -439,37 +439,63 @@ __global__ __launch_bounds__(64 * k4096Waves) void tensor4096_kernel(const c2 *i
 
 int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
-#define FFTOC_LAUNCH_INV(KERNEL, GRID, BLOCK, ...)                                                 \
-  do {                                                                                             \
-    if ((GRID) > 0x7fffffffLL) return FFTOC_ERROR_INVALID_BATCH;                                   \
-    if (a.inverse)                                                                                 \
-      hipLaunchKernelGGL((KERNEL<__VA_ARGS__ __VA_OPT__(, ) true>), dim3((unsigned)(GRID)),        \
-                         dim3(BLOCK), 0, a.stream, in, out, a.batch, tw, a.scale);                 \
-    else                                                                                           \
-      hipLaunchKernelGGL((KERNEL<__VA_ARGS__ __VA_OPT__(, ) false>), dim3((unsigned)(GRID)),       \
-                         dim3(BLOCK), 0, a.stream, in, out, a.batch, tw, a.scale);                 \
-  } while (0)
+#define FFTOC_TENSOR_LAUNCHER(FN, KERNEL, UNITS_PER_WG, BLOCK, ...)                              \
+  int FN(const LaunchArgs &a) {                                                                \
+    if (a.precision != FFTOC_PREC_FP64) return FFTOC_ERROR_UNSUPPORTED;                        \
+    const c2 *in = static_cast<const c2 *>(a.in);                                              \
+    c2 *out = static_cast<c2 *>(a.out);                                                        \
+    const c2 *tw = static_cast<const c2 *>(a.twiddles);                                        \
+    const int64_t grid = ceil_div(a.batch, (UNITS_PER_WG));                                    \
+    if (grid > 0x7fffffffLL) return FFTOC_ERROR_INVALID_BATCH;                                 \
+    if (a.inverse)                                                                             \
+      hipLaunchKernelGGL((KERNEL<__VA_ARGS__ __VA_OPT__(, ) true>), dim3((unsigned)grid),      \
+                         dim3(BLOCK), 0, a.stream, in, out, a.batch, tw, a.scale);             \
+    else                                                                                       \
+      hipLaunchKernelGGL((KERNEL<__VA_ARGS__ __VA_OPT__(, ) false>), dim3((unsigned)grid),     \
+                         dim3(BLOCK), 0, a.stream, in, out, a.batch, tw, a.scale);             \
+    return hipGetLastError() == hipSuccess ? FFTOC_SUCCESS : FFTOC_ERROR_LAUNCH;               \
+  }
+
+FFTOC_TENSOR_LAUNCHER(launch_t8, tensor8_kernel, 32 * kWaves, 64 * kWaves)
+FFTOC_TENSOR_LAUNCHER(launch_t16, tensor16_kernel, 16 * kWaves, 64 * kWaves)
+FFTOC_TENSOR_LAUNCHER(launch_t32, tensor16xm_kernel, 8 * kWaves, 64 * kWaves, 2)
+FFTOC_TENSOR_LAUNCHER(launch_t64, tensor16xm_kernel, 4 * kWaves, 64 * kWaves, 4)
+FFTOC_TENSOR_LAUNCHER(launch_t128, tensor16xm_kernel, 2 * kWaves, 64 * kWaves, 8)
+FFTOC_TENSOR_LAUNCHER(launch_t256, tensor256_kernel, kWaves, 64 * kWaves)
+FFTOC_TENSOR_LAUNCHER(launch_t512, tensor512_kernel, kWaves, 64 * kWaves)
+FFTOC_TENSOR_LAUNCHER(launch_t4096, tensor4096_kernel, 1, 64 * k4096Waves)
+#undef FFTOC_TENSOR_LAUNCHER
+
+constexpr unsigned TF = FAM_TENSOR | FAM_ANY;
+const Variant kT8[] = {{"tensor:diag(F8,F8)/32 per wave", TF, &launch_t8}};
+const Variant kT16[] = {{"tensor:16/16 per wave", TF, &launch_t16}};
+const Variant kT32[] = {{"tensor:16x2/8 per wave", TF, &launch_t32}};
+const Variant kT64[] = {{"tensor:16x4/4 per wave", TF, &launch_t64}};
+const Variant kT128[] = {{"tensor:16x8/2 per wave", TF, &launch_t128}};
+const Variant kT256[] = {{"tensor:16x16/1 per wave", TF, &launch_t256}};
+const Variant kT512[] = {{"tensor:16x2x16/1 per wave", TF, &launch_t512}};
+const Variant kT4096[] = {{"tensor:16x16x16/8 waves", TF, &launch_t4096}};
+
+template <size_t K>
+int set_out(const Variant (&v)[K], const Variant **out) {
+  *out = v;
+  return (int)K;
+}
 
 }  // namespace
 
-int launch_tensor(int n, const LaunchArgs &a) {
-  if (a.precision != FFTOC_PREC_FP64) return FFTOC_ERROR_UNSUPPORTED;
-  const c2 *in = static_cast<const c2 *>(a.in);
-  c2 *out = static_cast<c2 *>(a.out);
-  const c2 *tw = static_cast<const c2 *>(a.twiddles);
-  const int64_t b = a.batch;
+int tensor_variants(int n, const Variant **out) {
   switch (n) {
-    case 8: FFTOC_LAUNCH_INV(tensor8_kernel, ceil_div(b, 32 * kWaves), 64 * kWaves); break;
-    case 16: FFTOC_LAUNCH_INV(tensor16_kernel, ceil_div(b, 16 * kWaves), 64 * kWaves); break;
-    case 32: FFTOC_LAUNCH_INV(tensor16xm_kernel, ceil_div(b, 8 * kWaves), 64 * kWaves, 2); break;
-    case 64: FFTOC_LAUNCH_INV(tensor16xm_kernel, ceil_div(b, 4 * kWaves), 64 * kWaves, 4); break;
-    case 128: FFTOC_LAUNCH_INV(tensor16xm_kernel, ceil_div(b, 2 * kWaves), 64 * kWaves, 8); break;
-    case 256: FFTOC_LAUNCH_INV(tensor256_kernel, ceil_div(b, kWaves), 64 * kWaves); break;
-    case 512: FFTOC_LAUNCH_INV(tensor512_kernel, ceil_div(b, kWaves), 64 * kWaves); break;
-    case 4096: FFTOC_LAUNCH_INV(tensor4096_kernel, b, 64 * k4096Waves); break;
-    default: return FFTOC_ERROR_INVALID_SIZE;
+    case 8: return set_out(kT8, out);
+    case 16: return set_out(kT16, out);
+    case 32: return set_out(kT32, out);
+    case 64: return set_out(kT64, out);
+    case 128: return set_out(kT128, out);
+    case 256: return set_out(kT256, out);
+    case 512: return set_out(kT512, out);
+    case 4096: return set_out(kT4096, out);
+    default: *out = nullptr; return 0;
   }
-  return hipGetLastError() == hipSuccess ? FFTOC_SUCCESS : FFTOC_ERROR_LAUNCH;
 }
 
 }  // namespace fftoc

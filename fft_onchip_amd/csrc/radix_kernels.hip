@@ -14,8 +14,13 @@
 //    read T consecutive complex values (16 B per lane, dwordx4), so each
 //    load instruction covers whole 64..256 B runs.  The last pass writes
 //    out[j + q*L] with j < L, again T consecutive values per signal.  For
-//    T < 4 (N = 8, 16) the tile is instead staged through LDS with
+//    T < 4 (N = 8, 16, 32) the tile is instead staged through LDS with
 //    fully contiguous 1 KiB-per-instruction copies.
+//  * Streaming data uses non-temporal loads and stores.
+//  * Persistent variant: a workgroup loops over tiles and issues the HBM
+//    loads of its next tile right after the current tile's first pass has
+//    gone to LDS, so the next tile's reads are in flight while the current
+//    tile runs its LDS passes (workgroup barriers only wait for LDS traffic).
 //  * Between passes the data crosses lanes through LDS.  When a signal lives
 //    inside one wave (T <= 64) no workgroup barrier is needed: the LDS
 //    operations of a wave are executed in order.
@@ -68,33 +73,38 @@ struct StockhamKernel {
     }
   }
 
-  template <int p>
-  __device__ static void pass(c2 (&reg)[RM], c2 *lds, const c2 *gin, c2 *gout, int t,
-                              int sbase, bool valid, const c2 *__restrict__ tw, double scale) {
-    constexpr int R = SP::R[p];
-    constexpr int L = SP::L(p);
+  // Pass-0 inputs straight from HBM (non-staged kernels).
+  __device__ static void load_first(c2 (&reg)[RM], const c2 *gin, int t, bool valid) {
+    constexpr int R = SP::R[0];
     constexpr int NB = RM / R;
-    constexpr bool FIRST = (p == 0);
-    constexpr bool LAST = (p == SP::P - 1);
-
-    // ---- gather inputs of this pass
 #pragma unroll
     for (int i = 0; i < NB; ++i) {
-      const int j = t + i * T;
 #pragma unroll
       for (int m = 0; m < R; ++m) {
-        const int e = j + m * (N / R);
-        c2 v;
-        if constexpr (FIRST && !STAGED) {
-          v = valid ? gin[e] : make_double2(0.0, 0.0);
-          if constexpr (INV) v = cswap(v);
-        } else {
-          v = lds[lds_index(sbase + e)];
-        }
+        c2 v = valid ? ld_stream(gin + t + i * T + m * (N / R)) : make_double2(0.0, 0.0);
+        if constexpr (INV) v = cswap(v);
         reg[i * R + m] = v;
       }
     }
-    if constexpr (!FIRST || STAGED) {
+  }
+
+  // One Stockham pass.  IN_REGS: inputs already in reg (pass 0 of the
+  // non-staged kernels); otherwise they are gathered from LDS.
+  template <int p, bool IN_REGS>
+  __device__ static void pass(c2 (&reg)[RM], c2 *lds, c2 *gout, int t, int sbase, bool valid,
+                              const c2 *__restrict__ tw, double scale) {
+    constexpr int R = SP::R[p];
+    constexpr int L = SP::L(p);
+    constexpr int NB = RM / R;
+    constexpr bool LAST = (p == SP::P - 1);
+
+    if constexpr (!IN_REGS) {
+#pragma unroll
+      for (int i = 0; i < NB; ++i) {
+        const int j = t + i * T;
+#pragma unroll
+        for (int m = 0; m < R; ++m) reg[i * R + m] = lds[lds_index(sbase + j + m * (N / R))];
+      }
       // all lanes have read this pass's inputs before anyone overwrites them
       sync();
     }
@@ -106,10 +116,7 @@ struct StockhamKernel {
       if constexpr (L > 1) {
         const int k = j & (L - 1);
         constexpr int step = kTwN / (R * L);
-#pragma unroll
-        for (int m = 1; m < R; ++m) {
-          reg[i * R + m] = cmul(reg[i * R + m], tw[m * k * step]);
-        }
+        apply_twiddles<R>(&reg[i * R], tw, k * step);
       }
       dft<R>(&reg[i * R]);
     }
@@ -126,7 +133,7 @@ struct StockhamKernel {
         if constexpr (LAST && !STAGED) {
           if (scale != 1.0) v = cscale(v, scale);
           if constexpr (INV) v = cswap(v);
-          if (valid) gout[d] = v;
+          if (valid) st_stream(gout + d, v);
         } else {
           lds[lds_index(sbase + d)] = v;
         }
@@ -136,12 +143,13 @@ struct StockhamKernel {
   }
 
   template <int... ps>
-  __device__ static void run_passes(std::integer_sequence<int, ps...>, c2 (&reg)[RM], c2 *lds,
-                                    const c2 *gin, c2 *gout, int t, int sbase, bool valid,
-                                    const c2 *__restrict__ tw, double scale) {
-    (pass<ps>(reg, lds, gin, gout, t, sbase, valid, tw, scale), ...);
+  __device__ static void later_passes(std::integer_sequence<int, ps...>, c2 (&reg)[RM], c2 *lds,
+                                      c2 *gout, int t, int sbase, bool valid,
+                                      const c2 *__restrict__ tw, double scale) {
+    (pass<ps + 1, false>(reg, lds, gout, t, sbase, valid, tw, scale), ...);
   }
 
+  // One tile per workgroup.
   __device__ static void run(const c2 *in, c2 *out, int64_t batch, const c2 *__restrict__ tw,
                              double scale) {
     __shared__ c2 lds[lds_elems(S * N)];
@@ -151,6 +159,7 @@ struct StockhamKernel {
     const int64_t tile0 = (int64_t)blockIdx.x * S;
     const int64_t sig = tile0 + sl;
     const bool valid = sig < batch;
+    c2 reg[RM];
 
     if constexpr (STAGED) {
       // contiguous tile copy HBM -> LDS (16 B per lane per instruction)
@@ -159,16 +168,18 @@ struct StockhamKernel {
 #pragma unroll
       for (int c = 0; c < (S * N) / THREADS; ++c) {
         const int E = tid + c * THREADS;
-        c2 v = (E < nvalid * N) ? src[E] : make_double2(0.0, 0.0);
+        c2 v = (E < nvalid * N) ? ld_stream(src + E) : make_double2(0.0, 0.0);
         if constexpr (INV) v = cswap(v);
         lds[lds_index(E)] = v;
       }
       __syncthreads();
+      pass<0, false>(reg, lds, nullptr, t, sl * N, valid, tw, scale);
+    } else {
+      load_first(reg, in + sig * N, t, valid);
+      pass<0, true>(reg, lds, out + sig * N, t, sl * N, valid, tw, scale);
     }
-
-    c2 reg[RM];
-    run_passes(std::make_integer_sequence<int, SP::P>{}, reg, lds, in + sig * N, out + sig * N, t,
-               sl * N, valid, tw, scale);
+    later_passes(std::make_integer_sequence<int, SP::P - 1>{}, reg, lds, out + sig * N, t, sl * N,
+                 valid, tw, scale);
 
     if constexpr (STAGED) {
       __syncthreads();
@@ -180,8 +191,49 @@ struct StockhamKernel {
         c2 v = lds[lds_index(E)];
         if (scale != 1.0) v = cscale(v, scale);
         if constexpr (INV) v = cswap(v);
-        if (E < nvalid * N) dst[E] = v;
+        if (E < nvalid * N) st_stream(dst + E, v);
       }
+    }
+  }
+
+  // Persistent workgroups: tiles blockIdx.x, blockIdx.x + gridDim.x, ...;
+  // the next tile's HBM loads overlap the current tile's LDS passes.
+  __device__ static void run_persistent(const c2 *in, c2 *out, int64_t batch,
+                                        const c2 *__restrict__ tw, double scale) {
+    static_assert(!STAGED && SP::P >= 2, "persistent form needs an LDS pass");
+    __shared__ c2 lds[lds_elems(S * N)];
+    const int tid = threadIdx.x;
+    const int t = tid % T;
+    const int sl = tid / T;
+    const int64_t ntiles = (batch + S - 1) / S;
+    int64_t tile = blockIdx.x;
+    if (tile >= ntiles) return;  // uniform over the workgroup
+    c2 reg[RM];
+    {
+      const int64_t sig = tile * S + sl;
+      load_first(reg, in + sig * N, t, sig < batch);
+    }
+    for (;;) {
+      const int64_t sig = tile * S + sl;
+      const bool valid = sig < batch;
+      const int64_t next = tile + gridDim.x;
+      // Re-read the twiddles every tile instead of letting the compiler hoist
+      // all of them out of the loop (they would pin ~120 VGPRs and halve the
+      // occupancy); the empty asm only hides the pointer's loop invariance.
+      const c2 *twl = tw;
+      asm volatile("" : "+s"(twl));
+      pass<0, true>(reg, lds, out + sig * N, t, sl * N, valid, twl, scale);
+      c2 nxt[RM];
+      if (next < ntiles) {
+        const int64_t nsig = next * S + sl;
+        load_first(nxt, in + nsig * N, t, nsig < batch);
+      }
+      later_passes(std::make_integer_sequence<int, SP::P - 1>{}, reg, lds, out + sig * N, t,
+                   sl * N, valid, twl, scale);
+      if (next >= ntiles) break;
+      tile = next;
+#pragma unroll
+      for (int i = 0; i < RM; ++i) reg[i] = nxt[i];
     }
   }
 };
@@ -195,61 +247,165 @@ __global__ __launch_bounds__((kThreads<SP, S>)) void stockham_kernel(
   StockhamKernel<SP, S, INV, STAGED>::run(in, out, batch, tw, scale);
 }
 
-template <class SP, int S, bool STAGED>
+template <class SP, int S, bool INV>
+__global__ __launch_bounds__((kThreads<SP, S>)) __attribute__((amdgpu_waves_per_eu(2)))
+void stockham_persistent_kernel(
+    const c2 *in, c2 *out, int64_t batch, const c2 *__restrict__ tw, double scale) {
+  StockhamKernel<SP, S, INV, false>::run_persistent(in, out, batch, tw, scale);
+}
+
+// Resident workgroups per CU of a kernel, queried once per instantiation.
+template <class K>
+int blocks_per_cu(K kernel, int threads) {
+  static int cached = 0;
+  if (cached == 0) {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, threads, 0) != hipSuccess ||
+        nb <= 0)
+      nb = 1;
+    cached = nb;
+  }
+  return cached;
+}
+
+int device_cus() {
+  static int cus = 0;
+  if (cus == 0) {
+    int dev = 0, n = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && n > 0)
+      cus = n;
+    else
+      cus = 256;
+  }
+  return cus;
+}
+
+enum class Form { OneTile, Staged, Persistent };
+
+template <class SP, int S, Form F>
 int launch_split(const LaunchArgs &a) {
-  using KF = StockhamKernel<SP, S, false, STAGED>;
-  const int64_t grid = (a.batch + S - 1) / S;
-  if (grid > 0x7fffffffLL) return FFTOC_ERROR_INVALID_BATCH;
+  constexpr int threads = kThreads<SP, S>;
+  const int64_t ntiles = (a.batch + S - 1) / S;
+  if (ntiles > 0x7fffffffLL) return FFTOC_ERROR_INVALID_BATCH;
   const c2 *in = static_cast<const c2 *>(a.in);
   c2 *out = static_cast<c2 *>(a.out);
   const c2 *tw = static_cast<const c2 *>(a.twiddles);
-  if (a.inverse) {
-    hipLaunchKernelGGL((stockham_kernel<SP, S, true, STAGED>), dim3((unsigned)grid),
-                       dim3(KF::THREADS), 0, a.stream, in, out, a.batch, tw, a.scale);
+  if constexpr (F == Form::Persistent) {
+    const int per_cu = blocks_per_cu(stockham_persistent_kernel<SP, S, false>, threads);
+    const int64_t resident = (int64_t)per_cu * device_cus();
+    const unsigned grid = (unsigned)(ntiles < resident ? ntiles : resident);
+    if (a.inverse)
+      hipLaunchKernelGGL((stockham_persistent_kernel<SP, S, true>), dim3(grid), dim3(threads), 0,
+                         a.stream, in, out, a.batch, tw, a.scale);
+    else
+      hipLaunchKernelGGL((stockham_persistent_kernel<SP, S, false>), dim3(grid), dim3(threads), 0,
+                         a.stream, in, out, a.batch, tw, a.scale);
   } else {
-    hipLaunchKernelGGL((stockham_kernel<SP, S, false, STAGED>), dim3((unsigned)grid),
-                       dim3(KF::THREADS), 0, a.stream, in, out, a.batch, tw, a.scale);
+    constexpr bool staged = (F == Form::Staged);
+    if (a.inverse)
+      hipLaunchKernelGGL((stockham_kernel<SP, S, true, staged>), dim3((unsigned)ntiles),
+                         dim3(threads), 0, a.stream, in, out, a.batch, tw, a.scale);
+    else
+      hipLaunchKernelGGL((stockham_kernel<SP, S, false, staged>), dim3((unsigned)ntiles),
+                         dim3(threads), 0, a.stream, in, out, a.batch, tw, a.scale);
   }
   return hipGetLastError() == hipSuccess ? FFTOC_SUCCESS : FFTOC_ERROR_LAUNCH;
 }
 
+#define FFTOC_V(NAME, FAM, ...) \
+  Variant { NAME, FAM, &launch_split<__VA_ARGS__> }
+
+constexpr unsigned R8 = FAM_RADIX8 | FAM_ANY;
+constexpr unsigned R16 = FAM_RADIX16 | FAM_ANY;
+constexpr unsigned MIX = FAM_ANY;
+
+const Variant kV8[] = {
+    FFTOC_V("radix:8/staged/S128", R8, Split<8>, 128, Form::Staged),
+    FFTOC_V("radix:8/staged/S256", R8, Split<8>, 256, Form::Staged),
+    FFTOC_V("radix:4x2/staged/S128", MIX, Split<4, 2>, 128, Form::Staged),
+};
+const Variant kV16[] = {
+    FFTOC_V("radix:16/staged/S64", R16, Split<16>, 64, Form::Staged),
+    FFTOC_V("radix:16/staged/S128", R16, Split<16>, 128, Form::Staged),
+    FFTOC_V("radix:4x4/staged/S64", MIX, Split<4, 4>, 64, Form::Staged),
+};
+const Variant kV32[] = {
+    FFTOC_V("radix:8x4/staged/S32", MIX, Split<8, 4>, 32, Form::Staged),
+    FFTOC_V("radix:8x4/tile/S32", MIX, Split<8, 4>, 32, Form::OneTile),
+    FFTOC_V("radix:8x4/tile/S64", MIX, Split<8, 4>, 64, Form::OneTile),
+    FFTOC_V("radix:4x8/staged/S32", MIX, Split<4, 8>, 32, Form::Staged),
+    FFTOC_V("radix:4x4x2/staged/S64", MIX, Split<4, 4, 2>, 64, Form::Staged),
+};
+const Variant kV64[] = {
+    FFTOC_V("radix:8x8/tile/S16", R8, Split<8, 8>, 16, Form::OneTile),
+    FFTOC_V("radix:8x8/tile/S32", R8, Split<8, 8>, 32, Form::OneTile),
+    FFTOC_V("radix:8x8/persistent/S32", R8, Split<8, 8>, 32, Form::Persistent),
+    FFTOC_V("radix:16x4/tile/S16", MIX, Split<16, 4>, 16, Form::OneTile),
+};
+const Variant kV128[] = {
+    FFTOC_V("radix:16x8/tile/S16", MIX, Split<16, 8>, 16, Form::OneTile),
+    FFTOC_V("radix:16x8/tile/S8", MIX, Split<16, 8>, 8, Form::OneTile),
+    FFTOC_V("radix:16x8/persistent/S16", MIX, Split<16, 8>, 16, Form::Persistent),
+    FFTOC_V("radix:8x16/tile/S16", MIX, Split<8, 16>, 16, Form::OneTile),
+};
+const Variant kV256[] = {
+    FFTOC_V("radix:16x16/tile/S4", R16, Split<16, 16>, 4, Form::OneTile),
+    FFTOC_V("radix:16x16/tile/S8", R16, Split<16, 16>, 8, Form::OneTile),
+    FFTOC_V("radix:16x16/persistent/S8", R16, Split<16, 16>, 8, Form::Persistent),
+    FFTOC_V("radix:8x8x4/tile/S8", MIX, Split<8, 8, 4>, 8, Form::OneTile),
+};
+const Variant kV512[] = {
+    FFTOC_V("radix:8x8x8/tile/S1", R8, Split<8, 8, 8>, 1, Form::OneTile),
+    FFTOC_V("radix:8x8x8/tile/S2", R8, Split<8, 8, 8>, 2, Form::OneTile),
+    FFTOC_V("radix:8x8x8/tile/S4", R8, Split<8, 8, 8>, 4, Form::OneTile),
+    FFTOC_V("radix:8x8x8/persistent/S4", R8, Split<8, 8, 8>, 4, Form::Persistent),
+    FFTOC_V("radix:16x16x2/tile/S4", MIX, Split<16, 16, 2>, 4, Form::OneTile),
+    FFTOC_V("radix:16x16x2/persistent/S4", MIX, Split<16, 16, 2>, 4, Form::Persistent),
+    FFTOC_V("radix:16x16x2/tile/S2", MIX, Split<16, 16, 2>, 2, Form::OneTile),
+};
+const Variant kV1024[] = {
+    FFTOC_V("radix:16x16x4/persistent/S2", MIX, Split<16, 16, 4>, 2, Form::Persistent),
+    FFTOC_V("radix:16x16x4/tile/S2", MIX, Split<16, 16, 4>, 2, Form::OneTile),
+    FFTOC_V("radix:16x16x4/tile/S1", MIX, Split<16, 16, 4>, 1, Form::OneTile),
+    FFTOC_V("radix:8x8x16/tile/S2", MIX, Split<8, 8, 16>, 2, Form::OneTile),
+    FFTOC_V("radix:4x16x16/tile/S2", MIX, Split<4, 16, 16>, 2, Form::OneTile),
+};
+const Variant kV2048[] = {
+    FFTOC_V("radix:16x16x8/persistent/S1", MIX, Split<16, 16, 8>, 1, Form::Persistent),
+    FFTOC_V("radix:16x16x8/tile/S1", MIX, Split<16, 16, 8>, 1, Form::OneTile),
+    FFTOC_V("radix:16x8x16/persistent/S1", MIX, Split<16, 8, 16>, 1, Form::Persistent),
+    FFTOC_V("radix:8x16x16/tile/S1", MIX, Split<8, 16, 16>, 1, Form::OneTile),
+};
+const Variant kV4096[] = {
+    FFTOC_V("radix:16x16x16/tile/S1", R16, Split<16, 16, 16>, 1, Form::OneTile),
+    FFTOC_V("radix:16x16x16/persistent/S1", R16, Split<16, 16, 16>, 1, Form::Persistent),
+    FFTOC_V("radix:8x8x8x8/tile/S1", R8, Split<8, 8, 8, 8>, 1, Form::OneTile),
+    FFTOC_V("radix:8x8x8x8/persistent/S1", R8, Split<8, 8, 8, 8>, 1, Form::Persistent),
+};
+#undef FFTOC_V
+
+template <size_t K>
+int set_out(const Variant (&v)[K], const Variant **out) {
+  *out = v;
+  return (int)K;
+}
+
 }  // namespace
 
-// Radix-8 family (legacy_fft): N = 8^d.
-int launch_radix8(int n, const LaunchArgs &a) {
+int radix_variants(int n, const Variant **out) {
   switch (n) {
-    case 8: return launch_split<Split<8>, 128, true>(a);
-    case 64: return launch_split<Split<8, 8>, 32, false>(a);
-    case 512: return launch_split<Split<8, 8, 8>, 4, false>(a);
-    case 4096: return launch_split<Split<8, 8, 8, 8>, 1, false>(a);
-    default: return FFTOC_ERROR_INVALID_SIZE;
-  }
-}
-
-// Radix-16 family (legacy16_fft): N = 16^d.
-int launch_radix16(int n, const LaunchArgs &a) {
-  switch (n) {
-    case 16: return launch_split<Split<16>, 64, true>(a);
-    case 256: return launch_split<Split<16, 16>, 8, false>(a);
-    case 4096: return launch_split<Split<16, 16, 16>, 1, false>(a);
-    default: return FFTOC_ERROR_INVALID_SIZE;
-  }
-}
-
-// Mixed radix for every power of two 8..4096 (largest radices first).
-int launch_radix_mixed(int n, const LaunchArgs &a) {
-  switch (n) {
-    case 8: return launch_split<Split<8>, 128, true>(a);
-    case 16: return launch_split<Split<16>, 64, true>(a);
-    case 32: return launch_split<Split<8, 4>, 32, true>(a);
-    case 64: return launch_split<Split<8, 8>, 32, false>(a);
-    case 128: return launch_split<Split<16, 8>, 16, false>(a);
-    case 256: return launch_split<Split<16, 16>, 8, false>(a);
-    case 512: return launch_split<Split<8, 8, 8>, 4, false>(a);
-    case 1024: return launch_split<Split<16, 16, 4>, 2, false>(a);
-    case 2048: return launch_split<Split<16, 16, 8>, 1, false>(a);
-    case 4096: return launch_split<Split<16, 16, 16>, 1, false>(a);
-    default: return FFTOC_ERROR_INVALID_SIZE;
+    case 8: return set_out(kV8, out);
+    case 16: return set_out(kV16, out);
+    case 32: return set_out(kV32, out);
+    case 64: return set_out(kV64, out);
+    case 128: return set_out(kV128, out);
+    case 256: return set_out(kV256, out);
+    case 512: return set_out(kV512, out);
+    case 1024: return set_out(kV1024, out);
+    case 2048: return set_out(kV2048, out);
+    case 4096: return set_out(kV4096, out);
+    default: *out = nullptr; return 0;
   }
 }
 

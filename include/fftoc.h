@@ -116,6 +116,29 @@ int fftoc_legacy16_fft(const void *in, void *out, int64_t batch, int n,
 int fftoc_fft(const void *in, void *out, int64_t batch, int n, uint32_t flags,
               void *stream);
 
+/* ---- kernel variants and tuning (no counterpart in the reference, whose
+ * UPB/FPU choice was a compile-time sweep, scripts/generate_bench.py:93-107,
+ * with results kept in scripts/results/configs.csv) ------------------------ */
+/* Number of kernel variants for length n (0 if n is unsupported). */
+int fftoc_variant_count(int n);
+/* Stable name of variant v of length n ("tensor:..." or "radix:..."). */
+const char *fftoc_variant_name(int n, int variant);
+/* Runs one specific variant (same conventions as fftoc_fft). */
+int fftoc_fft_variant(const void *in, void *out, int64_t batch, int n,
+                      uint32_t flags, int variant, void *stream);
+/* Variant used by fftoc_fft for length n, and its override. */
+int fftoc_get_variant(int n);
+int fftoc_set_variant(int n, int variant);
+/* Times every variant of length n over `batch` signals (allocates and frees
+ * two device scratch buffers, synchronises `stream`) and makes the fastest
+ * one of each entry-point family its default. */
+int fftoc_autotune(int n, int64_t batch, int reps, void *stream);
+/* Median time in microseconds of a variant in the last autotune (<0: none). */
+double fftoc_variant_time_us(int n, int variant);
+/* Persisted tuning table (text: "<family> <n> <variant name>" per line). */
+int fftoc_tuning_save(const char *path);
+int fftoc_tuning_load(const char *path);
+
 #ifdef __cplusplus
 }
 #endif

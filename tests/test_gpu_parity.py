@@ -102,3 +102,47 @@ def test_pure_tones_land_in_their_bins(path):
     off[np.arange(64), k0] = 0
     assert np.max(np.abs(off)) < 1e-9
     assert np.max(np.abs(X[np.arange(64), k0] - n)) < 1e-9
+
+
+def _variants(n):
+    lib = fo.lib()
+    return [(v, lib.fftoc_variant_name(n, v).decode()) for v in range(lib.fftoc_variant_count(n))]
+
+
+@pytest.mark.parametrize("n", fo.ALL_SIZES)
+def test_every_variant_matches_oracle(n):
+    """Every kernel variant the tuner can pick, forward and inverse, including
+    a batch that leaves a partial last tile."""
+    x = signals(n, batch_for(n), seed=500 + n)
+    ref_f = oracle.reference(x)
+    ref_i = oracle.reference(x, inverse=True)
+    src = fo.DeviceBuffer.from_array(x)
+    dst = fo.DeviceBuffer(x.nbytes)
+    for v, name in _variants(n):
+        for inv, ref in ((False, ref_f), (True, ref_i)):
+            print(f"[variant] n={n} v={v} {name} inverse={inv}", flush=True)
+            rc = fo.lib().fftoc_fft_variant(src.ptr, dst.ptr, x.shape[0], n, fo.INVERSE if inv else 0,
+                                            v, None)
+            assert rc == 0, (name, rc)
+            fo.synchronize()
+            got = dst.download(x.shape)
+            err = np.max(oracle.rel_l2(got, ref, axis=-1))
+            assert err <= TOL, (n, name, inv, err)
+
+
+def test_autotune_and_tuning_roundtrip(tmp_path):
+    lib = fo.lib()
+    n = 256
+    assert lib.fftoc_autotune(n, 4096, 3, None) == 0
+    times = [lib.fftoc_variant_time_us(n, v) for v in range(lib.fftoc_variant_count(n))]
+    assert all(t > 0 for t in times)
+    best = lib.fftoc_get_variant(n)
+    assert times[best] == min(times)
+    path = str(tmp_path / "tune.txt").encode()
+    assert lib.fftoc_tuning_save(path) == 0
+    other = (best + 1) % len(times)
+    assert lib.fftoc_set_variant(n, other) == 0 and lib.fftoc_get_variant(n) == other
+    assert lib.fftoc_tuning_load(path) == 0
+    assert lib.fftoc_get_variant(n) == best
+    x = signals(n, 64, seed=9)
+    assert np.max(oracle.rel_l2(fo.fft_host(x), oracle.reference(x), axis=-1)) <= TOL
