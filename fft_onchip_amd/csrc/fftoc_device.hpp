@@ -1,10 +1,12 @@
-// Device-side building blocks shared by the gfx950 FFT kernels: complex FP64
-// arithmetic, fixed-root multiplications, and in-register DFT butterflies.
+// Device-side building blocks shared by the gfx950 FFT kernels: complex
+// arithmetic, fixed-root multiplications and in-register DFT butterflies,
+// generic over the complex type CT = double2 (complex128 data) or float2
+// (complex64 data).
 //
-// Conventions: a complex value is a double2 {x = re, y = im}; every butterfly
-// computes a forward DFT (exp(-2 pi i nk/R)) in place with natural-order
-// output.  Inverse transforms reuse the forward kernels by swapping re/im on
-// load and on store (swap(fwd(swap(x))) = inverse DFT of x).
+// Conventions: a complex value is {x = re, y = im}; every butterfly computes a
+// forward DFT (exp(-2 pi i nk/R)) in place with natural-order output.
+// Inverse transforms reuse the forward kernels by swapping re/im on load and
+// on store (swap(fwd(swap(x))) = inverse DFT of x).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -13,52 +15,84 @@
 namespace fftoc {
 
 using c2 = double2;
+using f2 = float2;
 
-// Correctly rounded FP64 constants of the 16th roots of unity.
-constexpr double kSqrtHalf = 0.70710678118654752440;  // cos(pi/4)
-constexpr double kCos8 = 0.92387953251128675613;      // cos(pi/8)
-constexpr double kSin8 = 0.38268343236508977173;      // sin(pi/8)
+template <class CT>
+using real_t = decltype(CT{}.x);
 
 // Twiddle table length: W_4096^j, j = 0..4095, covers every N <= 4096.
 constexpr int kTwN = 4096;
-constexpr int kTwLog = 12;
 
-__device__ __forceinline__ c2 cadd(c2 a, c2 b) { return make_double2(a.x + b.x, a.y + b.y); }
-__device__ __forceinline__ c2 csub(c2 a, c2 b) { return make_double2(a.x - b.x, a.y - b.y); }
-__device__ __forceinline__ c2 cmul(c2 a, c2 w) {
-  return make_double2(__fma_rn(a.x, w.x, -a.y * w.y), __fma_rn(a.x, w.y, a.y * w.x));
+// Correctly rounded constants of the 16th roots of unity in type R.
+template <class R>
+struct Roots {
+  static constexpr R sqrt_half = (R)0.70710678118654752440L;  // cos(pi/4)
+  static constexpr R cos8 = (R)0.92387953251128675613L;       // cos(pi/8)
+  static constexpr R sin8 = (R)0.38268343236508977173L;       // sin(pi/8)
+};
+
+template <class CT>
+__device__ __forceinline__ CT mk(real_t<CT> re, real_t<CT> im) {
+  CT v;
+  v.x = re;
+  v.y = im;
+  return v;
 }
-__device__ __forceinline__ c2 cswap(c2 a) { return make_double2(a.y, a.x); }
-__device__ __forceinline__ c2 cscale(c2 a, double s) { return make_double2(a.x * s, a.y * s); }
+
+template <class CT>
+__device__ __forceinline__ CT cadd(CT a, CT b) { return mk<CT>(a.x + b.x, a.y + b.y); }
+template <class CT>
+__device__ __forceinline__ CT csub(CT a, CT b) { return mk<CT>(a.x - b.x, a.y - b.y); }
+template <class CT>
+__device__ __forceinline__ CT cmul(CT a, CT w) {
+  return mk<CT>(fma(a.x, w.x, -a.y * w.y), fma(a.x, w.y, a.y * w.x));
+}
+template <class CT>
+__device__ __forceinline__ CT cswap(CT a) { return mk<CT>(a.y, a.x); }
+template <class CT>
+__device__ __forceinline__ CT cscale(CT a, real_t<CT> s) { return mk<CT>(a.x * s, a.y * s); }
 
 // Multiplication by fixed roots W_16^j = exp(-2 pi i j / 16).
-__device__ __forceinline__ c2 mul_mi(c2 a) { return make_double2(a.y, -a.x); }  // * -i = W16^4
-__device__ __forceinline__ c2 mul_w8_1(c2 a) {  // * (1 - i)/sqrt2 = W16^2
-  return make_double2(kSqrtHalf * (a.x + a.y), kSqrtHalf * (a.y - a.x));
+template <class CT>
+__device__ __forceinline__ CT mul_mi(CT a) { return mk<CT>(a.y, -a.x); }  // * -i = W16^4
+template <class CT>
+__device__ __forceinline__ CT mul_w8_1(CT a) {  // * (1 - i)/sqrt2 = W16^2
+  constexpr real_t<CT> r = Roots<real_t<CT>>::sqrt_half;
+  return mk<CT>(r * (a.x + a.y), r * (a.y - a.x));
 }
-__device__ __forceinline__ c2 mul_w8_3(c2 a) {  // * (-1 - i)/sqrt2 = W16^6
-  return make_double2(kSqrtHalf * (a.y - a.x), -kSqrtHalf * (a.x + a.y));
+template <class CT>
+__device__ __forceinline__ CT mul_w8_3(CT a) {  // * (-1 - i)/sqrt2 = W16^6
+  constexpr real_t<CT> r = Roots<real_t<CT>>::sqrt_half;
+  return mk<CT>(r * (a.y - a.x), -r * (a.x + a.y));
 }
-__device__ __forceinline__ c2 mul_w16_1(c2 a) {  // * (c - i s)
-  return make_double2(__fma_rn(a.x, kCos8, a.y * kSin8), __fma_rn(a.y, kCos8, -a.x * kSin8));
+template <class CT>
+__device__ __forceinline__ CT mul_w16_1(CT a) {  // * (c - i s)
+  constexpr real_t<CT> c = Roots<real_t<CT>>::cos8, s = Roots<real_t<CT>>::sin8;
+  return mk<CT>(fma(a.x, c, a.y * s), fma(a.y, c, -a.x * s));
 }
-__device__ __forceinline__ c2 mul_w16_3(c2 a) {  // * (s - i c)
-  return make_double2(__fma_rn(a.x, kSin8, a.y * kCos8), __fma_rn(a.y, kSin8, -a.x * kCos8));
+template <class CT>
+__device__ __forceinline__ CT mul_w16_3(CT a) {  // * (s - i c)
+  constexpr real_t<CT> c = Roots<real_t<CT>>::cos8, s = Roots<real_t<CT>>::sin8;
+  return mk<CT>(fma(a.x, s, a.y * c), fma(a.y, s, -a.x * c));
 }
-__device__ __forceinline__ c2 mul_w16_9(c2 a) {  // * (-c + i s)
-  return make_double2(-__fma_rn(a.x, kCos8, a.y * kSin8), __fma_rn(a.x, kSin8, -a.y * kCos8));
+template <class CT>
+__device__ __forceinline__ CT mul_w16_9(CT a) {  // * (-c + i s)
+  constexpr real_t<CT> c = Roots<real_t<CT>>::cos8, s = Roots<real_t<CT>>::sin8;
+  return mk<CT>(-fma(a.x, c, a.y * s), fma(a.x, s, -a.y * c));
 }
 
 // ---- in-register DFTs (natural order in, natural order out) ---------------
-__device__ __forceinline__ void dft2(c2 &a0, c2 &a1) {
-  c2 t = csub(a0, a1);
+template <class CT>
+__device__ __forceinline__ void dft2(CT &a0, CT &a1) {
+  CT t = csub(a0, a1);
   a0 = cadd(a0, a1);
   a1 = t;
 }
 
-__device__ __forceinline__ void dft4(c2 &a0, c2 &a1, c2 &a2, c2 &a3) {
-  c2 t0 = cadd(a0, a2), t1 = csub(a0, a2);
-  c2 t2 = cadd(a1, a3), t3 = mul_mi(csub(a1, a3));
+template <class CT>
+__device__ __forceinline__ void dft4(CT &a0, CT &a1, CT &a2, CT &a3) {
+  CT t0 = cadd(a0, a2), t1 = csub(a0, a2);
+  CT t2 = cadd(a1, a3), t3 = mul_mi(csub(a1, a3));
   a0 = cadd(t0, t2);
   a2 = csub(t0, t2);
   a1 = cadd(t1, t3);
@@ -66,9 +100,10 @@ __device__ __forceinline__ void dft4(c2 &a0, c2 &a1, c2 &a2, c2 &a3) {
 }
 
 // DFT8 = radix-2 DIT over two DFT4s: X[k] = E[k] + W8^k O[k].
-__device__ __forceinline__ void dft8(c2 *a) {
-  c2 e0 = a[0], e1 = a[2], e2 = a[4], e3 = a[6];
-  c2 o0 = a[1], o1 = a[3], o2 = a[5], o3 = a[7];
+template <class CT>
+__device__ __forceinline__ void dft8(CT *a) {
+  CT e0 = a[0], e1 = a[2], e2 = a[4], e3 = a[6];
+  CT o0 = a[1], o1 = a[3], o2 = a[5], o3 = a[7];
   dft4(e0, e1, e2, e3);
   dft4(o0, o1, o2, o3);
   o1 = mul_w8_1(o1);
@@ -82,11 +117,12 @@ __device__ __forceinline__ void dft8(c2 *a) {
 
 // DFT16 = 4 x 4: x[4 n1 + n2]; DFT4 over n1, twiddle W16^{n2 k1}, DFT4 over n2;
 // X[k1 + 4 k2].
-__device__ __forceinline__ void dft16(c2 *a) {
-  c2 y[16];
+template <class CT>
+__device__ __forceinline__ void dft16(CT *a) {
+  CT y[16];
 #pragma unroll
   for (int n2 = 0; n2 < 4; ++n2) {
-    c2 b0 = a[n2], b1 = a[4 + n2], b2 = a[8 + n2], b3 = a[12 + n2];
+    CT b0 = a[n2], b1 = a[4 + n2], b2 = a[8 + n2], b3 = a[12 + n2];
     dft4(b0, b1, b2, b3);
     y[4 * n2 + 0] = b0; y[4 * n2 + 1] = b1; y[4 * n2 + 2] = b2; y[4 * n2 + 3] = b3;
   }
@@ -102,14 +138,14 @@ __device__ __forceinline__ void dft16(c2 *a) {
   y[15] = mul_w16_9(y[15]); // W^9
 #pragma unroll
   for (int k1 = 0; k1 < 4; ++k1) {
-    c2 b0 = y[k1], b1 = y[4 + k1], b2 = y[8 + k1], b3 = y[12 + k1];
+    CT b0 = y[k1], b1 = y[4 + k1], b2 = y[8 + k1], b3 = y[12 + k1];
     dft4(b0, b1, b2, b3);
     a[k1] = b0; a[k1 + 4] = b1; a[k1 + 8] = b2; a[k1 + 12] = b3;
   }
 }
 
-template <int R>
-__device__ __forceinline__ void dft(c2 *a) {
+template <int R, class CT>
+__device__ __forceinline__ void dft(CT *a) {
   if constexpr (R == 2) {
     dft2(a[0], a[1]);
   } else if constexpr (R == 4) {
@@ -124,26 +160,26 @@ __device__ __forceinline__ void dft(c2 *a) {
 
 // Multiplies a[m] (m = 1..R-1) by W^{m k}, where tw[idx * m] = W^{m k}: only
 // W^k, W^2k, W^4k, W^8k are read from the table, the other powers are
-// products of at most four table values (error a few FP64 ulp).
-template <int R>
-__device__ __forceinline__ void apply_twiddles(c2 *a, const c2 *__restrict__ tw, int idx) {
+// products of at most four table values (error a few ulp).
+template <int R, class CT>
+__device__ __forceinline__ void apply_twiddles(CT *a, const CT *__restrict__ tw, int idx) {
   if constexpr (R == 2) {
     a[1] = cmul(a[1], tw[idx]);
   } else {
-    const c2 w1 = tw[idx], w2 = tw[2 * idx];
-    const c2 w3 = cmul(w2, w1);
+    const CT w1 = tw[idx], w2 = tw[2 * idx];
+    const CT w3 = cmul(w2, w1);
     a[1] = cmul(a[1], w1);
     a[2] = cmul(a[2], w2);
     a[3] = cmul(a[3], w3);
     if constexpr (R >= 8) {
-      const c2 w4 = tw[4 * idx];
+      const CT w4 = tw[4 * idx];
       a[4] = cmul(a[4], w4);
       a[5] = cmul(a[5], cmul(w4, w1));
       a[6] = cmul(a[6], cmul(w4, w2));
-      const c2 w7 = cmul(w4, w3);
+      const CT w7 = cmul(w4, w3);
       a[7] = cmul(a[7], w7);
       if constexpr (R == 16) {
-        const c2 w8 = tw[8 * idx];
+        const CT w8 = tw[8 * idx];
         a[8] = cmul(a[8], w8);
         a[9] = cmul(a[9], cmul(w8, w1));
         a[10] = cmul(a[10], cmul(w8, w2));
@@ -160,14 +196,19 @@ __device__ __forceinline__ void apply_twiddles(c2 *a, const c2 *__restrict__ tw,
 // Streaming (non-temporal) global access for data touched exactly once:
 // keeps the input/output stream from evicting the twiddle table and other
 // reused lines from L2 / MALL.
-typedef double dvec2 __attribute__((ext_vector_type(2)));
-__device__ __forceinline__ c2 ld_stream(const c2 *p) {
-  const dvec2 v = __builtin_nontemporal_load(reinterpret_cast<const dvec2 *>(p));
-  return make_double2(v.x, v.y);
+template <class CT>
+__device__ __forceinline__ CT ld_stream(const CT *p) {
+  typedef real_t<CT> R;
+  typedef R vec2 __attribute__((ext_vector_type(2)));
+  const vec2 v = __builtin_nontemporal_load(reinterpret_cast<const vec2 *>(p));
+  return mk<CT>(v.x, v.y);
 }
-__device__ __forceinline__ void st_stream(c2 *p, c2 v) {
-  const dvec2 t = {v.x, v.y};
-  __builtin_nontemporal_store(t, reinterpret_cast<dvec2 *>(p));
+template <class CT>
+__device__ __forceinline__ void st_stream(CT *p, CT v) {
+  typedef real_t<CT> R;
+  typedef R vec2 __attribute__((ext_vector_type(2)));
+  const vec2 t = {v.x, v.y};
+  __builtin_nontemporal_store(t, reinterpret_cast<vec2 *>(p));
 }
 
 // Wave-scope ordering point for LDS exchanges among the lanes of one wave:

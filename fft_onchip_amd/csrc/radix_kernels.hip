@@ -27,216 +27,11 @@
 //  * Twiddles W_4096^j come from a 64 KiB FP64 table built on the host in
 //    long double (L2/L1 resident); fixed roots inside the butterflies are
 //    exact-rounded constants.
-#include "fftoc_device.hpp"
 #include "fftoc_internal.hpp"
-
-#include <utility>
+#include "stockham.hpp"
 
 namespace fftoc {
 namespace {
-
-template <int... Rs>
-struct Split {
-  static constexpr int P = sizeof...(Rs);
-  static constexpr int R[P] = {Rs...};
-  static constexpr int N = (Rs * ...);
-  static constexpr int RM = [] {
-    int m = 1;
-    for (int r : {Rs...}) m = r > m ? r : m;
-    return m;
-  }();
-  static constexpr int L(int p) {
-    int l = 1;
-    for (int i = 0; i < p; ++i) l *= R[i];
-    return l;
-  }
-};
-
-// LDS element index of tile element E = s*N + e: one pad slot after every 16
-// elements keeps strided butterfly accesses on distinct banks.
-__device__ __forceinline__ int lds_index(int E) { return E + (E >> 4); }
-constexpr int lds_elems(int n) { return n + (n >> 4); }
-
-template <class SP, int S, bool INV, bool STAGED>
-struct StockhamKernel {
-  static constexpr int N = SP::N;
-  static constexpr int RM = SP::RM;
-  static constexpr int T = N / RM;
-  static constexpr int THREADS = T * S;
-  static constexpr bool WAVE_LOCAL = (T <= 64) && (64 % T == 0);
-
-  __device__ static void sync() {
-    if constexpr (WAVE_LOCAL) {
-      wave_lds_sync();
-    } else {
-      __syncthreads();
-    }
-  }
-
-  // Pass-0 inputs straight from HBM (non-staged kernels).
-  __device__ static void load_first(c2 (&reg)[RM], const c2 *gin, int t, bool valid) {
-    constexpr int R = SP::R[0];
-    constexpr int NB = RM / R;
-#pragma unroll
-    for (int i = 0; i < NB; ++i) {
-#pragma unroll
-      for (int m = 0; m < R; ++m) {
-        c2 v = valid ? ld_stream(gin + t + i * T + m * (N / R)) : make_double2(0.0, 0.0);
-        if constexpr (INV) v = cswap(v);
-        reg[i * R + m] = v;
-      }
-    }
-  }
-
-  // One Stockham pass.  IN_REGS: inputs already in reg (pass 0 of the
-  // non-staged kernels); otherwise they are gathered from LDS.
-  template <int p, bool IN_REGS>
-  __device__ static void pass(c2 (&reg)[RM], c2 *lds, c2 *gout, int t, int sbase, bool valid,
-                              const c2 *__restrict__ tw, double scale) {
-    constexpr int R = SP::R[p];
-    constexpr int L = SP::L(p);
-    constexpr int NB = RM / R;
-    constexpr bool LAST = (p == SP::P - 1);
-
-    if constexpr (!IN_REGS) {
-#pragma unroll
-      for (int i = 0; i < NB; ++i) {
-        const int j = t + i * T;
-#pragma unroll
-        for (int m = 0; m < R; ++m) reg[i * R + m] = lds[lds_index(sbase + j + m * (N / R))];
-      }
-      // all lanes have read this pass's inputs before anyone overwrites them
-      sync();
-    }
-
-    // ---- twiddle W_{R L}^{m k}, k = j mod L, and butterfly
-#pragma unroll
-    for (int i = 0; i < NB; ++i) {
-      const int j = t + i * T;
-      if constexpr (L > 1) {
-        const int k = j & (L - 1);
-        constexpr int step = kTwN / (R * L);
-        apply_twiddles<R>(&reg[i * R], tw, k * step);
-      }
-      dft<R>(&reg[i * R]);
-    }
-
-    // ---- scatter outputs
-#pragma unroll
-    for (int i = 0; i < NB; ++i) {
-      const int j = t + i * T;
-      const int k = j & (L - 1);
-#pragma unroll
-      for (int q = 0; q < R; ++q) {
-        const int d = (j - k) * R + k + q * L;
-        c2 v = reg[i * R + q];
-        if constexpr (LAST && !STAGED) {
-          if (scale != 1.0) v = cscale(v, scale);
-          if constexpr (INV) v = cswap(v);
-          if (valid) st_stream(gout + d, v);
-        } else {
-          lds[lds_index(sbase + d)] = v;
-        }
-      }
-    }
-    if constexpr (!LAST || STAGED) sync();
-  }
-
-  template <int... ps>
-  __device__ static void later_passes(std::integer_sequence<int, ps...>, c2 (&reg)[RM], c2 *lds,
-                                      c2 *gout, int t, int sbase, bool valid,
-                                      const c2 *__restrict__ tw, double scale) {
-    (pass<ps + 1, false>(reg, lds, gout, t, sbase, valid, tw, scale), ...);
-  }
-
-  // One tile per workgroup.
-  __device__ static void run(const c2 *in, c2 *out, int64_t batch, const c2 *__restrict__ tw,
-                             double scale) {
-    __shared__ c2 lds[lds_elems(S * N)];
-    const int tid = threadIdx.x;
-    const int t = tid % T;
-    const int sl = tid / T;
-    const int64_t tile0 = (int64_t)blockIdx.x * S;
-    const int64_t sig = tile0 + sl;
-    const bool valid = sig < batch;
-    c2 reg[RM];
-
-    if constexpr (STAGED) {
-      // contiguous tile copy HBM -> LDS (16 B per lane per instruction)
-      const int64_t nvalid = (batch - tile0) < S ? (batch - tile0) : S;
-      const c2 *src = in + tile0 * N;
-#pragma unroll
-      for (int c = 0; c < (S * N) / THREADS; ++c) {
-        const int E = tid + c * THREADS;
-        c2 v = (E < nvalid * N) ? ld_stream(src + E) : make_double2(0.0, 0.0);
-        if constexpr (INV) v = cswap(v);
-        lds[lds_index(E)] = v;
-      }
-      __syncthreads();
-      pass<0, false>(reg, lds, nullptr, t, sl * N, valid, tw, scale);
-    } else {
-      load_first(reg, in + sig * N, t, valid);
-      pass<0, true>(reg, lds, out + sig * N, t, sl * N, valid, tw, scale);
-    }
-    later_passes(std::make_integer_sequence<int, SP::P - 1>{}, reg, lds, out + sig * N, t, sl * N,
-                 valid, tw, scale);
-
-    if constexpr (STAGED) {
-      __syncthreads();
-      const int64_t nvalid = (batch - tile0) < S ? (batch - tile0) : S;
-      c2 *dst = out + tile0 * N;
-#pragma unroll
-      for (int c = 0; c < (S * N) / THREADS; ++c) {
-        const int E = tid + c * THREADS;
-        c2 v = lds[lds_index(E)];
-        if (scale != 1.0) v = cscale(v, scale);
-        if constexpr (INV) v = cswap(v);
-        if (E < nvalid * N) st_stream(dst + E, v);
-      }
-    }
-  }
-
-  // Persistent workgroups: tiles blockIdx.x, blockIdx.x + gridDim.x, ...;
-  // the next tile's HBM loads overlap the current tile's LDS passes.
-  __device__ static void run_persistent(const c2 *in, c2 *out, int64_t batch,
-                                        const c2 *__restrict__ tw, double scale) {
-    static_assert(!STAGED && SP::P >= 2, "persistent form needs an LDS pass");
-    __shared__ c2 lds[lds_elems(S * N)];
-    const int tid = threadIdx.x;
-    const int t = tid % T;
-    const int sl = tid / T;
-    const int64_t ntiles = (batch + S - 1) / S;
-    int64_t tile = blockIdx.x;
-    if (tile >= ntiles) return;  // uniform over the workgroup
-    c2 reg[RM];
-    {
-      const int64_t sig = tile * S + sl;
-      load_first(reg, in + sig * N, t, sig < batch);
-    }
-    for (;;) {
-      const int64_t sig = tile * S + sl;
-      const bool valid = sig < batch;
-      const int64_t next = tile + gridDim.x;
-      // Re-read the twiddles every tile instead of letting the compiler hoist
-      // all of them out of the loop (they would pin ~120 VGPRs and halve the
-      // occupancy); the empty asm only hides the pointer's loop invariance.
-      const c2 *twl = tw;
-      asm volatile("" : "+s"(twl));
-      pass<0, true>(reg, lds, out + sig * N, t, sl * N, valid, twl, scale);
-      c2 nxt[RM];
-      if (next < ntiles) {
-        const int64_t nsig = next * S + sl;
-        load_first(nxt, in + nsig * N, t, nsig < batch);
-      }
-      later_passes(std::make_integer_sequence<int, SP::P - 1>{}, reg, lds, out + sig * N, t,
-                   sl * N, valid, twl, scale);
-      if (next >= ntiles) break;
-      tile = next;
-#pragma unroll
-      for (int i = 0; i < RM; ++i) reg[i] = nxt[i];
-    }
-  }
-};
 
 template <class SP, int S>
 constexpr int kThreads = (SP::N / SP::RM) * S;
