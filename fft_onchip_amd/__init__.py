@@ -26,6 +26,7 @@ NORMALIZE = 0x2
 PREC_FP64 = 0x00
 PREC_FP32 = 0x10
 PREC_BF16X3 = 0x20
+DATA_C64 = 0x100
 _PREC = {"fp64": PREC_FP64, "fp32": PREC_FP32, "bf16x3": PREC_BF16X3}
 
 STATUS = {
@@ -130,13 +131,16 @@ _PATHS = {"tensor": tensor_fft, "legacy8": legacy_fft, "legacy16": legacy16_fft,
 
 def fft_host(x: np.ndarray, path: str = "best", inverse: bool = False, normalize: bool = False,
              precision: str = "fp64", in_place: bool = False) -> np.ndarray:
-    """Uploads complex128 signals (batch, n), transforms them on the GPU and
-    returns the result as a new host array."""
-    x = np.ascontiguousarray(x, dtype=np.complex128)
+    """Uploads signals (batch, n) -- complex128, or complex64 for FP32 data --
+    transforms them on the GPU and returns the result as a new host array."""
+    x = np.asarray(x)
+    if x.dtype != np.complex64:
+        x = x.astype(np.complex128)
+    x = np.ascontiguousarray(x)
     if x.ndim == 1:
         x = x.reshape(1, -1)
     batch, n = x.shape
-    flags = make_flags(inverse, normalize, precision)
+    flags = make_flags(inverse, normalize, precision) | (DATA_C64 if x.dtype == np.complex64 else 0)
     src = DeviceBuffer.from_array(x)
     if in_place:
         _PATHS[path](src, batch, n, None, flags)
@@ -145,7 +149,53 @@ def fft_host(x: np.ndarray, path: str = "best", inverse: bool = False, normalize
         res = DeviceBuffer(x.nbytes)
         _PATHS[path](src, batch, n, res, flags)
     synchronize()
-    return res.download(x.shape)
+    return res.download(x.shape, dtype=x.dtype)
+
+
+class Plan:
+    """Mixed-size batch plan (fftoc_plan_*): one persistent kernel launch over
+    many jobs.  jobs: iterable of (in_buf, out_buf, batch, n, flags)."""
+
+    def __init__(self, jobs):
+        self._jobs = list(jobs)
+        arr = (_native.Job * max(1, len(self._jobs)))()
+        for i, (src, dst, batch, n, flags) in enumerate(self._jobs):
+            arr[i] = _native.Job(_ptr(src), _ptr(dst) if dst is not None else _ptr(src),
+                                 int(batch), int(n), int(flags))
+        handle = ctypes.c_void_p()
+        rc = lib().fftoc_plan_create(arr, len(self._jobs), ctypes.byref(handle))
+        if rc != 0:
+            raise FftocError(rc, "fftoc_plan_create")
+        self.handle = handle.value
+
+    def execute(self, stream=None, graph: bool = False) -> None:
+        fn = lib().fftoc_plan_execute_graph if graph else lib().fftoc_plan_execute
+        rc = fn(self.handle, _stream(stream))
+        if rc != 0:
+            raise FftocError(rc, "fftoc_plan_execute")
+
+    def autotune(self, reps: int = 3, stream=None) -> None:
+        rc = lib().fftoc_plan_autotune(self.handle, int(reps), _stream(stream))
+        if rc != 0:
+            raise FftocError(rc, "fftoc_plan_autotune")
+
+    def routine(self, n: int, c64: bool = False) -> str:
+        name = ctypes.c_char_p()
+        rc = lib().fftoc_plan_routine(self.handle, int(n), DATA_C64 if c64 else 0, ctypes.byref(name))
+        if rc != 0:
+            raise FftocError(rc, "fftoc_plan_routine")
+        return name.value.decode()
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            lib().fftoc_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def init() -> None:

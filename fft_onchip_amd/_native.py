@@ -39,6 +39,21 @@ SIGNATURES.update({
     "fftoc_tuning_save": ([ctypes.c_char_p], _int),
     "fftoc_tuning_load": ([ctypes.c_char_p], _int),
 })
+
+
+class Job(ctypes.Structure):
+    """struct fftoc_job (include/fftoc.h)."""
+    _fields_ = [("in_", _vp), ("out", _vp), ("batch", _i64), ("n", _int), ("flags", _u32)]
+
+
+SIGNATURES.update({
+    "fftoc_plan_create": ([ctypes.POINTER(Job), _int, ctypes.POINTER(_vp)], _int),
+    "fftoc_plan_execute": ([_vp, _vp], _int),
+    "fftoc_plan_execute_graph": ([_vp, _vp], _int),
+    "fftoc_plan_autotune": ([_vp, _int, _vp], _int),
+    "fftoc_plan_routine": ([_vp, _int, _u32, ctypes.POINTER(ctypes.c_char_p)], _int),
+    "fftoc_plan_destroy": ([_vp], _int),
+})
 TENSOR_SIZES = (8, 16, 64, 128, 256, 512, 4096)
 for _n in TENSOR_SIZES:
     SIGNATURES[f"fftoc_tensor_fft_{_n}"] = ([_vp, _vp, _i64, _u32, _vp], _int)

@@ -22,7 +22,8 @@ constexpr int kMaxDevices = 64;
 struct DeviceState {
   bool ready = false;
   int status = FFTOC_SUCCESS;
-  void *twiddles = nullptr;  // kTwN x double2
+  void *twiddles = nullptr;    // kTwN x double2
+  void *twiddles32 = nullptr;  // kTwN x float2, rounded from the FP64 table
 };
 
 std::mutex g_mutex;
@@ -82,13 +83,22 @@ int init_current_device(int *dev_out) {
     return FFTOC_ERROR_DEVICE;
   }
   std::vector<double> tw = build_twiddles();
-  void *d = nullptr;
+  std::vector<float> tw32(tw.begin(), tw.end());
+  void *d = nullptr, *d32 = nullptr;
   if (hipMalloc(&d, tw.size() * sizeof(double)) != hipSuccess) return FFTOC_ERROR_ALLOC;
-  if (hipMemcpy(d, tw.data(), tw.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+  if (hipMalloc(&d32, tw32.size() * sizeof(float)) != hipSuccess) {
     (void)hipFree(d);
+    return FFTOC_ERROR_ALLOC;
+  }
+  if (hipMemcpy(d, tw.data(), tw.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(d32, tw32.data(), tw32.size() * sizeof(float), hipMemcpyHostToDevice) !=
+          hipSuccess) {
+    (void)hipFree(d);
+    (void)hipFree(d32);
     return FFTOC_ERROR_DEVICE;
   }
   st.twiddles = d;
+  st.twiddles32 = d32;
   st.status = FFTOC_SUCCESS;
   st.ready = true;
   return FFTOC_SUCCESS;
@@ -117,7 +127,7 @@ struct SizeRegistry {
   std::vector<double> tuned_us;  // last autotune time per variant (<0: not run)
 };
 
-SizeRegistry g_reg[13];
+SizeRegistry g_reg[13][2];  // [log2 n][complex64 data]
 std::once_flag g_reg_once;
 
 // Default choice of fftoc_fft per length, from MI355X measurements of the
@@ -139,11 +149,11 @@ const char *default_best(int n) {
 }
 
 void build_registry() {
-  for (int k = 3; k <= 12; ++k) {
+  for (int k = 3; k <= 12; ++k) for (int c64 = 0; c64 < 2; ++c64) {
     const int n = 1 << k;
-    SizeRegistry &r = g_reg[k];
+    SizeRegistry &r = g_reg[k][c64];
     const Variant *t = nullptr, *x = nullptr;
-    const int nt = tensor_variants(n, &t), nx = radix_variants(n, &x);
+    const int nt = tensor_variants(n, c64 != 0, &t), nx = radix_variants(n, c64 != 0, &x);
     for (int i = 0; i < nt; ++i) r.v.push_back(t[i]);
     for (int i = 0; i < nx; ++i) r.v.push_back(x[i]);
     r.tuned_us.assign(r.v.size(), -1.0);
@@ -156,7 +166,7 @@ void build_registry() {
           break;
         }
       }
-      if (f == Family::Best) {
+      if (f == Family::Best && !c64) {
         for (size_t i = 0; i < r.v.size(); ++i)
           if (std::strcmp(r.v[i].name, default_best(n)) == 0) pick = (int)i;
       }
@@ -165,15 +175,16 @@ void build_registry() {
   }
 }
 
-SizeRegistry *registry(int n) {
+SizeRegistry *registry(int n, bool c64 = false) {
   if (!is_pow2(n) || n < 8 || n > 4096) return nullptr;
   std::call_once(g_reg_once, build_registry);
-  return &g_reg[log2i(n)];
+  return &g_reg[log2i(n)][c64 ? 1 : 0];
 }
 
 int check_and_fill(const void *in, void *out, int64_t batch, int n, uint32_t flags,
                    void *stream, LaunchArgs &a) {
-  const uint32_t known = FFTOC_INVERSE | FFTOC_NORMALIZE | FFTOC_PREC_MASK | FFTOC_TILE_MASK;
+  const uint32_t known =
+      FFTOC_INVERSE | FFTOC_NORMALIZE | FFTOC_PREC_MASK | FFTOC_DATA_C64 | FFTOC_TILE_MASK;
   if (flags & ~known) return FFTOC_ERROR_INVALID_FLAGS;
   const uint32_t prec = flags & FFTOC_PREC_MASK;
   if (prec != FFTOC_PREC_FP64 && prec != FFTOC_PREC_FP32 && prec != FFTOC_PREC_BF16X3)
@@ -182,10 +193,11 @@ int check_and_fill(const void *in, void *out, int64_t batch, int n, uint32_t fla
   if (batch < 0) return FFTOC_ERROR_INVALID_BATCH;
   if (batch > 0) {
     if (!in || !out) return FFTOC_ERROR_NULL_POINTER;
-    if ((reinterpret_cast<uintptr_t>(in) | reinterpret_cast<uintptr_t>(out)) & 15u)
+    const uint64_t esize = (flags & FFTOC_DATA_C64) ? 8u : 16u;
+    if ((reinterpret_cast<uintptr_t>(in) | reinterpret_cast<uintptr_t>(out)) & (esize - 1u))
       return FFTOC_ERROR_MISALIGNED;
-    const uint64_t bytes = (uint64_t)batch * (uint64_t)n * 16u;
-    if (bytes / 16u / (uint64_t)n != (uint64_t)batch) return FFTOC_ERROR_INVALID_BATCH;
+    const uint64_t bytes = (uint64_t)batch * (uint64_t)n * esize;
+    if (bytes / esize / (uint64_t)n != (uint64_t)batch) return FFTOC_ERROR_INVALID_BATCH;
     const uintptr_t i0 = reinterpret_cast<uintptr_t>(in), o0 = reinterpret_cast<uintptr_t>(out);
     if (i0 != o0 && i0 < o0 + bytes && o0 < i0 + bytes) return FFTOC_ERROR_OVERLAP;
   }
@@ -198,6 +210,7 @@ int check_and_fill(const void *in, void *out, int64_t batch, int n, uint32_t fla
   a.tile = (int)((flags & FFTOC_TILE_MASK) >> FFTOC_TILE_SHIFT);
   a.precision = prec;
   a.twiddles = nullptr;
+  a.c64 = (flags & FFTOC_DATA_C64) != 0;
   return FFTOC_SUCCESS;
 }
 
@@ -224,15 +237,16 @@ int run(Family fam, const void *in, void *out, int64_t batch, int n, uint32_t fl
       if (a.precision != FFTOC_PREC_FP64) return FFTOC_ERROR_UNSUPPORTED;
       break;
   }
-  SizeRegistry *reg = registry(n);
+  SizeRegistry *reg = registry(n, a.c64);
   if (!reg) return FFTOC_ERROR_INVALID_SIZE;
   const int pick = variant >= 0 ? variant : reg->choice[family_index(fam)].load();
-  if (pick < 0 || pick >= (int)reg->v.size()) return FFTOC_ERROR_INVALID_SIZE;
+  if (pick < 0 || pick >= (int)reg->v.size())
+    return a.c64 ? FFTOC_ERROR_UNSUPPORTED : FFTOC_ERROR_INVALID_SIZE;
   if (batch == 0) return FFTOC_SUCCESS;
   int dev = 0;
   rc = init_current_device(&dev);
   if (rc != FFTOC_SUCCESS) return rc;
-  a.twiddles = g_dev[dev].twiddles;
+  a.twiddles = a.c64 ? g_dev[dev].twiddles32 : g_dev[dev].twiddles;
   return reg->v[pick].launch(a);
 }
 
@@ -261,7 +275,7 @@ int autotune(int n, int64_t batch, int reps, void *stream_p) {
     rc = FFTOC_ERROR_DEVICE;
   std::vector<double> us(reg->v.size(), -1.0);
   for (size_t i = 0; i < reg->v.size() && rc == FFTOC_SUCCESS; ++i) {
-    LaunchArgs a{in, out, batch, g_dev[dev].twiddles, stream, false, 1.0, 0, FFTOC_PREC_FP64};
+    LaunchArgs a{in, out, batch, g_dev[dev].twiddles, stream, false, 1.0, 0, FFTOC_PREC_FP64, false};
     if (reg->v[i].launch(a) != FFTOC_SUCCESS) continue;  // warm-up
     std::vector<float> t;
     for (int r = 0; r < reps; ++r) {
@@ -299,6 +313,19 @@ int autotune(int n, int64_t batch, int reps, void *stream_p) {
 const char *kFamilyNames[4] = {"tensor", "legacy8", "legacy16", "fft"};
 
 }  // namespace
+
+int device_tables(const void **tw64, const void **tw32) {
+  int dev = 0;
+  const int rc = init_current_device(&dev);
+  if (rc != FFTOC_SUCCESS) return rc;
+  *tw64 = g_dev[dev].twiddles;
+  *tw32 = g_dev[dev].twiddles32;
+  return FFTOC_SUCCESS;
+}
+
+namespace {
+
+}  // namespace
 }  // namespace fftoc
 
 using fftoc::Family;
@@ -316,9 +343,10 @@ int fftoc_shutdown(void) {
   (void)hipGetDevice(&cur);
   for (int d = 0; d < fftoc::kMaxDevices; ++d) {
     fftoc::DeviceState &st = fftoc::g_dev[d];
-    if (st.twiddles) {
+    if (st.twiddles || st.twiddles32) {
       (void)hipSetDevice(d);
-      (void)hipFree(st.twiddles);
+      if (st.twiddles) (void)hipFree(st.twiddles);
+      if (st.twiddles32) (void)hipFree(st.twiddles32);
     }
     st = fftoc::DeviceState{};
   }
@@ -394,7 +422,7 @@ const char *fftoc_variant_name(int n, int variant) {
 
 int fftoc_fft_variant(const void *in, void *out, int64_t batch, int n, uint32_t flags,
                       int variant, void *stream) {
-  fftoc::SizeRegistry *r = fftoc::registry(n);
+  fftoc::SizeRegistry *r = fftoc::registry(n, (flags & FFTOC_DATA_C64) != 0);
   if (!r) return FFTOC_ERROR_INVALID_SIZE;
   if (variant < 0 || variant >= (int)r->v.size()) return FFTOC_ERROR_INVALID_HANDLE;
   const Family fam = (r->v[variant].families & fftoc::FAM_TENSOR) ? Family::Tensor : Family::Best;

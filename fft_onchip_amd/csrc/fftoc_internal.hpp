@@ -12,13 +12,18 @@ struct LaunchArgs {
   const void *in;
   void *out;
   int64_t batch;
-  const void *twiddles;  // device table W_4096^j, j = 0..4095 (double2)
+  const void *twiddles;  // device table W_4096^j, j = 0..4095 (double2, or
+                         // float2 for complex64 data)
   hipStream_t stream;
   bool inverse;
   double scale;          // 1.0, or 1/N with FFTOC_NORMALIZE
   int tile;              // signals per workgroup override (0 = default)
   uint32_t precision;    // FFTOC_PREC_* bits
+  bool c64;              // complex64 data (FFTOC_DATA_C64)
 };
+
+// Per-device tables shared with the plan / serving code (api.cpp).
+int device_tables(const void **tw64, const void **tw32);
 
 // A kernel variant for one transform length: name, the entry-point families
 // it may serve, and its launcher.
@@ -36,8 +41,9 @@ struct Variant {
 };
 
 // radix_kernels.hip / mfma_kernels.hip: variants available for length n,
-// preferred (default) variant of each family first.
-int radix_variants(int n, const Variant **out);
-int tensor_variants(int n, const Variant **out);
+// preferred (default) variant of each family first.  c64: variants for
+// complex64 data (FP32 arithmetic).
+int radix_variants(int n, bool c64, const Variant **out);
+int tensor_variants(int n, bool c64, const Variant **out);
 
 }  // namespace fftoc

@@ -484,7 +484,11 @@ int set_out(const Variant (&v)[K], const Variant **out) {
 
 }  // namespace
 
-int tensor_variants(int n, const Variant **out) {
+int tensor_variants(int n, bool c64, const Variant **out) {
+  if (c64) {
+    *out = nullptr;
+    return 0;
+  }
   switch (n) {
     case 8: return set_out(kT8, out);
     case 16: return set_out(kT16, out);

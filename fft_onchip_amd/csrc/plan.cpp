@@ -1,0 +1,283 @@
+// Mixed-size batch plans: many FFT jobs of different length, direction and
+// data type executed by ONE persistent kernel launch (batch_kernels.hip),
+// optionally replayed from a captured hipGraph, with the tile routine of each
+// (length, data type) picked by an autotuner.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <new>
+#include <vector>
+
+#include "batch_internal.hpp"
+
+struct fftoc_plan_t {
+  int device = 0;
+  std::vector<fftoc_job> jobs;          // as given by the caller
+  std::vector<int> order;               // device order (heaviest items first)
+  std::vector<fftoc::JobDev> host;      // descriptors in device order
+  fftoc::JobDev *d_jobs = nullptr;
+  unsigned long long *d_ctr = nullptr;  // [0] next item, [1] finished workgroups
+  int64_t total_items = 0;
+  int grid = 0;
+  int routine[13][2];                   // [log2 n][complex64] -> routine id
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  const void *tw64 = nullptr, *tw32 = nullptr;
+};
+
+namespace fftoc {
+namespace {
+
+bool pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
+int lg(int n) { return 31 - __builtin_clz((unsigned)n); }
+
+int validate_job(const fftoc_job &j) {
+  const uint32_t known = FFTOC_INVERSE | FFTOC_NORMALIZE | FFTOC_DATA_C64;
+  if (j.flags & FFTOC_PREC_MASK) return FFTOC_ERROR_UNSUPPORTED;
+  if (j.flags & ~known) return FFTOC_ERROR_INVALID_FLAGS;
+  if (!pow2(j.n) || j.n < 8 || j.n > 4096) return FFTOC_ERROR_INVALID_SIZE;
+  if (j.batch < 0) return FFTOC_ERROR_INVALID_BATCH;
+  if (j.batch == 0) return FFTOC_SUCCESS;
+  if (!j.in || !j.out) return FFTOC_ERROR_NULL_POINTER;
+  const uint64_t es = (j.flags & FFTOC_DATA_C64) ? 8u : 16u;
+  const uintptr_t i0 = reinterpret_cast<uintptr_t>(j.in), o0 = reinterpret_cast<uintptr_t>(j.out);
+  if ((i0 | o0) & (es - 1)) return FFTOC_ERROR_MISALIGNED;
+  const uint64_t bytes = (uint64_t)j.batch * (uint64_t)j.n * es;
+  if (bytes / es / (uint64_t)j.n != (uint64_t)j.batch) return FFTOC_ERROR_INVALID_BATCH;
+  if (i0 != o0 && i0 < o0 + bytes && o0 < i0 + bytes) return FFTOC_ERROR_OVERLAP;
+  return FFTOC_SUCCESS;
+}
+
+int first_routine(int n) {
+  for (int id = 0; id < batch_routine_count(); ++id)
+    if (batch_routine_n(id) == n) return id;
+  return -1;
+}
+
+// Lays the jobs out as work items and uploads the descriptors.
+int layout(fftoc_plan_t *p) {
+  p->host.clear();
+  p->total_items = 0;
+  for (int idx : p->order) {
+    const fftoc_job &j = p->jobs[idx];
+    if (j.batch == 0) continue;
+    const bool c64 = (j.flags & FFTOC_DATA_C64) != 0;
+    JobDev d{};
+    d.in = j.in;
+    d.out = j.out;
+    d.batch = j.batch;
+    d.routine = p->routine[lg(j.n)][c64 ? 1 : 0];
+    const int S = batch_routine_signals(d.routine);
+    d.ntiles = (j.batch + S - 1) / S;
+    // about 64 KiB of input per claimed item
+    const int64_t tile_bytes = (int64_t)S * j.n * (c64 ? 8 : 16);
+    d.tiles_per_item = (int32_t)std::max<int64_t>(1, 65536 / tile_bytes);
+    d.item_begin = p->total_items;
+    d.inverse = (j.flags & FFTOC_INVERSE) ? 1 : 0;
+    d.c64 = c64 ? 1 : 0;
+    d.scale = (j.flags & FFTOC_NORMALIZE) ? 1.0 / (double)j.n : 1.0;
+    p->total_items += (d.ntiles + d.tiles_per_item - 1) / d.tiles_per_item;
+    p->host.push_back(d);
+  }
+  if (p->d_jobs) {
+    (void)hipFree(p->d_jobs);
+    p->d_jobs = nullptr;
+  }
+  if (!p->host.empty()) {
+    const size_t bytes = p->host.size() * sizeof(JobDev);
+    if (hipMalloc(&p->d_jobs, bytes) != hipSuccess) return FFTOC_ERROR_ALLOC;
+    if (hipMemcpy(p->d_jobs, p->host.data(), bytes, hipMemcpyHostToDevice) != hipSuccess)
+      return FFTOC_ERROR_DEVICE;
+  }
+  const int64_t resident = batch_resident_blocks();
+  p->grid = (int)std::max<int64_t>(1, std::min<int64_t>(resident, p->total_items));
+  if (p->exec) {
+    (void)hipGraphExecDestroy(p->exec);
+    p->exec = nullptr;
+  }
+  if (p->graph) {
+    (void)hipGraphDestroy(p->graph);
+    p->graph = nullptr;
+  }
+  return FFTOC_SUCCESS;
+}
+
+int launch(fftoc_plan_t *p, hipStream_t s) {
+  if (p->total_items == 0) return FFTOC_SUCCESS;
+  return launch_batch(p->d_jobs, (int)p->host.size(), p->total_items, p->d_ctr, p->tw64, p->tw32,
+                      p->grid, s);
+}
+
+int check_device(fftoc_plan_t *p) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess) return FFTOC_ERROR_DEVICE;
+  return dev == p->device ? FFTOC_SUCCESS : FFTOC_ERROR_DEVICE;
+}
+
+}  // namespace
+}  // namespace fftoc
+
+extern "C" {
+
+int fftoc_plan_create(const fftoc_job *jobs, int njobs, fftoc_plan *out) {
+  using namespace fftoc;
+  if (!out) return FFTOC_ERROR_NULL_POINTER;
+  *out = nullptr;
+  if (njobs < 0 || (njobs > 0 && !jobs)) return FFTOC_ERROR_NULL_POINTER;
+  for (int i = 0; i < njobs; ++i) {
+    const int rc = validate_job(jobs[i]);
+    if (rc != FFTOC_SUCCESS) return rc;
+  }
+  const void *tw64 = nullptr, *tw32 = nullptr;
+  int rc = device_tables(&tw64, &tw32);
+  if (rc != FFTOC_SUCCESS) return rc;
+  fftoc_plan_t *p = new (std::nothrow) fftoc_plan_t;
+  if (!p) return FFTOC_ERROR_ALLOC;
+  (void)hipGetDevice(&p->device);
+  p->tw64 = tw64;
+  p->tw32 = tw32;
+  p->jobs.assign(jobs, jobs + njobs);
+  p->order.resize(njobs);
+  for (int i = 0; i < njobs; ++i) p->order[i] = i;
+  // heaviest items first: longer transforms, then bigger batches
+  std::stable_sort(p->order.begin(), p->order.end(), [&](int a, int b) {
+    if (p->jobs[a].n != p->jobs[b].n) return p->jobs[a].n > p->jobs[b].n;
+    return p->jobs[a].batch > p->jobs[b].batch;
+  });
+  for (int k = 0; k < 13; ++k) p->routine[k][0] = p->routine[k][1] = first_routine(1 << k);
+  if (hipMalloc(&p->d_ctr, 2 * sizeof(unsigned long long)) != hipSuccess ||
+      hipMemset(p->d_ctr, 0, 2 * sizeof(unsigned long long)) != hipSuccess) {
+    fftoc_plan_destroy(p);
+    return FFTOC_ERROR_ALLOC;
+  }
+  rc = layout(p);
+  if (rc != FFTOC_SUCCESS) {
+    fftoc_plan_destroy(p);
+    return rc;
+  }
+  *out = p;
+  return FFTOC_SUCCESS;
+}
+
+int fftoc_plan_execute(fftoc_plan p, void *stream) {
+  if (!p) return FFTOC_ERROR_INVALID_HANDLE;
+  const int rc = fftoc::check_device(p);
+  if (rc != FFTOC_SUCCESS) return rc;
+  return fftoc::launch(p, static_cast<hipStream_t>(stream));
+}
+
+int fftoc_plan_execute_graph(fftoc_plan p, void *stream) {
+  if (!p) return FFTOC_ERROR_INVALID_HANDLE;
+  int rc = fftoc::check_device(p);
+  if (rc != FFTOC_SUCCESS) return rc;
+  if (p->total_items == 0) return FFTOC_SUCCESS;
+  if (!p->exec) {
+    hipStream_t cs = nullptr;
+    if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess) return FFTOC_ERROR_DEVICE;
+    rc = FFTOC_SUCCESS;
+    if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+      rc = FFTOC_ERROR_DEVICE;
+    } else {
+      const int lrc = fftoc::launch(p, cs);
+      hipGraph_t g = nullptr;
+      const hipError_t e = hipStreamEndCapture(cs, &g);
+      if (lrc != FFTOC_SUCCESS || e != hipSuccess || !g) {
+        rc = lrc != FFTOC_SUCCESS ? lrc : FFTOC_ERROR_DEVICE;
+        if (g) (void)hipGraphDestroy(g);
+      } else if (hipGraphInstantiate(&p->exec, g, nullptr, nullptr, 0) != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        rc = FFTOC_ERROR_DEVICE;
+      } else {
+        p->graph = g;
+      }
+    }
+    (void)hipStreamDestroy(cs);
+    if (rc != FFTOC_SUCCESS) return rc;
+  }
+  return hipGraphLaunch(p->exec, static_cast<hipStream_t>(stream)) == hipSuccess
+             ? FFTOC_SUCCESS
+             : FFTOC_ERROR_LAUNCH;
+}
+
+int fftoc_plan_autotune(fftoc_plan p, int reps, void *stream_p) {
+  using namespace fftoc;
+  if (!p) return FFTOC_ERROR_INVALID_HANDLE;
+  if (reps <= 0) return FFTOC_ERROR_INVALID_BATCH;
+  int rc = check_device(p);
+  if (rc != FFTOC_SUCCESS) return rc;
+  hipStream_t s = static_cast<hipStream_t>(stream_p);
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess) return FFTOC_ERROR_DEVICE;
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return FFTOC_ERROR_DEVICE;
+  }
+  auto time_plan = [&](double *us) -> int {
+    if ((rc = launch(p, s)) != FFTOC_SUCCESS) return rc;  // warm-up
+    std::vector<float> t;
+    for (int r = 0; r < reps; ++r) {
+      float ms = 0;
+      if (hipEventRecord(e0, s) != hipSuccess || launch(p, s) != FFTOC_SUCCESS ||
+          hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+          hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+        return FFTOC_ERROR_DEVICE;
+      t.push_back(ms);
+    }
+    std::sort(t.begin(), t.end());
+    *us = 1000.0 * t[t.size() / 2];
+    return FFTOC_SUCCESS;
+  };
+  // coordinate descent over the (length, data type) classes present
+  std::map<std::pair<int, int>, bool> classes;
+  for (const fftoc_job &j : p->jobs)
+    if (j.batch > 0) classes[{lg(j.n), (j.flags & FFTOC_DATA_C64) ? 1 : 0}] = true;
+  rc = FFTOC_SUCCESS;
+  for (const auto &kv : classes) {
+    const int k = kv.first.first, c = kv.first.second;
+    int best = p->routine[k][c];
+    double best_us = 1e300;
+    for (int id = 0; id < batch_routine_count() && rc == FFTOC_SUCCESS; ++id) {
+      if (batch_routine_n(id) != (1 << k)) continue;
+      p->routine[k][c] = id;
+      if ((rc = layout(p)) != FFTOC_SUCCESS) break;
+      double us = 0;
+      if ((rc = time_plan(&us)) != FFTOC_SUCCESS) break;
+      if (us < best_us) {
+        best_us = us;
+        best = id;
+      }
+    }
+    p->routine[k][c] = best;
+    if (rc != FFTOC_SUCCESS) break;
+  }
+  const int lrc = layout(p);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return rc != FFTOC_SUCCESS ? rc : lrc;
+}
+
+int fftoc_plan_routine(fftoc_plan p, int n, uint32_t flags, const char **name) {
+  if (!p) return FFTOC_ERROR_INVALID_HANDLE;
+  if (!fftoc::pow2(n) || n < 8 || n > 4096) return FFTOC_ERROR_INVALID_SIZE;
+  const int id = p->routine[fftoc::lg(n)][(flags & FFTOC_DATA_C64) ? 1 : 0];
+  if (name) *name = fftoc::batch_routine_name(id);
+  return FFTOC_SUCCESS;
+}
+
+int fftoc_plan_destroy(fftoc_plan p) {
+  if (!p) return FFTOC_ERROR_INVALID_HANDLE;
+  int cur = 0;
+  (void)hipGetDevice(&cur);
+  (void)hipSetDevice(p->device);
+  if (p->exec) (void)hipGraphExecDestroy(p->exec);
+  if (p->graph) (void)hipGraphDestroy(p->graph);
+  if (p->d_jobs) (void)hipFree(p->d_jobs);
+  if (p->d_ctr) (void)hipFree(p->d_ctr);
+  (void)hipSetDevice(cur);
+  delete p;
+  return FFTOC_SUCCESS;
+}
+
+}  // extern "C"

@@ -36,10 +36,10 @@ namespace {
 template <class SP, int S>
 constexpr int kThreads = (SP::N / SP::RM) * S;
 
-template <class SP, int S, bool INV, bool STAGED>
+template <class SP, int S, bool INV, bool STAGED, class CT>
 __global__ __launch_bounds__((kThreads<SP, S>)) void stockham_kernel(
-    const c2 *in, c2 *out, int64_t batch, const c2 *__restrict__ tw, double scale) {
-  StockhamKernel<SP, S, INV, STAGED>::run(in, out, batch, tw, scale);
+    const CT *in, CT *out, int64_t batch, const CT *__restrict__ tw, real_t<CT> scale) {
+  StockhamKernel<SP, S, INV, STAGED, CT>::run(in, out, batch, tw, scale);
 }
 
 template <class SP, int S, bool INV>
@@ -78,15 +78,17 @@ int device_cus() {
 
 enum class Form { OneTile, Staged, Persistent };
 
-template <class SP, int S, Form F>
+template <class SP, int S, Form F, class CT = c2>
 int launch_split(const LaunchArgs &a) {
   constexpr int threads = kThreads<SP, S>;
   const int64_t ntiles = (a.batch + S - 1) / S;
   if (ntiles > 0x7fffffffLL) return FFTOC_ERROR_INVALID_BATCH;
-  const c2 *in = static_cast<const c2 *>(a.in);
-  c2 *out = static_cast<c2 *>(a.out);
-  const c2 *tw = static_cast<const c2 *>(a.twiddles);
+  const CT *in = static_cast<const CT *>(a.in);
+  CT *out = static_cast<CT *>(a.out);
+  const CT *tw = static_cast<const CT *>(a.twiddles);
+  const real_t<CT> scale = (real_t<CT>)a.scale;
   if constexpr (F == Form::Persistent) {
+    static_assert(sizeof(CT) == 16, "persistent form is FP64 only");
     const int per_cu = blocks_per_cu(stockham_persistent_kernel<SP, S, false>, threads);
     const int64_t resident = (int64_t)per_cu * device_cus();
     const unsigned grid = (unsigned)(ntiles < resident ? ntiles : resident);
@@ -99,11 +101,11 @@ int launch_split(const LaunchArgs &a) {
   } else {
     constexpr bool staged = (F == Form::Staged);
     if (a.inverse)
-      hipLaunchKernelGGL((stockham_kernel<SP, S, true, staged>), dim3((unsigned)ntiles),
-                         dim3(threads), 0, a.stream, in, out, a.batch, tw, a.scale);
+      hipLaunchKernelGGL((stockham_kernel<SP, S, true, staged, CT>), dim3((unsigned)ntiles),
+                         dim3(threads), 0, a.stream, in, out, a.batch, tw, scale);
     else
-      hipLaunchKernelGGL((stockham_kernel<SP, S, false, staged>), dim3((unsigned)ntiles),
-                         dim3(threads), 0, a.stream, in, out, a.batch, tw, a.scale);
+      hipLaunchKernelGGL((stockham_kernel<SP, S, false, staged, CT>), dim3((unsigned)ntiles),
+                         dim3(threads), 0, a.stream, in, out, a.batch, tw, scale);
   }
   return hipGetLastError() == hipSuccess ? FFTOC_SUCCESS : FFTOC_ERROR_LAUNCH;
 }
@@ -178,6 +180,26 @@ const Variant kV4096[] = {
     FFTOC_V("radix:8x8x8x8/tile/S1", R8, Split<8, 8, 8, 8>, 1, Form::OneTile),
     FFTOC_V("radix:8x8x8x8/persistent/S1", R8, Split<8, 8, 8, 8>, 1, Form::Persistent),
 };
+
+// complex64 data (FP32 arithmetic, float2 twiddles rounded from the FP64 table)
+#define FFTOC_V32(NAME, FAM, SPLIT_S_FORM) \
+  Variant { NAME, FAM, &launch_split<FFTOC_UNPAREN SPLIT_S_FORM, f2> }
+#define FFTOC_UNPAREN(...) __VA_ARGS__
+const Variant kW8[] = {FFTOC_V32("radix:4x2/staged/S128", MIX, (Split<4, 2>, 128, Form::Staged)),
+                       FFTOC_V32("radix:8/staged/S128", R8, (Split<8>, 128, Form::Staged))};
+const Variant kW16[] = {FFTOC_V32("radix:4x4/staged/S64", MIX, (Split<4, 4>, 64, Form::Staged)),
+                        FFTOC_V32("radix:16/staged/S64", R16, (Split<16>, 64, Form::Staged))};
+const Variant kW32[] = {FFTOC_V32("radix:8x4/staged/S32", MIX, (Split<8, 4>, 32, Form::Staged))};
+const Variant kW64[] = {FFTOC_V32("radix:8x8/tile/S16", R8, (Split<8, 8>, 16, Form::OneTile))};
+const Variant kW128[] = {FFTOC_V32("radix:16x8/tile/S8", MIX, (Split<16, 8>, 8, Form::OneTile))};
+const Variant kW256[] = {FFTOC_V32("radix:16x16/tile/S4", R16, (Split<16, 16>, 4, Form::OneTile))};
+const Variant kW512[] = {FFTOC_V32("radix:8x8x8/tile/S2", R8, (Split<8, 8, 8>, 2, Form::OneTile))};
+const Variant kW1024[] = {FFTOC_V32("radix:8x8x16/tile/S2", MIX, (Split<8, 8, 16>, 2, Form::OneTile))};
+const Variant kW2048[] = {FFTOC_V32("radix:8x16x16/tile/S1", MIX, (Split<8, 16, 16>, 1, Form::OneTile))};
+const Variant kW4096[] = {FFTOC_V32("radix:16x16x16/tile/S1", R16, (Split<16, 16, 16>, 1, Form::OneTile)),
+                          FFTOC_V32("radix:8x8x8x8/tile/S1", R8, (Split<8, 8, 8, 8>, 1, Form::OneTile))};
+#undef FFTOC_V32
+#undef FFTOC_UNPAREN
 #undef FFTOC_V
 
 template <size_t K>
@@ -188,7 +210,22 @@ int set_out(const Variant (&v)[K], const Variant **out) {
 
 }  // namespace
 
-int radix_variants(int n, const Variant **out) {
+int radix_variants(int n, bool c64, const Variant **out) {
+  if (c64) {
+    switch (n) {
+      case 8: return set_out(kW8, out);
+      case 16: return set_out(kW16, out);
+      case 32: return set_out(kW32, out);
+      case 64: return set_out(kW64, out);
+      case 128: return set_out(kW128, out);
+      case 256: return set_out(kW256, out);
+      case 512: return set_out(kW512, out);
+      case 1024: return set_out(kW1024, out);
+      case 2048: return set_out(kW2048, out);
+      case 4096: return set_out(kW4096, out);
+      default: *out = nullptr; return 0;
+    }
+  }
   switch (n) {
     case 8: return set_out(kV8, out);
     case 16: return set_out(kV16, out);

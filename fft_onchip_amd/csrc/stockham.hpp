@@ -36,7 +36,9 @@ struct Split {
 __device__ __forceinline__ int lds_index(int E) { return E + (E >> 4); }
 constexpr int lds_elems(int n) { return n + (n >> 4); }
 
-template <class SP, int S, bool INV, bool STAGED, class CT = c2>
+// INVM: 0 = forward, 1 = inverse, 2 = direction chosen at run time by the
+// `inv` argument (used by the mixed-size kernels).
+template <class SP, int S, int INVM, bool STAGED, class CT = c2>
 struct StockhamKernel {
   using R_ = real_t<CT>;
   static constexpr int N = SP::N;
@@ -44,6 +46,12 @@ struct StockhamKernel {
   static constexpr int T = N / RM;
   static constexpr int THREADS = T * S;
   static constexpr bool WAVE_LOCAL = (T <= 64) && (64 % T == 0);
+
+  __device__ static bool swap_io(bool inv) {
+    if constexpr (INVM == 0) return false;
+    else if constexpr (INVM == 1) return true;
+    else return inv;
+  }
 
   __device__ static void sync() {
     if constexpr (WAVE_LOCAL) {
@@ -54,7 +62,7 @@ struct StockhamKernel {
   }
 
   // Pass-0 inputs straight from HBM (non-staged kernels).
-  __device__ static void load_first(CT (&reg)[RM], const CT *gin, int t, bool valid) {
+  __device__ static void load_first(CT (&reg)[RM], const CT *gin, int t, bool valid, bool inv) {
     constexpr int R = SP::R[0];
     constexpr int NB = RM / R;
 #pragma unroll
@@ -62,7 +70,7 @@ struct StockhamKernel {
 #pragma unroll
       for (int m = 0; m < R; ++m) {
         CT v = valid ? ld_stream(gin + t + i * T + m * (N / R)) : mk<CT>(0, 0);
-        if constexpr (INV) v = cswap(v);
+        if (swap_io(inv)) v = cswap(v);
         reg[i * R + m] = v;
       }
     }
@@ -72,7 +80,7 @@ struct StockhamKernel {
   // non-staged kernels); otherwise they are gathered from LDS.
   template <int p, bool IN_REGS>
   __device__ static void pass(CT (&reg)[RM], CT *lds, CT *gout, int t, int sbase, bool valid,
-                              const CT *__restrict__ tw, R_ scale) {
+                              const CT *__restrict__ tw, R_ scale, bool inv) {
     constexpr int R = SP::R[p];
     constexpr int L = SP::L(p);
     constexpr int NB = RM / R;
@@ -112,7 +120,7 @@ struct StockhamKernel {
         CT v = reg[i * R + q];
         if constexpr (LAST && !STAGED) {
           if (scale != 1.0) v = cscale(v, scale);
-          if constexpr (INV) v = cswap(v);
+          if (swap_io(inv)) v = cswap(v);
           if (valid) st_stream(gout + d, v);
         } else {
           lds[lds_index(sbase + d)] = v;
@@ -125,14 +133,14 @@ struct StockhamKernel {
   template <int... ps>
   __device__ static void later_passes(std::integer_sequence<int, ps...>, CT (&reg)[RM], CT *lds,
                                       CT *gout, int t, int sbase, bool valid,
-                                      const CT *__restrict__ tw, R_ scale) {
-    (pass<ps + 1, false>(reg, lds, gout, t, sbase, valid, tw, scale), ...);
+                                      const CT *__restrict__ tw, R_ scale, bool inv) {
+    (pass<ps + 1, false>(reg, lds, gout, t, sbase, valid, tw, scale, inv), ...);
   }
 
   // Tile `tile` (signals tile*S .. tile*S + S-1) on a workgroup of THREADS
   // threads with LDS buffer `lds` of lds_elems(S*N) elements.
   __device__ static void run_tile(const CT *in, CT *out, int64_t batch, int64_t tile, CT *lds,
-                                  const CT *__restrict__ tw, R_ scale) {
+                                  const CT *__restrict__ tw, R_ scale, bool inv = false) {
     const int tid = threadIdx.x;
     const int t = tid % T;
     const int sl = tid / T;
@@ -149,17 +157,17 @@ struct StockhamKernel {
       for (int c = 0; c < (S * N) / THREADS; ++c) {
         const int E = tid + c * THREADS;
         CT v = (E < nvalid * N) ? ld_stream(src + E) : mk<CT>(0, 0);
-        if constexpr (INV) v = cswap(v);
+        if (swap_io(inv)) v = cswap(v);
         lds[lds_index(E)] = v;
       }
       __syncthreads();
-      pass<0, false>(reg, lds, nullptr, t, sl * N, valid, tw, scale);
+      pass<0, false>(reg, lds, nullptr, t, sl * N, valid, tw, scale, inv);
     } else {
-      load_first(reg, in + sig * N, t, valid);
-      pass<0, true>(reg, lds, out + sig * N, t, sl * N, valid, tw, scale);
+      load_first(reg, in + sig * N, t, valid, inv);
+      pass<0, true>(reg, lds, out + sig * N, t, sl * N, valid, tw, scale, inv);
     }
     later_passes(std::make_integer_sequence<int, SP::P - 1>{}, reg, lds, out + sig * N, t, sl * N,
-                 valid, tw, scale);
+                 valid, tw, scale, inv);
 
     if constexpr (STAGED) {
       __syncthreads();
@@ -170,7 +178,7 @@ struct StockhamKernel {
         const int E = tid + c * THREADS;
         CT v = lds[lds_index(E)];
         if (scale != 1.0) v = cscale(v, scale);
-        if constexpr (INV) v = cswap(v);
+        if (swap_io(inv)) v = cswap(v);
         if (E < nvalid * N) st_stream(dst + E, v);
       }
     }
@@ -187,7 +195,7 @@ struct StockhamKernel {
   // the next tile's HBM loads overlap the current tile's LDS passes.
   __device__ static void run_persistent(const CT *in, CT *out, int64_t batch,
                                         const CT *__restrict__ tw, R_ scale) {
-    static_assert(!STAGED && SP::P >= 2, "persistent form needs an LDS pass");
+    static_assert(!STAGED && SP::P >= 2 && INVM != 2, "persistent form needs an LDS pass");
     __shared__ CT lds[lds_elems(S * N)];
     const int tid = threadIdx.x;
     const int t = tid % T;
@@ -198,7 +206,7 @@ struct StockhamKernel {
     CT reg[RM];
     {
       const int64_t sig = tile * S + sl;
-      load_first(reg, in + sig * N, t, sig < batch);
+      load_first(reg, in + sig * N, t, sig < batch, false);
     }
     for (;;) {
       const int64_t sig = tile * S + sl;
@@ -209,14 +217,14 @@ struct StockhamKernel {
       // occupancy); the empty asm only hides the pointer's loop invariance.
       const CT *twl = tw;
       asm volatile("" : "+s"(twl));
-      pass<0, true>(reg, lds, out + sig * N, t, sl * N, valid, twl, scale);
+      pass<0, true>(reg, lds, out + sig * N, t, sl * N, valid, twl, scale, false);
       CT nxt[RM];
       if (next < ntiles) {
         const int64_t nsig = next * S + sl;
-        load_first(nxt, in + nsig * N, t, nsig < batch);
+        load_first(nxt, in + nsig * N, t, nsig < batch, false);
       }
       later_passes(std::make_integer_sequence<int, SP::P - 1>{}, reg, lds, out + sig * N, t,
-                   sl * N, valid, twl, scale);
+                   sl * N, valid, twl, scale, false);
       if (next >= ntiles) break;
       tile = next;
 #pragma unroll

@@ -66,6 +66,9 @@ typedef enum fftoc_status {
 #define FFTOC_PREC_FP32 0x10u     /* FP32 MFMA stage GEMMs, FP32 accumulate  */
 #define FFTOC_PREC_BF16X3 0x20u   /* 3-term split-BF16 MFMA, FP32 accumulate */
 #define FFTOC_PREC_MASK 0xF0u
+/* data type: complex64 (float2) input and output with FP32 arithmetic and
+ * FP32 twiddles rounded from the FP64 table (default: complex128) */
+#define FFTOC_DATA_C64 0x100u
 /* signals per workgroup override (0 = library default / autotuned choice) */
 #define FFTOC_TILE_SHIFT 16
 #define FFTOC_TILE_MASK (0xFFu << FFTOC_TILE_SHIFT)
@@ -138,6 +141,35 @@ double fftoc_variant_time_us(int n, int variant);
 /* Persisted tuning table (text: "<family> <n> <variant name>" per line). */
 int fftoc_tuning_save(const char *path);
 int fftoc_tuning_load(const char *path);
+
+/* ---- mixed-size batches: one persistent launch over many transforms -----
+ * No counterpart in the reference, whose harness launches one size per
+ * kernel (includes/testing.cuh:59-92).  Each job is one call's worth of work
+ * (same conventions as fftoc_fft; flags: FFTOC_INVERSE, FFTOC_NORMALIZE,
+ * FFTOC_DATA_C64).  A plan fixes the buffers; executing it transforms their
+ * current contents.  Executions of one plan must not overlap (they share
+ * the plan's work counter); different plans are independent. */
+typedef struct fftoc_job {
+  const void *in;
+  void *out;
+  int64_t batch;
+  int n;
+  uint32_t flags;
+} fftoc_job;
+typedef struct fftoc_plan_t *fftoc_plan;
+
+/* Validates the jobs and uploads their descriptors (allocates device memory). */
+int fftoc_plan_create(const fftoc_job *jobs, int njobs, fftoc_plan *plan);
+/* One persistent kernel launch over every job, asynchronous on `stream`. */
+int fftoc_plan_execute(fftoc_plan plan, void *stream);
+/* The same launch replayed from a hipGraph captured on first use. */
+int fftoc_plan_execute_graph(fftoc_plan plan, void *stream);
+/* Times the candidate tile routines of every (length, data type) in the plan
+ * on its own buffers and keeps the fastest (overwrites the outputs). */
+int fftoc_plan_autotune(fftoc_plan plan, int reps, void *stream);
+/* Name of the tile routine the plan uses for (n, data type of flags). */
+int fftoc_plan_routine(fftoc_plan plan, int n, uint32_t flags, const char **name);
+int fftoc_plan_destroy(fftoc_plan plan);
 
 #ifdef __cplusplus
 }

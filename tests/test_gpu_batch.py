@@ -1,0 +1,145 @@
+"""GPU tests of complex64 (FP32) transforms and of mixed-size batch plans.
+
+Tolerances: FP64 data <= 1e-13 relative L2 per signal; complex64 data
+(FP32 arithmetic, FP32 twiddles) <= 2e-6 relative L2 per signal.
+"""
+import numpy as np
+import pytest
+
+import fft_onchip_amd as fo
+import oracle
+
+pytestmark = pytest.mark.gpu
+TOL64 = 1e-13
+TOL32 = 2e-6
+
+
+def signals(n, batch, seed, c64=False):
+    g = np.random.Generator(np.random.Philox(seed))
+    x = g.uniform(-1, 1, (batch, n)) + 1j * g.uniform(-1, 1, (batch, n))
+    return x.astype(np.complex64) if c64 else x
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _device():
+    assert fo.device_count() >= 1, "no HIP device visible"
+    fo.init()
+
+
+@pytest.mark.parametrize("n", fo.ALL_SIZES)
+@pytest.mark.parametrize("path", ["best", "legacy8", "legacy16"])
+def test_complex64_matches_oracle(n, path):
+    if path == "legacy8" and n not in fo.LEGACY8_SIZES or path == "legacy16" and n not in fo.LEGACY16_SIZES:
+        pytest.skip("size not in this family")
+    x = signals(n, 257, seed=n, c64=True)
+    ref = oracle.reference(x.astype(np.complex128))
+    got = fo.fft_host(x, path=path)
+    assert got.dtype == np.complex64
+    assert np.max(oracle.rel_l2(got, ref, axis=-1)) <= TOL32
+    refi = oracle.reference(x.astype(np.complex128), inverse=True)
+    goti = fo.fft_host(x, path=path, inverse=True)
+    assert np.max(oracle.rel_l2(goti, refi, axis=-1)) <= TOL32
+
+
+def make_jobs(seed, c64_mix=True):
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for i, n in enumerate(list(fo.ALL_SIZES) * 2):
+        batch = int(rng.integers(1, max(2, 40000 // n)))
+        c64 = c64_mix and (i % 2 == 1)
+        inv = bool(rng.integers(2))
+        norm = inv and bool(rng.integers(2))
+        x = signals(n, batch, seed=1000 * seed + i, c64=c64)
+        flags = (fo.INVERSE if inv else 0) | (fo.NORMALIZE if norm else 0) | (fo.DATA_C64 if c64 else 0)
+        jobs.append(dict(n=n, x=x, flags=flags, c64=c64, inv=inv, norm=norm))
+    return jobs
+
+
+def check_job(j, got):
+    ref = oracle.reference(j["x"].astype(np.complex128), inverse=j["inv"])
+    if j["norm"]:
+        ref = oracle.DD(ref.hi / j["n"], ref.lo / j["n"])
+    err = np.max(oracle.rel_l2(got, ref, axis=-1))
+    assert err <= (TOL32 if j["c64"] else TOL64), (j["n"], j["c64"], j["inv"], err)
+
+
+def run_plan(jobs, graph=False, replays=1, autotune=False):
+    bufs = [(fo.DeviceBuffer.from_array(j["x"]), fo.DeviceBuffer(j["x"].nbytes)) for j in jobs]
+    plan = fo.Plan([(i, o, j["x"].shape[0], j["n"], j["flags"]) for (i, o), j in zip(bufs, jobs)])
+    if autotune:
+        plan.autotune(reps=2)
+    outs = []
+    for r in range(replays):
+        plan.execute(graph=graph)
+        fo.synchronize()
+        outs.append([o.download(j["x"].shape, dtype=j["x"].dtype) for (_, o), j in zip(bufs, jobs)])
+    plan.close()
+    return outs, bufs, plan
+
+
+def test_plan_mixed_sizes_and_types_match_oracle():
+    jobs = make_jobs(1)
+    print(f"[plan] {len(jobs)} jobs", flush=True)
+    (outs,), _, _ = run_plan(jobs)
+    for j, got in zip(jobs, outs):
+        check_job(j, got)
+
+
+def test_plan_graph_replays_match_eager():
+    """Replays of the captured graph, with eager work and new input data in
+    between, each match the eager launch."""
+    jobs = make_jobs(2)
+    bufs = [(fo.DeviceBuffer.from_array(j["x"]), fo.DeviceBuffer(j["x"].nbytes)) for j in jobs]
+    plan = fo.Plan([(i, o, j["x"].shape[0], j["n"], j["flags"]) for (i, o), j in zip(bufs, jobs)])
+    plan.execute()
+    fo.synchronize()
+    eager = [o.download(j["x"].shape, dtype=j["x"].dtype) for (_, o), j in zip(bufs, jobs)]
+    for j, got in zip(jobs, eager):
+        check_job(j, got)
+    for rep in range(4):
+        print(f"[plan] graph replay {rep}", flush=True)
+        # eager work on the device between replays
+        fo.fft_host(signals(256, 64, seed=rep))
+        if rep == 2:  # new contents in the same buffers
+            for (i, _), j in zip(bufs, jobs):
+                j["x"] = signals(j["n"], j["x"].shape[0], seed=99 + j["n"], c64=j["c64"])
+                i.upload(j["x"])
+        plan.execute(graph=True)
+        fo.synchronize()
+        for (_, o), j, e in zip(bufs, jobs, eager):
+            got = o.download(j["x"].shape, dtype=j["x"].dtype)
+            if rep < 2:
+                assert np.array_equal(got, e)
+            else:
+                check_job(j, got)
+    plan.close()
+
+
+def test_plan_result_same_alone_and_in_mix():
+    jobs = make_jobs(3)
+    (mixed,), _, _ = run_plan(jobs)
+    for k in (0, 5, 12, len(jobs) - 1):
+        (alone,), _, _ = run_plan([jobs[k]])
+        assert np.array_equal(alone[0], mixed[k]), (jobs[k]["n"], jobs[k]["c64"])
+
+
+def test_plan_autotune_keeps_results_correct():
+    jobs = make_jobs(4, c64_mix=True)
+    (outs,), _, _ = run_plan(jobs, autotune=True)
+    for j, got in zip(jobs, outs):
+        check_job(j, got)
+
+
+def test_plan_rejects_bad_jobs():
+    a = fo.DeviceBuffer(64 * 16)
+    with pytest.raises(fo.FftocError) as e:
+        fo.Plan([(a, a, 1, 100, 0)])
+    assert e.value.name == "INVALID_SIZE"
+    with pytest.raises(fo.FftocError) as e:
+        fo.Plan([(a, a, 1, 64, fo.PREC_FP32)])
+    assert e.value.name == "UNSUPPORTED"
+    empty = fo.Plan([(a, a, 0, 64, 0)])
+    empty.execute()
+    empty.execute(graph=True)
+    fo.synchronize()
+    empty.close()
