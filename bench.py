@@ -111,6 +111,96 @@ def run_gpu(fo, wl: Workload, steps: int, warmup: int, host_x: np.ndarray | None
     }, x, sample
 
 
+def c5_requests(count: int, seed: int, max_batch: int = 4096):
+    """Request mix of the served configuration: N in 8..4096 with P ~ 1/log2 N,
+    batch uniform 1..max_batch, FP64 or FP32 (complex64) data 50/50."""
+    rng = np.random.default_rng(seed)
+    sizes = np.array([1 << k for k in range(3, 13)])
+    w = 1.0 / np.log2(sizes)
+    ns = rng.choice(sizes, size=count, p=w / w.sum())
+    batches = rng.integers(1, max_batch + 1, size=count)
+    c64 = rng.integers(0, 2, size=count).astype(bool)
+    return rng, ns, batches, c64
+
+
+def run_batched(fo, steps: int, warmup: int, njobs: int = 64):
+    """Mixed-size plan (C5 job mix) as ONE persistent launch from a hipGraph,
+    against the same jobs issued as one fftoc_fft call each."""
+    from fft_onchip_amd import hip
+    rng, ns, batches, c64 = c5_requests(njobs, seed=13)
+    bufs, jobs, nsig, nbytes = [], [], 0, 0
+    for n, b, c in zip(ns, batches, c64):
+        es = 8 if c else 16
+        x = rng.uniform(-1, 1, 2 * int(n) * int(b)).astype(np.float32 if c else np.float64)
+        src, dst = hip.DeviceBuffer.from_array(x), hip.DeviceBuffer(x.nbytes)
+        bufs += [src, dst]
+        jobs.append((src, dst, int(b), int(n), fo.DATA_C64 if c else 0))
+        nsig += int(b)
+        nbytes += 2 * int(n) * int(b) * es
+    plan = fo.Plan(jobs)
+    stream = hip.Stream()
+    plan.autotune(reps=2, stream=stream)
+
+    def timed(run):
+        for _ in range(warmup):
+            run()
+        stream.synchronize()
+        a, b = hip.Event(), hip.Event()
+        a.record(stream)
+        for _ in range(steps):
+            run()
+        b.record(stream)
+        stream.synchronize()
+        return a.elapsed_ms(b) / steps
+
+    ms_graph = timed(lambda: plan.execute(stream=stream, graph=True))
+    ms_plain = timed(lambda: [fo.fft(j[0], j[2], j[3], out=j[1], flags=j[4], stream=stream) for j in jobs])
+    plan.close()
+    for b in bufs:
+        b.free()
+    return {"jobs": njobs, "signals": nsig, "bytes": nbytes,
+            "graph_ms": round(ms_graph, 4), "graph_ffts_per_s": nsig / (ms_graph * 1e-3),
+            "graph_GBps": round(nbytes / (ms_graph * 1e-3) / 1e9, 1),
+            "per_call_ms": round(ms_plain, 4), "per_call_ffts_per_s": nsig / (ms_plain * 1e-3),
+            "per_call_GBps": round(nbytes / (ms_plain * 1e-3) / 1e9, 1)}
+
+
+def run_served(fo, seconds: float = 1.0, rate: float = 20000.0, saturated: int = 4000):
+    """Poisson request stream through the device-resident server (C5)."""
+    from fft_onchip_amd import hip
+    count = int(seconds * rate)
+    rng, ns, batches, c64 = c5_requests(count + saturated, seed=13)
+    lake_bytes = 1 << 30
+    lake = hip.DeviceBuffer.from_array(rng.uniform(-1, 1, lake_bytes // 8))
+    reqs = []
+    for n, b, c in zip(ns, batches, c64):
+        nb = int(n) * int(b) * (8 if c else 16)
+        off = int(rng.integers(0, (lake_bytes - nb) // 256)) * 256
+        reqs.append((lake.ptr + off, int(b), int(n), fo.DATA_C64 if c else 0))
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, count))
+    out = {}
+    with fo.Server(8 << 30, 2048) as srv:
+        warm = [0.0] * 64
+        srv.replay(reqs[:64], warm)
+        lat, span = srv.replay(reqs[:count], arrivals.tolist())
+        sig = sum(r[1] for r in reqs[:count])
+        out["poisson"] = {"requests": count, "rate_per_s": rate, "span_s": round(span, 4),
+                          "ffts_per_s": sig / span,
+                          "p50_us": round(float(np.percentile(lat, 50)) * 1e6, 1),
+                          "p99_us": round(float(np.percentile(lat, 99)) * 1e6, 1),
+                          "mean_us": round(float(np.mean(lat)) * 1e6, 1)}
+        sat = reqs[count:]
+        lat2, span2 = srv.replay(sat, [0.0] * len(sat))
+        sig2 = sum(r[1] for r in sat)
+        nbytes2 = sum(2 * r[1] * r[2] * (8 if r[3] else 16) for r in sat)
+        out["saturated"] = {"requests": len(sat), "span_s": round(span2, 4),
+                            "ffts_per_s": sig2 / span2, "GBps": round(nbytes2 / span2 / 1e9, 1),
+                            "p50_us": round(float(np.percentile(lat2, 50)) * 1e6, 1),
+                            "p99_us": round(float(np.percentile(lat2, 99)) * 1e6, 1)}
+    lake.free()
+    return out
+
+
 def copy_bandwidth(nbytes: int = 1 << 30, reps: int = 10) -> float:
     """Device-to-device hipMemcpy of `nbytes` (read + write) in GB/s."""
     import ctypes
@@ -296,6 +386,11 @@ def main():
             extra[wl.name] = {k: (round(v, 6) if isinstance(v, float) else v) for k, v in r.items()}
             print(f"[bench] {wl.name}: {r['ffts_per_s']:.4g} FFT/s, {r['GBps']:.0f} GB/s", file=sys.stderr, flush=True)
     extra["hip_d2d_copy_GBps"] = round(copy_bandwidth(), 1)
+    if not args.quick:
+        extra["batched_mixed_c5"] = run_batched(fo, args.steps, args.warmup)
+        print(f"[bench] batched plan: {extra['batched_mixed_c5']}", file=sys.stderr, flush=True)
+        extra["served_c5"] = run_served(fo)
+        print(f"[bench] served: {extra['served_c5']}", file=sys.stderr, flush=True)
 
     cpu = None
     if not args.no_cpu:

@@ -198,6 +198,90 @@ class Plan:
             pass
 
 
+class Server:
+    """Device-resident FFT server (fftoc_server_*)."""
+
+    def __init__(self, budget_bytes: int, max_inflight: int = 256):
+        h = ctypes.c_void_p()
+        rc = lib().fftoc_server_create(int(budget_bytes), int(max_inflight), ctypes.byref(h))
+        if rc != 0:
+            raise FftocError(rc, "fftoc_server_create")
+        self.handle = h.value
+
+    def submit(self, x, batch: int, n: int, flags: int = 0):
+        """Returns the request id, or None if no slot / budget is free now."""
+        rid = ctypes.c_uint64()
+        rc = lib().fftoc_server_submit(self.handle, _ptr(x), int(batch), int(n), int(flags),
+                                       ctypes.byref(rid))
+        if rc == 12:  # FFTOC_ERROR_QUEUE_FULL
+            return None
+        if rc != 0:
+            raise FftocError(rc, "fftoc_server_submit")
+        return rid.value
+
+    def poll(self, rid: int):
+        """Device pointer of the result when done, else None."""
+        done, out = ctypes.c_int(0), ctypes.c_void_p()
+        rc = lib().fftoc_server_poll(self.handle, ctypes.c_uint64(rid), ctypes.byref(done), ctypes.byref(out))
+        if rc != 0:
+            raise FftocError(rc, "fftoc_server_poll")
+        return out.value if done.value else None
+
+    def wait(self, rid: int, timeout_s: float = 20.0):
+        import time
+        t0 = time.perf_counter()
+        while True:
+            p = self.poll(rid)
+            if p is not None:
+                return p
+            if time.perf_counter() - t0 > timeout_s:
+                raise TimeoutError(f"request {rid} not complete after {timeout_s} s")
+
+    def release(self, rid: int) -> None:
+        rc = lib().fftoc_server_release(self.handle, ctypes.c_uint64(rid))
+        if rc != 0:
+            raise FftocError(rc, "fftoc_server_release")
+
+    def stats(self):
+        used, inflight = ctypes.c_size_t(0), ctypes.c_int(0)
+        lib().fftoc_server_stats(self.handle, ctypes.byref(used), ctypes.byref(inflight))
+        return used.value, inflight.value
+
+    def replay(self, reqs, arrivals):
+        """reqs: list of (in_buf, batch, n, flags); arrivals: seconds.
+        Returns (latencies_s, span_s)."""
+        count = len(reqs)
+        arr = (_native.Job * max(1, count))()
+        for i, (src, batch, n, flags) in enumerate(reqs):
+            arr[i] = _native.Job(_ptr(src), None, int(batch), int(n), int(flags))
+        at = (ctypes.c_double * max(1, count))(*arrivals)
+        lat = (ctypes.c_double * max(1, count))()
+        span = ctypes.c_double(0)
+        rc = lib().fftoc_server_replay(self.handle, arr, count, at, lat, ctypes.byref(span))
+        if rc != 0:
+            raise FftocError(rc, "fftoc_server_replay")
+        return np.array(lat[:count]), span.value
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            rc = lib().fftoc_server_destroy(self.handle)
+            self.handle = None
+            if rc != 0:
+                raise FftocError(rc, "fftoc_server_destroy")
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def init() -> None:
     rc = lib().fftoc_init()
     if rc != 0:

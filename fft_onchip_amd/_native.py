@@ -53,6 +53,15 @@ SIGNATURES.update({
     "fftoc_plan_autotune": ([_vp, _int, _vp], _int),
     "fftoc_plan_routine": ([_vp, _int, _u32, ctypes.POINTER(ctypes.c_char_p)], _int),
     "fftoc_plan_destroy": ([_vp], _int),
+    "fftoc_server_create": ([ctypes.c_size_t, _int, ctypes.POINTER(_vp)], _int),
+    "fftoc_server_submit": ([_vp, _vp, _i64, _int, _u32, ctypes.POINTER(ctypes.c_uint64)], _int),
+    "fftoc_server_poll": ([_vp, ctypes.c_uint64, ctypes.POINTER(_int), ctypes.POINTER(_vp)], _int),
+    "fftoc_server_release": ([_vp, ctypes.c_uint64], _int),
+    "fftoc_server_gpu_time": ([_vp, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64)], _int),
+    "fftoc_server_replay": ([_vp, ctypes.POINTER(Job), _int, ctypes.POINTER(ctypes.c_double),
+                             ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)], _int),
+    "fftoc_server_stats": ([_vp, ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(_int)], _int),
+    "fftoc_server_destroy": ([_vp], _int),
 })
 TENSOR_SIZES = (8, 16, 64, 128, 256, 512, 4096)
 for _n in TENSOR_SIZES:

@@ -25,6 +25,52 @@ struct JobDev {
   double scale;
 };
 
+// ---- serving loop -----------------------------------------------------------
+// Request descriptor in fine-grained (coherent) pinned host memory, written by
+// the host under a sequence lock: seq = kSeqInvalid, fields, seq = request id.
+constexpr uint64_t kSeqInvalid = ~0ull;
+struct alignas(64) ReqDesc {
+  uint64_t seq;
+  const void *in;
+  void *out;
+  int64_t batch;
+  int64_t ntiles;
+  int64_t nitems;
+  int32_t n;
+  int32_t routine;
+  int32_t tiles_per_item;
+  int32_t flags;  // bit 0 inverse, bit 1 complex64
+  double scale;
+};
+struct alignas(16) Completion {
+  uint64_t seq;       // request id once complete
+  uint64_t gpu_time;  // wall_clock64() at completion (100 MHz)
+};
+struct HostRing {  // pinned, coherent, mapped into the device address space
+  uint64_t published;  // requests 0 .. published-1 have descriptors
+  uint32_t stop;       // host asks the loop to exit
+  uint32_t pad;
+};
+struct ServerDev {
+  const HostRing *ring;    // device view of the host ring header
+  uint32_t *live;          // [grid] in host memory: 1 while that workgroup runs
+  const ReqDesc *descs;    // [nslots] host memory, read by the dispatcher only
+  ReqDesc *dtab;           // [nslots] device copy of the descriptors
+  unsigned long long *dev_pub;   // requests whose dtab entry is valid
+  unsigned *dev_stop;      // the dispatcher has left: workers leave when idle
+  Completion *comps;       // [nslots]
+  unsigned *claim;         // [max_requests] items handed out per request
+  unsigned *done;          // [max_requests] items finished per request
+  unsigned long long *head;  // lowest request that may still have items
+  int nslots;
+  uint64_t max_requests;
+  uint64_t idle_timeout_ticks;  // wall_clock64 ticks without work before exit
+  const void *tw64;
+  const void *tw32;
+};
+int launch_server(const ServerDev &sd, int grid, hipStream_t stream);
+int server_grid();
+
 int batch_routine_count();
 int batch_routine_n(int id);
 const char *batch_routine_name(int id);

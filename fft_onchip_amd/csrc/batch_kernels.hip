@@ -128,7 +128,235 @@ __global__ __launch_bounds__(kBatchThreads) __attribute__((amdgpu_waves_per_eu(2
   }
 }
 
+// ---- device-resident serving loop -------------------------------------------
+
+__device__ __forceinline__ uint64_t ld_sys(const uint64_t *p) {
+  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+__device__ __forceinline__ uint32_t ld_sys32(const uint32_t *p) {
+  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+// Copies descriptor `c` from host memory under its sequence lock; false if
+// the host slot does not hold request c.
+__device__ bool read_host_desc(const ReqDesc *slot, uint64_t c, ReqDesc &d) {
+  if (ld_sys(&slot->seq) != c) return false;
+  const uint64_t *src = reinterpret_cast<const uint64_t *>(slot);
+  uint64_t *dst = reinterpret_cast<uint64_t *>(&d);
+#pragma unroll
+  for (int w = 1; w < (int)(sizeof(ReqDesc) / 8); ++w)
+    dst[w] = __hip_atomic_load(src + w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  __atomic_thread_fence(__ATOMIC_ACQUIRE);
+  if (ld_sys(&slot->seq) != c) return false;
+  d.seq = c;
+  return true;
+}
+
+// Device descriptor table: the dispatcher writes an entry as seq = invalid,
+// fields, seq = c (release between the steps); workers read it under the
+// same sequence lock at agent scope.
+__device__ bool read_dtab(const ReqDesc *e, uint64_t c, JobDev &J) {
+  if (__hip_atomic_load(&e->seq, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != c) return false;
+  uint64_t w[sizeof(ReqDesc) / 8];
+  const uint64_t *src = reinterpret_cast<const uint64_t *>(e);
+#pragma unroll
+  for (int k = 1; k < (int)(sizeof(ReqDesc) / 8); ++k)
+    w[k] = __hip_atomic_load(src + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __atomic_thread_fence(__ATOMIC_ACQUIRE);
+  if (__hip_atomic_load(&e->seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != c) return false;
+  ReqDesc d;
+  __builtin_memcpy(&d, w, sizeof(d));
+  J.in = d.in;
+  J.out = d.out;
+  J.batch = d.batch;
+  J.ntiles = d.ntiles;
+  J.item_begin = d.nitems;  // holds nitems for served requests
+  J.routine = d.routine;
+  J.tiles_per_item = d.tiles_per_item;
+  J.inverse = d.flags & 1;
+  J.c64 = (d.flags >> 1) & 1;
+  J.scale = d.scale;
+  return true;
+}
+
+enum ServeState : int { kWork = 0, kIdle = 1, kExit = 2 };
+
+// Wave 0 of workgroup 0: moves newly published requests from the host ring
+// into the device table, up to 64 per round (one per lane, so one PCIe round
+// trip and three release fences per round).  Entries follow the sequence
+// lock: seq = invalid, fields, seq = id, each step ordered by a release
+// fence, then dev_pub advances.  Leaves when the host sets `stop` or after
+// idle_timeout_ticks without requests, then raises dev_stop so the workers
+// leave once idle.
+__device__ void dispatcher(const ServerDev &sd) {
+  const int lane = threadIdx.x;  // 0..63
+  uint64_t dpub = __hip_atomic_load(sd.dev_pub, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  uint64_t idle_since = wall_clock64();
+  const uint64_t ns = (uint64_t)sd.nslots;
+  for (;;) {
+    uint64_t pub = ld_sys(&sd.ring->published);
+    pub = ((uint64_t)__builtin_amdgcn_readfirstlane((unsigned)(pub >> 32)) << 32) |
+          (uint64_t)__builtin_amdgcn_readfirstlane((unsigned)pub);
+    if (pub > dpub) {
+      const uint64_t cnt = (pub - dpub) < 64 ? (pub - dpub) : 64;
+      const uint64_t c = dpub + (uint64_t)lane;
+      ReqDesc d;
+      const bool ok = ((uint64_t)lane < cnt) && read_host_desc(&sd.descs[c % ns], c, d);
+      const unsigned long long mask = __ballot(ok);
+      const int k = (~mask == 0ull) ? 64 : __builtin_ctzll(~mask);  // leading lanes ready
+      if (k > 0) {
+        ReqDesc *e = &sd.dtab[c % ns];
+        uint64_t *dst = reinterpret_cast<uint64_t *>(e);
+        const uint64_t *src = reinterpret_cast<const uint64_t *>(&d);
+        if (lane < k) __hip_atomic_store(&e->seq, kSeqInvalid, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __atomic_thread_fence(__ATOMIC_RELEASE);
+        if (lane < k) {
+#pragma unroll
+          for (int w = 1; w < (int)(sizeof(ReqDesc) / 8); ++w)
+            __hip_atomic_store(dst + w, src[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        __atomic_thread_fence(__ATOMIC_RELEASE);
+        if (lane < k) __hip_atomic_store(&e->seq, c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __atomic_thread_fence(__ATOMIC_RELEASE);
+        dpub += (uint64_t)k;
+        if (lane == 0)
+          __hip_atomic_store(sd.dev_pub, (unsigned long long)dpub, __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+        idle_since = wall_clock64();
+        continue;
+      }
+    }
+    const unsigned stop = __builtin_amdgcn_readfirstlane(ld_sys32(&sd.ring->stop));
+    const uint64_t now = wall_clock64();
+    const bool timeout = __builtin_amdgcn_readfirstlane((unsigned)(now - idle_since > sd.idle_timeout_ticks));
+    if (stop || timeout) break;
+    __builtin_amdgcn_s_sleep(2);
+  }
+  if (lane == 0) __hip_atomic_store(sd.dev_stop, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Persistent serving kernel.  Workgroup 0 is the dispatcher; the others claim
+// items of dispatched requests in id order through per-request counters
+// (never reused, so no reset races), run the request's tile routine and
+// count finished items; the workgroup that finishes a request's last item
+// publishes its completion record in host memory.
+__global__ __launch_bounds__(kBatchThreads) __attribute__((amdgpu_waves_per_eu(2)))
+void serve_kernel(ServerDev sd) {
+  if (blockIdx.x == 0) {
+    if (threadIdx.x < 64) {
+      dispatcher(sd);
+      if (threadIdx.x == 0)
+        __hip_atomic_store(&sd.live[0], 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+    return;
+  }
+  __shared__ c2 lds[kLdsElems];
+  __shared__ JobDev s_job;
+  __shared__ long long s_item;
+  __shared__ unsigned long long s_req;
+  __shared__ int s_state;
+  // Scheduling state lives in thread 0; its decision reaches the workgroup
+  // through s_state, so every wave takes the same branch.
+  uint64_t c = 0;          // request cursor
+  bool have = false;       // s_job holds request c
+  uint64_t idle_since = wall_clock64();
+  for (;;) {
+    if (threadIdx.x == 0) {
+      int state = kIdle;
+      for (int guard = 0; guard < 64; ++guard) {
+        const uint64_t h = __hip_atomic_load(sd.head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (h > c) {
+          c = h;
+          have = false;
+        }
+        if (c >= sd.max_requests) break;
+        if (!have) {
+          const uint64_t pub =
+              __hip_atomic_load(sd.dev_pub, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          if (c >= pub) break;  // nothing dispatched beyond the cursor
+          JobDev J;
+          if (!read_dtab(&sd.dtab[c % (uint64_t)sd.nslots], c, J)) {
+            // dispatched, but the entry no longer holds c: c finished and recycled
+            ++c;
+            continue;
+          }
+          s_job = J;
+          have = true;
+        }
+        const unsigned item = atomicAdd(&sd.claim[c], 1u);
+        if ((int64_t)item < s_job.item_begin) {
+          s_item = item;
+          s_req = c;
+          state = kWork;
+          break;
+        }
+        atomicMax(sd.head, (unsigned long long)(c + 1));
+        ++c;
+        have = false;
+      }
+      if (state == kIdle) {
+        if (__hip_atomic_load(sd.dev_stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ||
+            wall_clock64() - idle_since > sd.idle_timeout_ticks + 100000000ull)
+          state = kExit;
+      } else {
+        idle_since = wall_clock64();
+      }
+      s_state = state;
+    }
+    __syncthreads();
+    const int state = s_state;
+    if (state == kExit) break;
+    if (state == kIdle) {
+      __builtin_amdgcn_s_sleep(4);
+      __syncthreads();  // s_state has been read before thread 0 rewrites it
+      continue;
+    }
+    const JobDev J = s_job;
+    const int64_t item = s_item;
+    const int64_t t0 = item * J.tiles_per_item;
+    const int64_t t1 = (t0 + J.tiles_per_item < J.ntiles) ? t0 + J.tiles_per_item : J.ntiles;
+    for (int64_t t = t0; t < t1; ++t) {
+      if (J.c64)
+        run_routine<f2>(J.routine, J, t, reinterpret_cast<f2 *>(lds), static_cast<const f2 *>(sd.tw32));
+      else
+        run_routine<c2>(J.routine, J, t, lds, static_cast<const c2 *>(sd.tw64));
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+      const uint64_t r = s_req;
+      // this item's outputs reach memory before it is counted
+      __threadfence();
+      const unsigned fin = atomicAdd(&sd.done[r], 1u);
+      if ((int64_t)fin + 1 == J.item_begin) {
+        __threadfence_system();
+        Completion *cp = &sd.comps[r % (uint64_t)sd.nslots];
+        __hip_atomic_store(&cp->gpu_time, wall_clock64(), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_SYSTEM);
+        __hip_atomic_store(&cp->seq, r, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+      }
+    }
+    __syncthreads();
+  }
+  // Lets the host see, by reading its own memory, that this workgroup left.
+  if (threadIdx.x == 0)
+    __hip_atomic_store(&sd.live[blockIdx.x], 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 }  // namespace
+
+int server_grid() {
+  int dev = 0, cus = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      cus <= 0)
+    cus = 256;
+  return cus;  // one workgroup per CU leaves room for other work on the device
+}
+
+int launch_server(const ServerDev &sd, int grid, hipStream_t stream) {
+  hipLaunchKernelGGL(serve_kernel, dim3(grid), dim3(kBatchThreads), 0, stream, sd);
+  return hipGetLastError() == hipSuccess ? FFTOC_SUCCESS : FFTOC_ERROR_LAUNCH;
+}
 
 int batch_routine_count() { return kNumRoutines; }
 int batch_routine_n(int id) { return (id >= 0 && id < kNumRoutines) ? kRoutines[id].n : 0; }

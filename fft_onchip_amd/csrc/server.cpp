@@ -1,0 +1,457 @@
+// Continuous-batching FFT server: a device-resident request loop
+// (serve_kernel in batch_kernels.hip) fed through a request ring in coherent
+// pinned host memory.  The host admits requests while a ring slot and enough
+// of the HBM budget (the output pool) are free; requests that arrive while
+// others run are picked up by idle workgroups at once, so their work overlaps
+// on the device.  Completions are posted by the device into the ring.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <new>
+#include <vector>
+
+#include "batch_internal.hpp"
+
+namespace {
+
+// First-fit allocator over [0, size) with coalescing; 256-byte granules.
+class PoolAllocator {
+ public:
+  void reset(size_t size) {
+    free_.clear();
+    free_[0] = size;
+    size_ = size;
+  }
+  bool alloc(size_t bytes, size_t *off) {
+    bytes = (bytes + 255) & ~size_t(255);
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+      if (it->second >= bytes) {
+        *off = it->first;
+        const size_t rest = it->second - bytes;
+        const size_t start = it->first + bytes;
+        free_.erase(it);
+        if (rest) free_[start] = rest;
+        used_ += bytes;
+        return true;
+      }
+    }
+    return false;
+  }
+  void release(size_t off, size_t bytes) {
+    bytes = (bytes + 255) & ~size_t(255);
+    used_ -= bytes;
+    auto it = free_.emplace(off, bytes).first;
+    auto next = std::next(it);
+    if (next != free_.end() && it->first + it->second == next->first) {
+      it->second += next->second;
+      free_.erase(next);
+    }
+    if (it != free_.begin()) {
+      auto prev = std::prev(it);
+      if (prev->first + prev->second == it->first) {
+        prev->second += it->second;
+        free_.erase(it);
+      }
+    }
+  }
+  size_t capacity() const { return size_; }
+  size_t used() const { return used_; }
+
+ private:
+  std::map<size_t, size_t> free_;
+  size_t size_ = 0, used_ = 0;
+};
+
+struct Slot {
+  bool busy = false;
+  uint64_t seq = 0;
+  size_t out_off = 0, out_bytes = 0;
+};
+
+// The device loop exits after this long without work (so it never outlives
+// a vanished host); submit / poll relaunch it when it is needed again.
+constexpr uint64_t kIdleTimeoutTicks = 2ull * 100000000ull;  // 2 s at 100 MHz
+
+}  // namespace
+
+struct fftoc_server_t {
+  int device = 0;
+  std::mutex m;
+  PoolAllocator pool;
+  char *d_pool = nullptr;
+  int nslots = 0;
+  std::vector<Slot> slots;
+  std::map<uint64_t, int> slot_of;  // request id -> slot
+  fftoc::HostRing *ring = nullptr;  // pinned host memory
+  fftoc::ReqDesc *descs = nullptr;
+  fftoc::Completion *comps = nullptr;
+  uint32_t *live = nullptr;         // pinned: per-workgroup "running" flags
+  double last_check = 0;            // time of the last ensure_running from poll
+  int grid = 0;
+  fftoc::ServerDev sd{};
+  uint64_t next_seq = 0;
+  hipStream_t stream = nullptr;
+  bool running = false;
+  int routine[13][2];
+  const void *tw64 = nullptr, *tw32 = nullptr;
+};
+
+namespace fftoc {
+namespace {
+
+bool pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
+int lg(int n) { return 31 - __builtin_clz((unsigned)n); }
+
+void free_server(fftoc_server_t *s) {
+  if (s->d_pool) (void)hipFree(s->d_pool);
+  if (s->sd.claim) (void)hipFree(s->sd.claim);
+  if (s->sd.done) (void)hipFree(s->sd.done);
+  if (s->sd.head) (void)hipFree(s->sd.head);
+  if (s->sd.dtab) (void)hipFree(s->sd.dtab);
+  if (s->sd.dev_pub) (void)hipFree(s->sd.dev_pub);
+  if (s->sd.dev_stop) (void)hipFree(s->sd.dev_stop);
+  if (s->ring) (void)hipHostFree(s->ring);
+  if (s->descs) (void)hipHostFree(s->descs);
+  if (s->comps) (void)hipHostFree(s->comps);
+  if (s->live) (void)hipHostFree(s->live);
+  if (s->stream) (void)hipStreamDestroy(s->stream);
+  delete s;
+}
+
+template <class T>
+int host_alloc(T **h, const T **d_const, T **d, size_t count) {
+  void *p = nullptr;
+  if (hipHostMalloc(&p, count * sizeof(T), hipHostMallocCoherent | hipHostMallocMapped) !=
+      hipSuccess)
+    return FFTOC_ERROR_ALLOC;
+  std::memset(p, 0, count * sizeof(T));
+  void *dp = nullptr;
+  if (hipHostGetDevicePointer(&dp, p, 0) != hipSuccess) {
+    (void)hipHostFree(p);
+    return FFTOC_ERROR_DEVICE;
+  }
+  *h = static_cast<T *>(p);
+  if (d_const) *d_const = static_cast<const T *>(dp);
+  if (d) *d = static_cast<T *>(dp);
+  return FFTOC_SUCCESS;
+}
+
+// Launches the device loop.  The live flags are raised before the launch and
+// each workgroup lowers its own flag as it exits, so "all flags clear" means
+// the loop is gone (a relaunch on the same stream starts after it ends).
+int start_loop(fftoc_server_t *s) {
+  for (int i = 0; i < s->grid; ++i) __atomic_store_n(&s->live[i], 1u, __ATOMIC_RELAXED);
+  std::atomic_thread_fence(std::memory_order_seq_cst);
+  // ordered on the loop's stream, after the previous launch has ended
+  if (hipMemsetAsync(s->sd.dev_stop, 0, sizeof(unsigned), s->stream) != hipSuccess)
+    return FFTOC_ERROR_DEVICE;
+  return launch_server(s->sd, s->grid, s->stream);
+}
+
+// Relaunches the device loop if it has exited after its idle timeout.  Called
+// with the server mutex held.  Workgroup 0's flag is read first; the full
+// scan runs only once it is clear (the loop is leaving or gone).
+int ensure_running(fftoc_server_t *s) {
+  if (__atomic_load_n(&s->live[0], __ATOMIC_ACQUIRE)) return FFTOC_SUCCESS;
+  for (int i = 1; i < s->grid; ++i)
+    if (__atomic_load_n(&s->live[i], __ATOMIC_ACQUIRE)) return FFTOC_SUCCESS;
+  return start_loop(s);
+}
+
+double now_s() {
+  using namespace std::chrono;
+  return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace
+}  // namespace fftoc
+
+extern "C" {
+
+int fftoc_server_create(size_t hbm_budget_bytes, int max_inflight, fftoc_server *out) {
+  using namespace fftoc;
+  if (!out) return FFTOC_ERROR_NULL_POINTER;
+  *out = nullptr;
+  if (hbm_budget_bytes < 4096 || max_inflight <= 0 || max_inflight > (1 << 16))
+    return FFTOC_ERROR_INVALID_BATCH;
+  const void *tw64 = nullptr, *tw32 = nullptr;
+  int rc = device_tables(&tw64, &tw32);
+  if (rc != FFTOC_SUCCESS) return rc;
+  fftoc_server_t *s = new (std::nothrow) fftoc_server_t;
+  if (!s) return FFTOC_ERROR_ALLOC;
+  (void)hipGetDevice(&s->device);
+  s->tw64 = tw64;
+  s->tw32 = tw32;
+  for (int k = 0; k < 13; ++k) {
+    int id = -1;
+    for (int r = 0; r < batch_routine_count(); ++r)
+      if (batch_routine_n(r) == (1 << k)) {
+        id = r;
+        break;
+      }
+    s->routine[k][0] = s->routine[k][1] = id;
+  }
+  s->nslots = max_inflight;
+  s->slots.assign(max_inflight, Slot{});
+  s->sd.max_requests = 1ull << 22;
+  s->sd.nslots = max_inflight;
+  s->sd.idle_timeout_ticks = kIdleTimeoutTicks;
+  s->sd.tw64 = tw64;
+  s->sd.tw32 = tw32;
+  rc = FFTOC_SUCCESS;
+  if (hipMalloc(&s->d_pool, hbm_budget_bytes) != hipSuccess ||
+      hipMalloc(&s->sd.claim, s->sd.max_requests * sizeof(unsigned)) != hipSuccess ||
+      hipMalloc(&s->sd.done, s->sd.max_requests * sizeof(unsigned)) != hipSuccess ||
+      hipMalloc(&s->sd.head, sizeof(unsigned long long)) != hipSuccess ||
+      hipMalloc(&s->sd.dtab, (size_t)max_inflight * sizeof(ReqDesc)) != hipSuccess ||
+      hipMalloc(&s->sd.dev_pub, sizeof(unsigned long long)) != hipSuccess ||
+      hipMalloc(&s->sd.dev_stop, sizeof(unsigned)) != hipSuccess)
+    rc = FFTOC_ERROR_ALLOC;
+  if (rc == FFTOC_SUCCESS &&
+      (hipMemset(s->sd.claim, 0, s->sd.max_requests * sizeof(unsigned)) != hipSuccess ||
+       hipMemset(s->sd.done, 0, s->sd.max_requests * sizeof(unsigned)) != hipSuccess ||
+       hipMemset(s->sd.head, 0, sizeof(unsigned long long)) != hipSuccess ||
+       hipMemset(s->sd.dev_pub, 0, sizeof(unsigned long long)) != hipSuccess ||
+       hipMemset(s->sd.dtab, 0xff, (size_t)max_inflight * sizeof(ReqDesc)) != hipSuccess))
+    rc = FFTOC_ERROR_DEVICE;
+  HostRing *ring_d = nullptr;
+  s->grid = server_grid();
+  if (rc == FFTOC_SUCCESS) rc = host_alloc<uint32_t>(&s->live, nullptr, &s->sd.live, s->grid);
+  if (rc == FFTOC_SUCCESS) rc = host_alloc<HostRing>(&s->ring, nullptr, &ring_d, 1);
+  if (rc == FFTOC_SUCCESS) rc = host_alloc<ReqDesc>(&s->descs, &s->sd.descs, nullptr, max_inflight);
+  if (rc == FFTOC_SUCCESS) rc = host_alloc<Completion>(&s->comps, nullptr, &s->sd.comps, max_inflight);
+  if (rc != FFTOC_SUCCESS) {
+    free_server(s);
+    return rc;
+  }
+  s->sd.ring = ring_d;
+  for (int i = 0; i < max_inflight; ++i) {
+    s->descs[i].seq = kSeqInvalid;
+    s->comps[i].seq = kSeqInvalid;
+  }
+  s->pool.reset(hbm_budget_bytes);
+  if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
+    free_server(s);
+    return FFTOC_ERROR_DEVICE;
+  }
+  std::atomic_thread_fence(std::memory_order_seq_cst);
+  rc = start_loop(s);
+  if (rc != FFTOC_SUCCESS) {
+    free_server(s);
+    return rc;
+  }
+  s->running = true;
+  *out = s;
+  return FFTOC_SUCCESS;
+}
+
+int fftoc_server_submit(fftoc_server s, const void *in, int64_t batch, int n, uint32_t flags,
+                        uint64_t *request_id) {
+  using namespace fftoc;
+  if (!s || !s->running) return FFTOC_ERROR_INVALID_HANDLE;
+  if (!request_id) return FFTOC_ERROR_NULL_POINTER;
+  const uint32_t known = FFTOC_INVERSE | FFTOC_NORMALIZE | FFTOC_DATA_C64;
+  if (flags & FFTOC_PREC_MASK) return FFTOC_ERROR_UNSUPPORTED;
+  if (flags & ~known) return FFTOC_ERROR_INVALID_FLAGS;
+  if (!pow2(n) || n < 8 || n > 4096) return FFTOC_ERROR_INVALID_SIZE;
+  if (batch <= 0) return FFTOC_ERROR_INVALID_BATCH;
+  if (!in) return FFTOC_ERROR_NULL_POINTER;
+  const bool c64 = (flags & FFTOC_DATA_C64) != 0;
+  const size_t es = c64 ? 8 : 16;
+  if (reinterpret_cast<uintptr_t>(in) & (es - 1)) return FFTOC_ERROR_MISALIGNED;
+  const size_t bytes = (size_t)batch * (size_t)n * es;
+  if (bytes / es / (size_t)n != (size_t)batch) return FFTOC_ERROR_INVALID_BATCH;
+  std::lock_guard<std::mutex> lock(s->m);
+  if (bytes > s->pool.capacity()) return FFTOC_ERROR_BUDGET;
+  if (s->next_seq >= s->sd.max_requests) return FFTOC_ERROR_QUEUE_FULL;
+  const int slot = (int)(s->next_seq % (uint64_t)s->nslots);
+  if (s->slots[slot].busy) return FFTOC_ERROR_QUEUE_FULL;  // ring full
+  size_t off = 0;
+  if (!s->pool.alloc(bytes, &off)) return FFTOC_ERROR_QUEUE_FULL;  // budget in use
+  const uint64_t seq = s->next_seq++;
+  Slot &sl = s->slots[slot];
+  sl.busy = true;
+  sl.seq = seq;
+  sl.out_off = off;
+  sl.out_bytes = bytes;
+  s->slot_of[seq] = slot;
+
+  const int id = s->routine[lg(n)][c64 ? 1 : 0];
+  const int S = batch_routine_signals(id);
+  const int64_t ntiles = (batch + S - 1) / S;
+  // items of about 256 KiB of input, but spread over the workers
+  const int64_t tile_bytes = (int64_t)S * n * (int64_t)es;
+  const int64_t cap = std::max<int64_t>(1, 262144 / tile_bytes);
+  const int64_t spread = std::max<int64_t>(1, ntiles / (2 * (int64_t)(s->grid - 1)));
+  const int32_t tpi = (int32_t)std::min(cap, spread);
+  ReqDesc &d = s->descs[slot];
+  volatile uint64_t *seqp = &d.seq;
+  *seqp = kSeqInvalid;
+  std::atomic_thread_fence(std::memory_order_seq_cst);
+  d.in = in;
+  d.out = s->d_pool + off;
+  d.batch = batch;
+  d.ntiles = ntiles;
+  d.nitems = (ntiles + tpi - 1) / tpi;
+  d.n = n;
+  d.routine = id;
+  d.tiles_per_item = tpi;
+  d.flags = ((flags & FFTOC_INVERSE) ? 1 : 0) | (c64 ? 2 : 0);
+  d.scale = (flags & FFTOC_NORMALIZE) ? 1.0 / (double)n : 1.0;
+  s->comps[slot].seq = kSeqInvalid;
+  std::atomic_thread_fence(std::memory_order_seq_cst);
+  *seqp = seq;
+  std::atomic_thread_fence(std::memory_order_seq_cst);
+  __atomic_store_n(&s->ring->published, seq + 1, __ATOMIC_RELEASE);
+  *request_id = seq;
+  return ensure_running(s);
+}
+
+// *done = 1 and *out = the result (device pointer, valid until release) once
+// the request has completed; *done = 0 while it runs.
+int fftoc_server_poll(fftoc_server s, uint64_t request_id, int *done, void **out) {
+  if (!s) return FFTOC_ERROR_INVALID_HANDLE;
+  if (!done) return FFTOC_ERROR_NULL_POINTER;
+  std::lock_guard<std::mutex> lock(s->m);
+  auto it = s->slot_of.find(request_id);
+  if (it == s->slot_of.end()) return FFTOC_ERROR_INVALID_HANDLE;
+  const int slot = it->second;
+  const uint64_t seq = __atomic_load_n(&s->comps[slot].seq, __ATOMIC_ACQUIRE);
+  *done = (seq == request_id) ? 1 : 0;
+  if (out) *out = *done ? (void *)(s->d_pool + s->slots[slot].out_off) : nullptr;
+  if (*done) return FFTOC_SUCCESS;
+  // a pending request needs a running loop; check at most once per millisecond
+  const double t = fftoc::now_s();
+  if (t - s->last_check < 1e-3) return FFTOC_SUCCESS;
+  s->last_check = t;
+  return fftoc::ensure_running(s);
+}
+
+// Returns a completed request's output memory to the budget and its slot to
+// the ring.
+int fftoc_server_release(fftoc_server s, uint64_t request_id) {
+  if (!s) return FFTOC_ERROR_INVALID_HANDLE;
+  std::lock_guard<std::mutex> lock(s->m);
+  auto it = s->slot_of.find(request_id);
+  if (it == s->slot_of.end()) return FFTOC_ERROR_INVALID_HANDLE;
+  const int slot = it->second;
+  if (__atomic_load_n(&s->comps[slot].seq, __ATOMIC_ACQUIRE) != request_id)
+    return FFTOC_ERROR_UNSUPPORTED;  // still running
+  Slot &sl = s->slots[slot];
+  s->pool.release(sl.out_off, sl.out_bytes);
+  sl.busy = false;
+  s->slot_of.erase(it);
+  return FFTOC_SUCCESS;
+}
+
+// Device time (100 MHz ticks) at which a completed request finished.
+int fftoc_server_gpu_time(fftoc_server s, uint64_t request_id, uint64_t *ticks) {
+  if (!s || !ticks) return FFTOC_ERROR_NULL_POINTER;
+  std::lock_guard<std::mutex> lock(s->m);
+  auto it = s->slot_of.find(request_id);
+  if (it == s->slot_of.end()) return FFTOC_ERROR_INVALID_HANDLE;
+  const fftoc::Completion &c = s->comps[it->second];
+  if (__atomic_load_n(&c.seq, __ATOMIC_ACQUIRE) != request_id) return FFTOC_ERROR_UNSUPPORTED;
+  *ticks = __atomic_load_n(&c.gpu_time, __ATOMIC_RELAXED);
+  return FFTOC_SUCCESS;
+}
+
+// Replays a request trace against the server: request i arrives at
+// arrival_s[i] seconds after the call starts; requests are admitted in
+// arrival order as slots and budget free up, completions are collected and
+// released.  latency_s[i] = completion seen by the host - arrival.
+int fftoc_server_replay(fftoc_server s, const fftoc_job *reqs, int count, const double *arrival_s,
+                        double *latency_s, double *span_s) {
+  using namespace fftoc;
+  if (!s || !reqs || !arrival_s || !latency_s) return FFTOC_ERROR_NULL_POINTER;
+  for (int i = 0; i < count; ++i) {
+    if (reqs[i].batch <= 0 || !reqs[i].in) return FFTOC_ERROR_INVALID_BATCH;
+    if ((size_t)reqs[i].batch * (size_t)reqs[i].n * ((reqs[i].flags & FFTOC_DATA_C64) ? 8 : 16) >
+        s->pool.capacity())
+      return FFTOC_ERROR_BUDGET;
+  }
+  std::deque<int> waiting;
+  struct Live {
+    uint64_t id;
+    int slot;
+    int index;
+  };
+  std::vector<Live> inflight;
+  int next = 0, completed = 0, rc = FFTOC_SUCCESS;
+  double last_check = 0;
+  const double t0 = now_s();
+  while (completed < count && rc == FFTOC_SUCCESS) {
+    const double now = now_s() - t0;
+    while (next < count && arrival_s[next] <= now) waiting.push_back(next++);
+    while (!waiting.empty()) {
+      const fftoc_job &r = reqs[waiting.front()];
+      uint64_t id = 0;
+      const int src = fftoc_server_submit(s, r.in, r.batch, r.n, r.flags, &id);
+      if (src == FFTOC_ERROR_QUEUE_FULL) break;
+      if (src != FFTOC_SUCCESS) {
+        rc = src;
+        break;
+      }
+      inflight.push_back({id, (int)(id % (uint64_t)s->nslots), waiting.front()});
+      waiting.pop_front();
+    }
+    // one pass over the completion records, one lock
+    std::lock_guard<std::mutex> lock(s->m);
+    for (size_t k = 0; k < inflight.size();) {
+      const Live &lv = inflight[k];
+      if (__atomic_load_n(&s->comps[lv.slot].seq, __ATOMIC_ACQUIRE) == lv.id) {
+        latency_s[lv.index] = (now_s() - t0) - arrival_s[lv.index];
+        Slot &sl = s->slots[lv.slot];
+        s->pool.release(sl.out_off, sl.out_bytes);
+        sl.busy = false;
+        s->slot_of.erase(lv.id);
+        inflight[k] = inflight.back();
+        inflight.pop_back();
+        ++completed;
+      } else {
+        ++k;
+      }
+    }
+    if (!inflight.empty()) {
+      const double t = now_s();
+      if (t - last_check > 1e-3) {
+        last_check = t;
+        rc = ensure_running(s);
+      }
+    }
+  }
+  if (span_s) *span_s = now_s() - t0;
+  return rc;
+}
+
+int fftoc_server_stats(fftoc_server s, size_t *budget_used, int *inflight) {
+  if (!s) return FFTOC_ERROR_INVALID_HANDLE;
+  std::lock_guard<std::mutex> lock(s->m);
+  if (budget_used) *budget_used = s->pool.used();
+  if (inflight) *inflight = (int)s->slot_of.size();
+  return FFTOC_SUCCESS;
+}
+
+int fftoc_server_destroy(fftoc_server s) {
+  if (!s) return FFTOC_ERROR_INVALID_HANDLE;
+  int cur = 0;
+  (void)hipGetDevice(&cur);
+  (void)hipSetDevice(s->device);
+  int rc = FFTOC_SUCCESS;
+  if (s->running) {
+    __atomic_store_n(&s->ring->stop, 1u, __ATOMIC_RELEASE);
+    if (hipStreamSynchronize(s->stream) != hipSuccess) rc = FFTOC_ERROR_DEVICE;
+    s->running = false;
+  }
+  fftoc::free_server(s);
+  (void)hipSetDevice(cur);
+  return rc;
+}
+
+}  // extern "C"

@@ -61,6 +61,13 @@ def device_count() -> int:
 
 
 def synchronize() -> None:
+    """Waits for the legacy default stream (and the blocking streams it
+    orders with).  Unlike hipDeviceSynchronize it does not wait for work on
+    non-blocking streams, such as a running server loop."""
+    check(lib().hipStreamSynchronize(None), "hipStreamSynchronize")
+
+
+def device_synchronize() -> None:
     check(lib().hipDeviceSynchronize(), "hipDeviceSynchronize")
 
 
@@ -68,6 +75,14 @@ def mem_info():
     free, total = ctypes.c_size_t(0), ctypes.c_size_t(0)
     check(lib().hipMemGetInfo(ctypes.byref(free), ctypes.byref(total)), "hipMemGetInfo")
     return free.value, total.value
+
+
+def download_ptr(ptr: int, shape, dtype=np.complex128) -> np.ndarray:
+    """Copies device memory at a raw address into a new host array."""
+    out = np.empty(shape, dtype=dtype)
+    check(lib().hipMemcpy(out.ctypes.data_as(ctypes.c_void_p), ctypes.c_void_p(ptr), out.nbytes,
+                          hipMemcpyDeviceToHost), "hipMemcpy D2H")
+    return out
 
 
 class DeviceBuffer:

@@ -34,6 +34,7 @@
 #ifndef FFTOC_H
 #define FFTOC_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -170,6 +171,39 @@ int fftoc_plan_autotune(fftoc_plan plan, int reps, void *stream);
 /* Name of the tile routine the plan uses for (n, data type of flags). */
 int fftoc_plan_routine(fftoc_plan plan, int n, uint32_t flags, const char **name);
 int fftoc_plan_destroy(fftoc_plan plan);
+
+/* ---- serving: device-resident request loop with HBM-budget admission -----
+ * No counterpart in the reference.  fftoc_server_create launches a
+ * persistent kernel (one workgroup per CU) that serves requests posted
+ * through a ring in pinned host memory until fftoc_server_destroy.  A request
+ * is admitted when a ring slot is free and its output fits in the free part
+ * of the HBM budget (the server's output pool); requests that arrive while
+ * others run start on idle workgroups at once.  While a server runs,
+ * hipDeviceSynchronize() waits for the server to stop: synchronise streams
+ * instead. */
+typedef struct fftoc_server_t *fftoc_server;
+
+/* hbm_budget_bytes: size of the output pool; max_inflight: ring slots. */
+int fftoc_server_create(size_t hbm_budget_bytes, int max_inflight, fftoc_server *server);
+/* Admits one request (flags: FFTOC_INVERSE, FFTOC_NORMALIZE, FFTOC_DATA_C64).
+ * FFTOC_ERROR_QUEUE_FULL: no slot or budget free now (retry later);
+ * FFTOC_ERROR_BUDGET: the output can never fit the budget. */
+int fftoc_server_submit(fftoc_server server, const void *in, int64_t batch, int n,
+                        uint32_t flags, uint64_t *request_id);
+/* *done = 1 and *out = device pointer of the result once complete. */
+int fftoc_server_poll(fftoc_server server, uint64_t request_id, int *done, void **out);
+/* Returns a completed request's output memory and ring slot. */
+int fftoc_server_release(fftoc_server server, uint64_t request_id);
+/* Device clock (100 MHz ticks) at which a completed request finished. */
+int fftoc_server_gpu_time(fftoc_server server, uint64_t request_id, uint64_t *ticks);
+/* Replays a trace: request i (in, batch, n, flags of reqs[i]; out ignored)
+ * arrives arrival_s[i] seconds after the call; admission in arrival order;
+ * latency_s[i] = completion seen by the host - arrival; *span_s = wall time. */
+int fftoc_server_replay(fftoc_server server, const fftoc_job *reqs, int count,
+                        const double *arrival_s, double *latency_s, double *span_s);
+int fftoc_server_stats(fftoc_server server, size_t *budget_used, int *inflight);
+/* Stops the loop (waits for it to drain) and frees everything. */
+int fftoc_server_destroy(fftoc_server server);
 
 #ifdef __cplusplus
 }
