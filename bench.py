@@ -38,16 +38,19 @@ def gen_signals(n: int, batch: int, seed: int, kind: str) -> np.ndarray:
     g = np.random.Generator(np.random.Philox(seed))
     x = np.empty((batch, n), dtype=np.complex128)
     v = x.view(np.float64)
-    if kind == "normal":
+    if kind in ("normal", "stress"):
         v[...] = g.standard_normal(v.shape)
+        if kind == "stress":  # dynamic-range stress: 1% of the entries x 1e4
+            x[g.random((batch, n)) < 0.01] *= 1e4
     else:
         v[...] = g.uniform(-1.0, 1.0, v.shape)
     return x
 
 
 class Workload:
-    def __init__(self, name, n, batch, seed, kind, path):
+    def __init__(self, name, n, batch, seed, kind, path, flags=0):
         self.name, self.n, self.batch, self.seed, self.kind, self.path = name, n, batch, seed, kind, path
+        self.flags = flags
 
 
 def run_gpu(fo, wl: Workload, steps: int, warmup: int, host_x: np.ndarray | None = None):
@@ -70,11 +73,11 @@ def run_gpu(fo, wl: Workload, steps: int, warmup: int, host_x: np.ndarray | None
     t_ramp = time.perf_counter()
     while (time.perf_counter() - t_ramp) * 1e3 < RAMP_MS:
         for _ in range(8):
-            fn(ins[ramp % pairs], batch, n, out=outs[ramp % pairs], stream=stream)
+            fn(ins[ramp % pairs], batch, n, out=outs[ramp % pairs], flags=wl.flags, stream=stream)
             ramp += 1
         stream.synchronize()
     for i in range(warmup):
-        fn(ins[i % pairs], batch, n, out=outs[i % pairs], stream=stream)
+        fn(ins[i % pairs], batch, n, out=outs[i % pairs], flags=wl.flags, stream=stream)
     stream.synchronize()
     ev = [(hip.Event(), hip.Event()) for _ in range(steps)]
     t_start, t_end = hip.Event(), hip.Event()
@@ -83,7 +86,7 @@ def run_gpu(fo, wl: Workload, steps: int, warmup: int, host_x: np.ndarray | None
     t_start.record(stream)
     for i in range(steps):
         ev[i][0].record(stream)
-        fn(ins[i % pairs], batch, n, out=outs[i % pairs], stream=stream)
+        fn(ins[i % pairs], batch, n, out=outs[i % pairs], flags=wl.flags, stream=stream)
         ev[i][1].record(stream)
     t_end.record(stream)
     stream.synchronize()
@@ -109,6 +112,45 @@ def run_gpu(fo, wl: Workload, steps: int, warmup: int, host_x: np.ndarray | None
         "GFLOPps": flops / (mean_launch_ms * 1e-3) / 1e9,
         "rotation_pairs": pairs, "clock_ramp_launches": ramp,
     }, x, sample
+
+
+def run_precision(fo, steps: int, warmup: int):
+    """C4: FP64 vs FP32 vs 3-term split-BF16 MFMA stage GEMMs on the
+    dynamic-range stress inputs (seed 11), N = 64 x 65536 and 4096 x 16384.
+    Errors here are against the FP64 GPU result of the same signals (itself
+    within 1e-13 of the oracle); the CPU leg adds errors against the oracle."""
+    out, samples = {}, {}
+    for n, batch in ((64, 65536), (4096, 16384)):
+        x = gen_signals(n, batch, 11, "stress")
+        ref = None
+        for prec, flag in (("fp64", 0), ("fp32", fo.PREC_FP32), ("bf16x3", fo.PREC_BF16X3)):
+            wl = Workload(f"c4_n{n}_{prec}", n, batch, 11, "stress", "tensor", flag)
+            r, _, sample = run_gpu(fo, wl, steps, warmup, host_x=x)
+            if ref is None:
+                ref = sample
+            d = np.sqrt(np.sum(np.abs(sample - ref) ** 2, axis=-1) / np.sum(np.abs(ref) ** 2, axis=-1))
+            entry = {k: (round(v, 6) if isinstance(v, float) else v) for k, v in r.items()}
+            entry.update({"rel_l2_vs_fp64_max": float(d.max()), "rel_l2_vs_fp64_median": float(np.median(d)),
+                          "max_abs_vs_fp64": float(np.max(np.abs(sample - ref))),
+                          "checked_signals": int(sample.shape[0])})
+            out[wl.name] = entry
+            samples[wl.name] = (x[: sample.shape[0]], sample)
+            print(f"[bench] {wl.name}: {r['ffts_per_s']:.4g} FFT/s, {r['GBps']:.0f} GB/s, "
+                  f"rel L2 vs fp64 {d.max():.2e}", file=sys.stderr, flush=True)
+    return out, samples
+
+
+def precision_vs_oracle(samples, count: int = 64):
+    """Per-result error of each C4 run against the long-double oracle (the
+    CPU leg, where the oracle is the checker)."""
+    import oracle
+    res = {}
+    for name, (x, got) in samples.items():
+        ref = oracle.reference(x[:count])
+        e = oracle.rel_l2(got[:count], ref, axis=-1)
+        res[name] = {"rel_l2_max": float(e.max()), "rel_l2_median": float(np.median(e)),
+                     "max_abs": float(oracle.max_abs(got[:count], ref)), "signals": count}
+    return res
 
 
 def c5_requests(count: int, seed: int, max_batch: int = 4096):
@@ -386,7 +428,9 @@ def main():
             extra[wl.name] = {k: (round(v, 6) if isinstance(v, float) else v) for k, v in r.items()}
             print(f"[bench] {wl.name}: {r['ffts_per_s']:.4g} FFT/s, {r['GBps']:.0f} GB/s", file=sys.stderr, flush=True)
     extra["hip_d2d_copy_GBps"] = round(copy_bandwidth(), 1)
+    prec_samples = {}
     if not args.quick:
+        extra["precision_c4"], prec_samples = run_precision(fo, args.steps, args.warmup)
         extra["batched_mixed_c5"] = run_batched(fo, args.steps, args.warmup)
         print(f"[bench] batched plan: {extra['batched_mixed_c5']}", file=sys.stderr, flush=True)
         extra["served_c5"] = run_served(fo)
@@ -396,6 +440,8 @@ def main():
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         cpu = cpu_baseline(x, sample, threads)
+        if prec_samples:
+            extra["precision_c4_vs_oracle"] = precision_vs_oracle(prec_samples)
         print(f"[bench] cpu baseline: {cpu['value']:.4g} FFT/s on {threads} threads", file=sys.stderr, flush=True)
 
     traffic = load_traffic(head.name)

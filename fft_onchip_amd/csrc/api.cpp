@@ -231,7 +231,8 @@ int run(Family fam, const void *in, void *out, int64_t batch, int n, uint32_t fl
       break;
     case Family::Tensor:
       if (n == 1024 || n == 2048) return FFTOC_ERROR_INVALID_SIZE;
-      if (a.precision != FFTOC_PREC_FP64) return FFTOC_ERROR_UNSUPPORTED;
+      if (a.precision != FFTOC_PREC_FP64 && (a.c64 || (n != 64 && n != 256 && n != 4096)))
+        return FFTOC_ERROR_UNSUPPORTED;
       break;
     case Family::Best:
       if (a.precision != FFTOC_PREC_FP64) return FFTOC_ERROR_UNSUPPORTED;
@@ -247,6 +248,11 @@ int run(Family fam, const void *in, void *out, int64_t batch, int n, uint32_t fl
   rc = init_current_device(&dev);
   if (rc != FFTOC_SUCCESS) return rc;
   a.twiddles = a.c64 ? g_dev[dev].twiddles32 : g_dev[dev].twiddles;
+  a.twiddles32 = g_dev[dev].twiddles32;
+  if (fam == Family::Tensor && a.precision != FFTOC_PREC_FP64) {
+    a.twiddles = g_dev[dev].twiddles;
+    return launch_tensor_precision(n, a);
+  }
   return reg->v[pick].launch(a);
 }
 
@@ -275,7 +281,8 @@ int autotune(int n, int64_t batch, int reps, void *stream_p) {
     rc = FFTOC_ERROR_DEVICE;
   std::vector<double> us(reg->v.size(), -1.0);
   for (size_t i = 0; i < reg->v.size() && rc == FFTOC_SUCCESS; ++i) {
-    LaunchArgs a{in, out, batch, g_dev[dev].twiddles, stream, false, 1.0, 0, FFTOC_PREC_FP64, false};
+    LaunchArgs a{in, out, batch, g_dev[dev].twiddles, g_dev[dev].twiddles32, stream, false, 1.0, 0,
+                 FFTOC_PREC_FP64, false};
     if (reg->v[i].launch(a) != FFTOC_SUCCESS) continue;  // warm-up
     std::vector<float> t;
     for (int r = 0; r < reps; ++r) {

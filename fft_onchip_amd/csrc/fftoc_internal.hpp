@@ -14,6 +14,7 @@ struct LaunchArgs {
   int64_t batch;
   const void *twiddles;  // device table W_4096^j, j = 0..4095 (double2, or
                          // float2 for complex64 data)
+  const void *twiddles32;  // the same table rounded to float2
   hipStream_t stream;
   bool inverse;
   double scale;          // 1.0, or 1/N with FFTOC_NORMALIZE
@@ -21,6 +22,9 @@ struct LaunchArgs {
   uint32_t precision;    // FFTOC_PREC_* bits
   bool c64;              // complex64 data (FFTOC_DATA_C64)
 };
+
+// mfma_prec_kernels.hip: FP32 / split-BF16 stage GEMMs (n = 64, 256, 4096)
+int launch_tensor_precision(int n, const LaunchArgs &a);
 
 // Per-device tables shared with the plan / serving code (api.cpp).
 int device_tables(const void **tw64, const void **tw32);
