@@ -32,8 +32,8 @@ SIGNATURES.update({
     "fftoc_variant_count": ([_int], _int),
     "fftoc_variant_name": ([_int, _int], ctypes.c_char_p),
     "fftoc_fft_variant": ([_vp, _vp, _i64, _int, _u32, _int, _vp], _int),
-    "fftoc_get_variant": ([_int], _int),
-    "fftoc_set_variant": ([_int, _int], _int),
+    "fftoc_get_variant": ([_int, _i64], _int),
+    "fftoc_set_variant": ([_int, _i64, _int], _int),
     "fftoc_autotune": ([_int, _i64, _int, _vp], _int),
     "fftoc_variant_time_us": ([_int, _int], ctypes.c_double),
     "fftoc_tuning_save": ([ctypes.c_char_p], _int),

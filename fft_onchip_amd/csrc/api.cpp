@@ -121,9 +121,16 @@ int family_index(Family f) { return (int)f; }
 
 // ---- variant registry and tuning table --------------------------------------
 // Variants of length n = 2^k (k = 3..12): tensor variants first, then radix.
+constexpr int kBuckets = 32;  // batch bucket = floor(log2(batch)), 0..31
+int batch_bucket(int64_t batch) {
+  if (batch <= 1) return 0;
+  const int b = 63 - __builtin_clzll((unsigned long long)batch);
+  return b < kBuckets ? b : kBuckets - 1;
+}
+
 struct SizeRegistry {
   std::vector<Variant> v;
-  std::atomic<int> choice[4];  // per Family: index into v, -1 = none
+  std::atomic<int> choice[4][kBuckets];  // [Family][batch bucket] -> index into v
   std::vector<double> tuned_us;  // last autotune time per variant (<0: not run)
 };
 
@@ -133,12 +140,13 @@ std::once_flag g_reg_once;
 // Default choice of fftoc_fft per length, from MI355X measurements of the
 // variants at 256 MiB - 1 GiB batches (replaced by fftoc_autotune /
 // fftoc_tuning_load).
-const char *default_best(int n) {
+const char *default_best(int n, int bucket) {
   switch (n) {
+    case 64:  // up to 2^16 signals the persistent form hides the launch tail
+      return bucket <= 16 ? "radix:8x8/persistent/S32" : "radix:8x8/tile/S16";
     case 8: return "radix:4x2/staged/S128";
     case 16: return "radix:4x4/staged/S64";
     case 32: return "tensor:16x2/8 per wave";
-    case 64: return "radix:8x8/tile/S16";
     case 128: return "radix:16x8/tile/S8";
     case 256: return "radix:16x16/tile/S4";
     case 512: return "radix:8x8x8/tile/S1";
@@ -159,18 +167,21 @@ void build_registry() {
     r.tuned_us.assign(r.v.size(), -1.0);
     const Family fams[4] = {Family::Tensor, Family::Radix8, Family::Radix16, Family::Best};
     for (Family f : fams) {
-      int pick = -1;
+      int first = -1;
       for (size_t i = 0; i < r.v.size(); ++i) {
         if (r.v[i].families & family_bit(f)) {
-          pick = (int)i;
+          first = (int)i;
           break;
         }
       }
-      if (f == Family::Best && !c64) {
-        for (size_t i = 0; i < r.v.size(); ++i)
-          if (std::strcmp(r.v[i].name, default_best(n)) == 0) pick = (int)i;
+      for (int b = 0; b < kBuckets; ++b) {
+        int pick = first;
+        if (f == Family::Best && !c64) {
+          for (size_t i = 0; i < r.v.size(); ++i)
+            if (std::strcmp(r.v[i].name, default_best(n, b)) == 0) pick = (int)i;
+        }
+        r.choice[family_index(f)][b].store(pick);
       }
-      r.choice[family_index(f)].store(pick);
     }
   }
 }
@@ -240,7 +251,8 @@ int run(Family fam, const void *in, void *out, int64_t batch, int n, uint32_t fl
   }
   SizeRegistry *reg = registry(n, a.c64);
   if (!reg) return FFTOC_ERROR_INVALID_SIZE;
-  const int pick = variant >= 0 ? variant : reg->choice[family_index(fam)].load();
+  const int pick =
+      variant >= 0 ? variant : reg->choice[family_index(fam)][batch_bucket(batch)].load();
   if (pick < 0 || pick >= (int)reg->v.size())
     return a.c64 ? FFTOC_ERROR_UNSUPPORTED : FFTOC_ERROR_INVALID_SIZE;
   if (batch == 0) return FFTOC_SUCCESS;
@@ -312,7 +324,7 @@ int autotune(int n, int64_t batch, int reps, void *stream_p) {
     for (size_t i = 0; i < us.size(); ++i)
       if ((reg->v[i].families & family_bit(f)) && us[i] >= 0 && (best < 0 || us[i] < us[best]))
         best = (int)i;
-    if (best >= 0) reg->choice[family_index(f)].store(best);
+    if (best >= 0) reg->choice[family_index(f)][batch_bucket(batch)].store(best);
   }
   return FFTOC_SUCCESS;
 }
@@ -436,16 +448,18 @@ int fftoc_fft_variant(const void *in, void *out, int64_t batch, int n, uint32_t 
   return fftoc::run(fam, in, out, batch, n, flags, stream, variant);
 }
 
-int fftoc_get_variant(int n) {
+int fftoc_get_variant(int n, int64_t batch) {
   fftoc::SizeRegistry *r = fftoc::registry(n);
-  return r ? r->choice[fftoc::family_index(Family::Best)].load() : -1;
+  return r ? r->choice[fftoc::family_index(Family::Best)][fftoc::batch_bucket(batch)].load() : -1;
 }
 
-int fftoc_set_variant(int n, int variant) {
+int fftoc_set_variant(int n, int64_t batch, int variant) {
   fftoc::SizeRegistry *r = fftoc::registry(n);
   if (!r) return FFTOC_ERROR_INVALID_SIZE;
   if (variant < 0 || variant >= (int)r->v.size()) return FFTOC_ERROR_INVALID_HANDLE;
-  r->choice[fftoc::family_index(Family::Best)].store(variant);
+  for (int b = 0; b < fftoc::kBuckets; ++b)
+    if (batch < 0 || b == fftoc::batch_bucket(batch))
+      r->choice[fftoc::family_index(Family::Best)][b].store(variant);
   return FFTOC_SUCCESS;
 }
 
@@ -459,18 +473,21 @@ double fftoc_variant_time_us(int n, int variant) {
   return r->tuned_us[variant];
 }
 
-// Text format, one line per (entry point, length):
-//   <family> <n> <variant name up to end of line>
+// Text format, one line per (entry point, length, batch bucket):
+//   <family> <n> <floor(log2 batch)> <variant name up to end of line>
 int fftoc_tuning_save(const char *path) {
   if (!path) return FFTOC_ERROR_NULL_POINTER;
   FILE *f = std::fopen(path, "w");
   if (!f) return FFTOC_ERROR_INVALID_HANDLE;
-  std::fprintf(f, "# fftoc tuning table: family n variant\n");
+  std::fprintf(f, "# fftoc tuning table: family n log2(batch) variant\n");
   for (int k = 3; k <= 12; ++k) {
     fftoc::SizeRegistry *r = fftoc::registry(1 << k);
     for (int fi = 0; fi < 4; ++fi) {
-      const int c = r->choice[fi].load();
-      if (c >= 0) std::fprintf(f, "%s %d %s\n", fftoc::kFamilyNames[fi], 1 << k, r->v[c].name);
+      for (int b = 0; b < fftoc::kBuckets; ++b) {
+        const int c = r->choice[fi][b].load();
+        if (c >= 0)
+          std::fprintf(f, "%s %d %d %s\n", fftoc::kFamilyNames[fi], 1 << k, b, r->v[c].name);
+      }
     }
   }
   const int err = std::ferror(f);
@@ -487,8 +504,9 @@ int fftoc_tuning_load(const char *path) {
   while (std::fgets(line, sizeof line, f)) {
     if (line[0] == '#' || line[0] == '\n') continue;
     char fam[32];
-    int n = 0, used = 0;
-    if (std::sscanf(line, "%31s %d %n", fam, &n, &used) != 2) {
+    int n = 0, bucket = 0, used = 0;
+    if (std::sscanf(line, "%31s %d %d %n", fam, &n, &bucket, &used) != 3 || bucket < 0 ||
+        bucket >= fftoc::kBuckets) {
       rc = FFTOC_ERROR_INVALID_HANDLE;
       continue;
     }
@@ -513,7 +531,7 @@ int fftoc_tuning_load(const char *path) {
       rc = FFTOC_ERROR_INVALID_HANDLE;
       continue;
     }
-    r->choice[fi].store(match);
+    r->choice[fi][bucket].store(match);
   }
   std::fclose(f);
   return rc;

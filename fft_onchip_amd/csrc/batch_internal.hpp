@@ -56,20 +56,28 @@ struct ServerDev {
   uint32_t *live;          // [grid] in host memory: 1 while that workgroup runs
   const ReqDesc *descs;    // [nslots] host memory, read by the dispatcher only
   ReqDesc *dtab;           // [nslots] device copy of the descriptors
-  unsigned long long *dev_pub;   // requests whose dtab entry is valid
+  unsigned long long *dev_pub;   // requests copied into dtab (dispatcher state)
+  unsigned long long *witems;    // [kWorkRing] work items: request << 24 | item
+  unsigned long long *wpub;      // work items published
+  unsigned long long *wnext;     // next work ticket
   unsigned *dev_stop;      // the dispatcher has left: workers leave when idle
   Completion *comps;       // [nslots]
   unsigned *claim;         // [max_requests] items handed out per request
   unsigned *done;          // [max_requests] items finished per request
   unsigned long long *head;  // lowest request that may still have items
   int nslots;
+  unsigned *cu_seats;      // [kCuKeys] workers seated per CU in this launch
+  int claim_backlog;       // unclaimed items above which secondary workers join
   uint64_t max_requests;
   uint64_t idle_timeout_ticks;  // wall_clock64 ticks without work before exit
   const void *tw64;
   const void *tw32;
 };
+constexpr uint64_t kWorkRing = 1ull << 22;  // work-item ring entries
+constexpr int kItemBits = 24;                // items per request < 2^24
+constexpr int kCuKeys = 2048;                // (XCC, SE, SH, CU) of a workgroup
 int launch_server(const ServerDev &sd, int grid, hipStream_t stream);
-int server_grid();
+int server_grid(int max_per_cu);
 
 int batch_routine_count();
 int batch_routine_n(int id);

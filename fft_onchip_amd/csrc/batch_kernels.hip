@@ -179,18 +179,20 @@ __device__ bool read_dtab(const ReqDesc *e, uint64_t c, JobDev &J) {
   return true;
 }
 
-enum ServeState : int { kWork = 0, kIdle = 1, kExit = 2 };
+enum ServeState : int { kWork = 0, kIdle = 1, kExit = 2, kIdleNoTicket = 3 };
 
 // Wave 0 of workgroup 0: moves newly published requests from the host ring
 // into the device table, up to 64 per round (one per lane, so one PCIe round
-// trip and three release fences per round).  Entries follow the sequence
-// lock: seq = invalid, fields, seq = id, each step ordered by a release
-// fence, then dev_pub advances.  Leaves when the host sets `stop` or after
+// trip per round), and expands them into the work-item ring.  Entries follow
+// the sequence lock seq = invalid, fields, seq = id, each step ordered by a
+// release fence; the work items are published (wpub) after their
+// descriptors.  Leaves when the host sets `stop` or after
 // idle_timeout_ticks without requests, then raises dev_stop so the workers
 // leave once idle.
 __device__ void dispatcher(const ServerDev &sd) {
   const int lane = threadIdx.x;  // 0..63
   uint64_t dpub = __hip_atomic_load(sd.dev_pub, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  uint64_t wp = __hip_atomic_load(sd.wpub, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   uint64_t idle_since = wall_clock64();
   const uint64_t ns = (uint64_t)sd.nslots;
   for (;;) {
@@ -217,11 +219,29 @@ __device__ void dispatcher(const ServerDev &sd) {
         }
         __atomic_thread_fence(__ATOMIC_RELEASE);
         if (lane < k) __hip_atomic_store(&e->seq, c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        // work items of these k requests: exclusive prefix sum of nitems
+        uint64_t ni = (lane < k) ? (uint64_t)d.nitems : 0;
+        uint64_t incl = ni;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+          const uint64_t v = __shfl_up(incl, off);
+          if (lane >= off) incl += v;
+        }
+        const uint64_t total = __shfl(incl, 63);
+        const uint64_t first = wp + incl - ni;
+        for (uint64_t i = 0; i < ni; ++i)
+          __hip_atomic_store(&sd.witems[(first + i) & (kWorkRing - 1)],
+                             (unsigned long long)((c << kItemBits) | i), __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
         __atomic_thread_fence(__ATOMIC_RELEASE);
+        wp += total;
         dpub += (uint64_t)k;
-        if (lane == 0)
+        if (lane == 0) {
+          __hip_atomic_store(sd.wpub, (unsigned long long)wp, __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
           __hip_atomic_store(sd.dev_pub, (unsigned long long)dpub, __ATOMIC_RELAXED,
                              __HIP_MEMORY_SCOPE_AGENT);
+        }
         idle_since = wall_clock64();
         continue;
       }
@@ -235,11 +255,36 @@ __device__ void dispatcher(const ServerDev &sd) {
   if (lane == 0) __hip_atomic_store(sd.dev_stop, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Persistent serving kernel.  Workgroup 0 is the dispatcher; the others claim
-// items of dispatched requests in id order through per-request counters
-// (never reused, so no reset races), run the request's tile routine and
-// count finished items; the workgroup that finishes a request's last item
-// publishes its completion record in host memory.
+// Identifies the CU a wave runs on from the hardware ID registers: XCC_ID
+// (hwreg 20) bits 2:0, HW_ID (hwreg 4) SE bits 15:13, SH bit 12, CU bits 11:8.
+// s_getreg's operand is id | offset << 6 | (size - 1) << 11.
+__device__ inline unsigned cu_key() {
+  const unsigned hw = __builtin_amdgcn_s_getreg(4 | (31 << 11));
+  const unsigned xcc = __builtin_amdgcn_s_getreg(20 | (31 << 11)) & 7u;
+  return (xcc << 8) | (((hw >> 13) & 7u) << 5) | (((hw >> 12) & 1u) << 4) | ((hw >> 8) & 15u);
+}
+
+// The first worker seated on a CU is its primary and takes work tickets in
+// order, holding one while it waits.  Any other worker on the CU is secondary:
+// it takes a ticket only while more published items wait unclaimed than there
+// are primaries, so at low load items run alone on their CUs and few workers
+// poll the queue, and under a backlog every slot works.  (Claiming single
+// published items by compare-and-swap instead starves the primaries' atomics.)
+__device__ inline bool claim_ticket(const ServerDev &sd, bool secondary, uint64_t &ticket) {
+  if (secondary) {
+    const uint64_t p = __hip_atomic_load(sd.wpub, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint64_t t = __hip_atomic_load(sd.wnext, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if ((int64_t)(p - t) <= (int64_t)sd.claim_backlog) return false;
+  }
+  ticket = atomicAdd(sd.wnext, 1ull);
+  return true;
+}
+
+// Persistent serving kernel.  Workgroup 0 is the dispatcher; each other
+// workgroup takes work tickets in order (one atomic), waits until its ticket
+// is published, reads the work item (request, item), runs the request's tile
+// routine and counts the finished item; the workgroup that finishes a
+// request's last item publishes its completion record in host memory.
 __global__ __launch_bounds__(kBatchThreads) __attribute__((amdgpu_waves_per_eu(2)))
 void serve_kernel(ServerDev sd) {
   if (blockIdx.x == 0) {
@@ -257,47 +302,47 @@ void serve_kernel(ServerDev sd) {
   __shared__ int s_state;
   // Scheduling state lives in thread 0; its decision reaches the workgroup
   // through s_state, so every wave takes the same branch.
-  uint64_t c = 0;          // request cursor
-  bool have = false;       // s_job holds request c
+  __shared__ unsigned s_seat;
+  if (threadIdx.x == 0) s_seat = atomicAdd(&sd.cu_seats[cu_key() & (kCuKeys - 1)], 1u);
+  __syncthreads();
+  const bool secondary = s_seat != 0;
+  bool has_ticket = false;
+  uint64_t ticket = 0;
+  bool have = false;       // s_job holds request cached_c
+  uint64_t cached_c = 0;
   uint64_t idle_since = wall_clock64();
   for (;;) {
     if (threadIdx.x == 0) {
       int state = kIdle;
-      for (int guard = 0; guard < 64; ++guard) {
-        const uint64_t h = __hip_atomic_load(sd.head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (h > c) {
-          c = h;
-          have = false;
-        }
-        if (c >= sd.max_requests) break;
-        if (!have) {
-          const uint64_t pub =
-              __hip_atomic_load(sd.dev_pub, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          if (c >= pub) break;  // nothing dispatched beyond the cursor
+      if (!has_ticket) has_ticket = claim_ticket(sd, secondary, ticket);
+      if (has_ticket &&
+          ticket < __hip_atomic_load(sd.wpub, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)) {
+        const uint64_t e = __hip_atomic_load(&sd.witems[ticket & (kWorkRing - 1)], __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT);
+        const uint64_t c = e >> kItemBits;
+        bool ready = have && cached_c == c;
+        if (!ready) {
           JobDev J;
-          if (!read_dtab(&sd.dtab[c % (uint64_t)sd.nslots], c, J)) {
-            // dispatched, but the entry no longer holds c: c finished and recycled
-            ++c;
-            continue;
+          ready = read_dtab(&sd.dtab[c % (uint64_t)sd.nslots], c, J);
+          if (ready) {
+            s_job = J;
+            have = true;
+            cached_c = c;
           }
-          s_job = J;
-          have = true;
         }
-        const unsigned item = atomicAdd(&sd.claim[c], 1u);
-        if ((int64_t)item < s_job.item_begin) {
-          s_item = item;
+        if (ready) {
+          s_item = (long long)(e & ((1ull << kItemBits) - 1));
           s_req = c;
+          has_ticket = false;
           state = kWork;
-          break;
         }
-        atomicMax(sd.head, (unsigned long long)(c + 1));
-        ++c;
-        have = false;
       }
       if (state == kIdle) {
         if (__hip_atomic_load(sd.dev_stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ||
             wall_clock64() - idle_since > sd.idle_timeout_ticks + 100000000ull)
           state = kExit;
+        else if (!has_ticket)
+          state = kIdleNoTicket;
       } else {
         idle_since = wall_clock64();
       }
@@ -306,8 +351,11 @@ void serve_kernel(ServerDev sd) {
     __syncthreads();
     const int state = s_state;
     if (state == kExit) break;
-    if (state == kIdle) {
-      __builtin_amdgcn_s_sleep(4);
+    if (state != kWork) {
+      if (state == kIdleNoTicket)  // a secondary looks for unclaimed items less often
+        __builtin_amdgcn_s_sleep(64);
+      else
+        __builtin_amdgcn_s_sleep(4);
       __syncthreads();  // s_state has been read before thread 0 rewrites it
       continue;
     }
@@ -315,6 +363,8 @@ void serve_kernel(ServerDev sd) {
     const int64_t item = s_item;
     const int64_t t0 = item * J.tiles_per_item;
     const int64_t t1 = (t0 + J.tiles_per_item < J.ntiles) ? t0 + J.tiles_per_item : J.ntiles;
+    if (threadIdx.x == 0 && !has_ticket)  // claimed before the work to hide its latency
+      has_ticket = claim_ticket(sd, secondary, ticket);
     for (int64_t t = t0; t < t1; ++t) {
       if (J.c64)
         run_routine<f2>(J.routine, J, t, reinterpret_cast<f2 *>(lds), static_cast<const f2 *>(sd.tw32));
@@ -342,18 +392,37 @@ void serve_kernel(ServerDev sd) {
     __hip_atomic_store(&sd.live[blockIdx.x], 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
+// Before each (re)launch: tickets handed out beyond the published items
+// belonged to workgroups that left while idle; start over from wpub.
+__global__ void serve_reset_kernel(ServerDev sd) {
+  for (int i = threadIdx.x; i < kCuKeys; i += blockDim.x) sd.cu_seats[i] = 0u;
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    const unsigned long long p =
+        __hip_atomic_load(sd.wpub, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(sd.wnext, p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(sd.dev_stop, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
 }  // namespace
 
-int server_grid() {
-  int dev = 0, cus = 0;
+int server_grid(int max_per_cu) {
+  // every resident slot (two workgroups per CU on gfx950: LDS and VGPRs),
+  // the dispatcher workgroup included
+  int per_cu = 0, dev = 0, cus = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, serve_kernel, kBatchThreads, 0) !=
+          hipSuccess || per_cu <= 0)
+    per_cu = 1;
+  if (max_per_cu > 0 && per_cu > max_per_cu) per_cu = max_per_cu;
   if (hipGetDevice(&dev) != hipSuccess ||
       hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
       cus <= 0)
     cus = 256;
-  return cus;  // one workgroup per CU leaves room for other work on the device
+  return per_cu * cus;
 }
 
 int launch_server(const ServerDev &sd, int grid, hipStream_t stream) {
+  hipLaunchKernelGGL(serve_reset_kernel, dim3(1), dim3(64), 0, stream, sd);
   hipLaunchKernelGGL(serve_kernel, dim3(grid), dim3(kBatchThreads), 0, stream, sd);
   return hipGetLastError() == hipSuccess ? FFTOC_SUCCESS : FFTOC_ERROR_LAUNCH;
 }

@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <map>
@@ -72,11 +73,16 @@ struct Slot {
   bool busy = false;
   uint64_t seq = 0;
   size_t out_off = 0, out_bytes = 0;
+  int64_t items = 0;
 };
 
 // The device loop exits after this long without work (so it never outlives
 // a vanished host); submit / poll relaunch it when it is needed again.
 constexpr uint64_t kIdleTimeoutTicks = 2ull * 100000000ull;  // 2 s at 100 MHz
+
+// Output bytes in flight above which the device counts as busy (about four
+// 256 KiB items per worker).
+constexpr size_t kDeepBacklogBytes = 256ull << 20;
 
 }  // namespace
 
@@ -94,8 +100,10 @@ struct fftoc_server_t {
   uint32_t *live = nullptr;         // pinned: per-workgroup "running" flags
   double last_check = 0;            // time of the last ensure_running from poll
   int grid = 0;
+  int spread_workers = 1;           // a request at low load is cut into this many items
   fftoc::ServerDev sd{};
   uint64_t next_seq = 0;
+  int64_t inflight_items = 0;       // bounded by the device work-item ring
   hipStream_t stream = nullptr;
   bool running = false;
   int routine[13][2];
@@ -116,6 +124,10 @@ void free_server(fftoc_server_t *s) {
   if (s->sd.dtab) (void)hipFree(s->sd.dtab);
   if (s->sd.dev_pub) (void)hipFree(s->sd.dev_pub);
   if (s->sd.dev_stop) (void)hipFree(s->sd.dev_stop);
+  if (s->sd.witems) (void)hipFree(s->sd.witems);
+  if (s->sd.wpub) (void)hipFree(s->sd.wpub);
+  if (s->sd.wnext) (void)hipFree(s->sd.wnext);
+  if (s->sd.cu_seats) (void)hipFree(s->sd.cu_seats);
   if (s->ring) (void)hipHostFree(s->ring);
   if (s->descs) (void)hipHostFree(s->descs);
   if (s->comps) (void)hipHostFree(s->comps);
@@ -148,9 +160,8 @@ int host_alloc(T **h, const T **d_const, T **d, size_t count) {
 int start_loop(fftoc_server_t *s) {
   for (int i = 0; i < s->grid; ++i) __atomic_store_n(&s->live[i], 1u, __ATOMIC_RELAXED);
   std::atomic_thread_fence(std::memory_order_seq_cst);
-  // ordered on the loop's stream, after the previous launch has ended
-  if (hipMemsetAsync(s->sd.dev_stop, 0, sizeof(unsigned), s->stream) != hipSuccess)
-    return FFTOC_ERROR_DEVICE;
+  // ordered on the loop's stream after the previous launch; launch_server
+  // resets the ticket counter and dev_stop first
   return launch_server(s->sd, s->grid, s->stream);
 }
 
@@ -211,17 +222,35 @@ int fftoc_server_create(size_t hbm_budget_bytes, int max_inflight, fftoc_server 
       hipMalloc(&s->sd.head, sizeof(unsigned long long)) != hipSuccess ||
       hipMalloc(&s->sd.dtab, (size_t)max_inflight * sizeof(ReqDesc)) != hipSuccess ||
       hipMalloc(&s->sd.dev_pub, sizeof(unsigned long long)) != hipSuccess ||
-      hipMalloc(&s->sd.dev_stop, sizeof(unsigned)) != hipSuccess)
+      hipMalloc(&s->sd.dev_stop, sizeof(unsigned)) != hipSuccess ||
+      hipMalloc(&s->sd.witems, kWorkRing * sizeof(unsigned long long)) != hipSuccess ||
+      hipMalloc(&s->sd.wpub, sizeof(unsigned long long)) != hipSuccess ||
+      hipMalloc(&s->sd.wnext, sizeof(unsigned long long)) != hipSuccess ||
+      hipMalloc(&s->sd.cu_seats, kCuKeys * sizeof(unsigned)) != hipSuccess)
     rc = FFTOC_ERROR_ALLOC;
   if (rc == FFTOC_SUCCESS &&
       (hipMemset(s->sd.claim, 0, s->sd.max_requests * sizeof(unsigned)) != hipSuccess ||
        hipMemset(s->sd.done, 0, s->sd.max_requests * sizeof(unsigned)) != hipSuccess ||
        hipMemset(s->sd.head, 0, sizeof(unsigned long long)) != hipSuccess ||
        hipMemset(s->sd.dev_pub, 0, sizeof(unsigned long long)) != hipSuccess ||
+       hipMemset(s->sd.wpub, 0, sizeof(unsigned long long)) != hipSuccess ||
        hipMemset(s->sd.dtab, 0xff, (size_t)max_inflight * sizeof(ReqDesc)) != hipSuccess))
     rc = FFTOC_ERROR_DEVICE;
   HostRing *ring_d = nullptr;
-  s->grid = server_grid();
+  {
+    // FFTOC_SERVER_WORKERS_PER_CU caps the workers per CU (default: as many as fit)
+    const char *e = getenv("FFTOC_SERVER_WORKERS_PER_CU");
+    s->grid = server_grid(e ? atoi(e) : 0);
+  }
+  {
+    // one item per CU: finer cuts only add per-item claims and dispatcher work
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+      cus = 0;
+    s->spread_workers = std::max(1, std::min(cus, s->grid - 1));
+    s->sd.claim_backlog = s->spread_workers;
+  }
   if (rc == FFTOC_SUCCESS) rc = host_alloc<uint32_t>(&s->live, nullptr, &s->sd.live, s->grid);
   if (rc == FFTOC_SUCCESS) rc = host_alloc<HostRing>(&s->ring, nullptr, &ring_d, 1);
   if (rc == FFTOC_SUCCESS) rc = host_alloc<ReqDesc>(&s->descs, &s->sd.descs, nullptr, max_inflight);
@@ -267,11 +296,26 @@ int fftoc_server_submit(fftoc_server s, const void *in, int64_t batch, int n, ui
   if (reinterpret_cast<uintptr_t>(in) & (es - 1)) return FFTOC_ERROR_MISALIGNED;
   const size_t bytes = (size_t)batch * (size_t)n * es;
   if (bytes / es / (size_t)n != (size_t)batch) return FFTOC_ERROR_INVALID_BATCH;
+  const int id = s->routine[lg(n)][c64 ? 1 : 0];
+  const int S = batch_routine_signals(id);
+  const int64_t ntiles = (batch + S - 1) / S;
   std::lock_guard<std::mutex> lock(s->m);
-  if (bytes > s->pool.capacity()) return FFTOC_ERROR_BUDGET;
+  // Work-item size follows the load: with little work queued a request is
+  // spread over the workers (latency); with a deep queue items hold about
+  // 256 KiB of input (throughput: fewer per-item claims and fences).
+  const int64_t tile_bytes = (int64_t)S * n * (int64_t)es;
+  const int64_t big = std::max<int64_t>(1, 262144 / tile_bytes);
+  const bool deep = s->pool.used() > (size_t)kDeepBacklogBytes;
+  const int64_t spread = std::max<int64_t>(1, ntiles / (int64_t)s->spread_workers);
+  const int32_t tpi = (int32_t)(deep ? big : std::min(big, spread));
+  const int64_t nitems = (ntiles + tpi - 1) / tpi;
+  constexpr int64_t kRingItems = (int64_t)kWorkRing / 2;  // keep half the ring as margin
+  if (bytes > s->pool.capacity() || nitems > kRingItems || nitems >= (1 << kItemBits))
+    return FFTOC_ERROR_BUDGET;
   if (s->next_seq >= s->sd.max_requests) return FFTOC_ERROR_QUEUE_FULL;
   const int slot = (int)(s->next_seq % (uint64_t)s->nslots);
   if (s->slots[slot].busy) return FFTOC_ERROR_QUEUE_FULL;  // ring full
+  if (s->inflight_items + nitems > kRingItems) return FFTOC_ERROR_QUEUE_FULL;
   size_t off = 0;
   if (!s->pool.alloc(bytes, &off)) return FFTOC_ERROR_QUEUE_FULL;  // budget in use
   const uint64_t seq = s->next_seq++;
@@ -280,16 +324,10 @@ int fftoc_server_submit(fftoc_server s, const void *in, int64_t batch, int n, ui
   sl.seq = seq;
   sl.out_off = off;
   sl.out_bytes = bytes;
+  sl.items = nitems;
+  s->inflight_items += nitems;
   s->slot_of[seq] = slot;
 
-  const int id = s->routine[lg(n)][c64 ? 1 : 0];
-  const int S = batch_routine_signals(id);
-  const int64_t ntiles = (batch + S - 1) / S;
-  // items of about 256 KiB of input, but spread over the workers
-  const int64_t tile_bytes = (int64_t)S * n * (int64_t)es;
-  const int64_t cap = std::max<int64_t>(1, 262144 / tile_bytes);
-  const int64_t spread = std::max<int64_t>(1, ntiles / (2 * (int64_t)(s->grid - 1)));
-  const int32_t tpi = (int32_t)std::min(cap, spread);
   ReqDesc &d = s->descs[slot];
   volatile uint64_t *seqp = &d.seq;
   *seqp = kSeqInvalid;
@@ -298,7 +336,7 @@ int fftoc_server_submit(fftoc_server s, const void *in, int64_t batch, int n, ui
   d.out = s->d_pool + off;
   d.batch = batch;
   d.ntiles = ntiles;
-  d.nitems = (ntiles + tpi - 1) / tpi;
+  d.nitems = nitems;
   d.n = n;
   d.routine = id;
   d.tiles_per_item = tpi;
@@ -345,6 +383,7 @@ int fftoc_server_release(fftoc_server s, uint64_t request_id) {
     return FFTOC_ERROR_UNSUPPORTED;  // still running
   Slot &sl = s->slots[slot];
   s->pool.release(sl.out_off, sl.out_bytes);
+  s->inflight_items -= sl.items;
   sl.busy = false;
   s->slot_of.erase(it);
   return FFTOC_SUCCESS;
@@ -409,6 +448,7 @@ int fftoc_server_replay(fftoc_server s, const fftoc_job *reqs, int count, const 
         latency_s[lv.index] = (now_s() - t0) - arrival_s[lv.index];
         Slot &sl = s->slots[lv.slot];
         s->pool.release(sl.out_off, sl.out_bytes);
+        s->inflight_items -= sl.items;
         sl.busy = false;
         s->slot_of.erase(lv.id);
         inflight[k] = inflight.back();

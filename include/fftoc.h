@@ -133,16 +133,17 @@ const char *fftoc_variant_name(int n, int variant);
 /* Runs one specific variant (same conventions as fftoc_fft). */
 int fftoc_fft_variant(const void *in, void *out, int64_t batch, int n,
                       uint32_t flags, int variant, void *stream);
-/* Variant used by fftoc_fft for length n, and its override. */
-int fftoc_get_variant(int n);
-int fftoc_set_variant(int n, int variant);
+/* Variant fftoc_fft uses for length n at this batch size (the choice is kept
+ * per floor(log2(batch)) bucket), and its override (batch < 0: every bucket). */
+int fftoc_get_variant(int n, int64_t batch);
+int fftoc_set_variant(int n, int64_t batch, int variant);
 /* Times every variant of length n over `batch` signals (allocates and frees
  * two device scratch buffers, synchronises `stream`) and makes the fastest
- * one of each entry-point family its default. */
+ * one of each entry-point family its default for that batch bucket. */
 int fftoc_autotune(int n, int64_t batch, int reps, void *stream);
 /* Median time in microseconds of a variant in the last autotune (<0: none). */
 double fftoc_variant_time_us(int n, int variant);
-/* Persisted tuning table (text: "<family> <n> <variant name>" per line). */
+/* Persisted tuning table (text: "<family> <n> <log2 batch> <variant>"). */
 int fftoc_tuning_save(const char *path);
 int fftoc_tuning_load(const char *path);
 
@@ -177,11 +178,15 @@ int fftoc_plan_destroy(fftoc_plan plan);
 
 /* ---- serving: device-resident request loop with HBM-budget admission -----
  * No counterpart in the reference.  fftoc_server_create launches a
- * persistent kernel (one workgroup per CU) that serves requests posted
- * through a ring in pinned host memory until fftoc_server_destroy.  A request
- * is admitted when a ring slot is free and its output fits in the free part
- * of the HBM budget (the server's output pool); requests that arrive while
- * others run start on idle workgroups at once.  While a server runs,
+ * persistent kernel that serves requests posted through a ring in pinned
+ * host memory until fftoc_server_destroy.  It seats as many workgroups as fit
+ * (two per CU): the first on each CU always takes work, the second joins only
+ * while a backlog waits.  The environment variable
+ * FFTOC_SERVER_WORKERS_PER_CU=1 keeps one workgroup per CU, which lowers the
+ * latency at light load and the throughput at saturation.  A request is
+ * admitted when a ring slot is free and its output fits in the free part of
+ * the HBM budget (the server's output pool); requests that arrive while others
+ * run start on idle workgroups at once.  While a server runs,
  * hipDeviceSynchronize() waits for the server to stop: synchronise streams
  * instead. */
 typedef struct fftoc_server_t *fftoc_server;

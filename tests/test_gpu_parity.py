@@ -136,13 +136,13 @@ def test_autotune_and_tuning_roundtrip(tmp_path):
     assert lib.fftoc_autotune(n, 4096, 3, None) == 0
     times = [lib.fftoc_variant_time_us(n, v) for v in range(lib.fftoc_variant_count(n))]
     assert all(t > 0 for t in times)
-    best = lib.fftoc_get_variant(n)
+    best = lib.fftoc_get_variant(n, 4096)
     assert times[best] == min(times)
     path = str(tmp_path / "tune.txt").encode()
     assert lib.fftoc_tuning_save(path) == 0
     other = (best + 1) % len(times)
-    assert lib.fftoc_set_variant(n, other) == 0 and lib.fftoc_get_variant(n) == other
+    assert lib.fftoc_set_variant(n, 4096, other) == 0 and lib.fftoc_get_variant(n, 4096) == other
     assert lib.fftoc_tuning_load(path) == 0
-    assert lib.fftoc_get_variant(n) == best
+    assert lib.fftoc_get_variant(n, 4096) == best
     x = signals(n, 64, seed=9)
     assert np.max(oracle.rel_l2(fo.fft_host(x), oracle.reference(x), axis=-1)) <= TOL
