@@ -23,6 +23,24 @@ struct LaunchArgs {
   bool c64;              // complex64 data (FFTOC_DATA_C64)
 };
 
+// Workgroups of KERNEL (THREADS threads each) resident on the whole device,
+// queried once per kernel: the grid of a persistent launch.
+template <auto KERNEL, int THREADS>
+int64_t resident_workgroups() {
+  static const int64_t n = [] {
+    int per_cu = 0, dev = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, KERNEL, THREADS, 0) != hipSuccess ||
+        per_cu <= 0)
+      per_cu = 1;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        cus <= 0)
+      cus = 256;
+    return (int64_t)per_cu * cus;
+  }();
+  return n;
+}
+
 // mfma_prec_kernels.hip: FP32 / split-BF16 stage GEMMs (n = 64, 256, 4096)
 int launch_tensor_precision(int n, const LaunchArgs &a);
 

@@ -114,53 +114,68 @@ __device__ __forceinline__ int64_t wave_global() {
 constexpr int kWaves = 4;  // waves per workgroup for the one-unit-per-wave kernels
 
 // ---------------------------------------------------------------------------
-// N = 8: 32 signals per wave.  A = [x_a | x_b] (16 rows, K = 16: chunks 0-1
+// Every kernel below takes G units per wave (the reference's FFTs per unit):
+// all of a wave's HBM loads are issued before its first matrix product, so G
+// multiplies the bytes each wave has in flight.
+
+// N = 8: 32 signals per unit.  A = [x_a | x_b] (16 rows, K = 16: chunks 0-1
 // hold signal i's 8 inputs, chunks 2-3 those of signal 16+i), B = diag(F8,F8).
-template <bool INV>
+template <int G, bool INV>
 __global__ __launch_bounds__(64 * kWaves) void tensor8_kernel(const c2 *in, c2 *out, int64_t batch,
                                                               const c2 *__restrict__ tw,
                                                               double scale) {
   const int lane = lane_id(), g = lane >> 4, j = lane & 15;
-  const int64_t s0 = wave_global() * 32;
-  if (s0 >= batch) return;
+  const int64_t sw = wave_global() * (32 * G);
+  if (sw >= batch) return;
   const Frag f = dft8x2_frag(tw, lane);
-  c2 a[4];
+  c2 a[G][4];
 #pragma unroll
-  for (int c = 0; c < 4; ++c) {
-    const int64_t s = s0 + (c < 2 ? 0 : 16) + j;
-    const int e = 4 * (c & 1) + g;
-    a[c] = (s < batch) ? ld_in<INV>(in + s * 8 + e) : make_double2(0.0, 0.0);
-  }
-  v4d yr, yi;
-  kgemm(a, f, yr, yi);
+  for (int u = 0; u < G; ++u)
 #pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    const int64_t s = s0 + (j < 8 ? 0 : 16) + 4 * r + g;
-    if (s < batch) st_out<INV>(out + s * 8 + (j & 7), make_double2(yr[r], yi[r]), scale);
+    for (int c = 0; c < 4; ++c) {
+      const int64_t s = sw + 32 * u + (c < 2 ? 0 : 16) + j;
+      const int e = 4 * (c & 1) + g;
+      a[u][c] = (s < batch) ? ld_in<INV>(in + s * 8 + e) : make_double2(0.0, 0.0);
+    }
+#pragma unroll
+  for (int u = 0; u < G; ++u) {
+    v4d yr, yi;
+    kgemm(a[u], f, yr, yi);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int64_t s = sw + 32 * u + (j < 8 ? 0 : 16) + 4 * r + g;
+      if (s < batch) st_out<INV>(out + s * 8 + (j & 7), make_double2(yr[r], yi[r]), scale);
+    }
   }
 }
 
 // ---------------------------------------------------------------------------
-// N = 16: 16 signals per wave, one DFT16 per row.
-template <bool INV>
+// N = 16: 16 signals per unit, one DFT16 per row.
+template <int G, bool INV>
 __global__ __launch_bounds__(64 * kWaves) void tensor16_kernel(const c2 *in, c2 *out, int64_t batch,
                                                                const c2 *__restrict__ tw,
                                                                double scale) {
   const int lane = lane_id(), g = lane >> 4, j = lane & 15;
-  const int64_t s0 = wave_global() * 16;
-  if (s0 >= batch) return;
+  const int64_t sw = wave_global() * (16 * G);
+  if (sw >= batch) return;
   const Frag f = dft16_frag(tw, lane);
-  c2 a[4];
-  const int64_t sa = s0 + j;
+  c2 a[G][4];
 #pragma unroll
-  for (int c = 0; c < 4; ++c)
-    a[c] = (sa < batch) ? ld_in<INV>(in + sa * 16 + 4 * c + g) : make_double2(0.0, 0.0);
-  v4d yr, yi;
-  kgemm(a, f, yr, yi);
+  for (int u = 0; u < G; ++u) {
+    const int64_t sa = sw + 16 * u + j;
 #pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    const int64_t s = s0 + 4 * r + g;
-    if (s < batch) st_out<INV>(out + s * 16 + j, make_double2(yr[r], yi[r]), scale);
+    for (int c = 0; c < 4; ++c)
+      a[u][c] = (sa < batch) ? ld_in<INV>(in + sa * 16 + 4 * c + g) : make_double2(0.0, 0.0);
+  }
+#pragma unroll
+  for (int u = 0; u < G; ++u) {
+    v4d yr, yi;
+    kgemm(a[u], f, yr, yi);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int64_t s = sw + 16 * u + 4 * r + g;
+      if (s < batch) st_out<INV>(out + s * 16 + j, make_double2(yr[r], yi[r]), scale);
+    }
   }
 }
 
@@ -173,27 +188,32 @@ __global__ __launch_bounds__(64 * kWaves) void tensor16_kernel(const c2 *in, c2 
 //   M = 4: signal g, n2 = r                 -> all four n2 in-lane
 //   M = 8: signal g % 2, n2 = 2r + g / 2    -> even n2 here, odd n2 in lane
 //          l ^ 32; DFT4 in-lane, then one radix-2 exchange across the pair.
-template <int M, bool INV>
+template <int M, int G, bool INV>
 __global__ __launch_bounds__(64 * kWaves) void tensor16xm_kernel(const c2 *in, c2 *out,
                                                                  int64_t batch,
                                                                  const c2 *__restrict__ tw,
                                                                  double scale) {
   constexpr int N = 16 * M;
-  constexpr int SPW = 16 / M;  // signals per wave
+  constexpr int SPW = 16 / M;  // signals per unit
   const int lane = lane_id(), g = lane >> 4, j = lane & 15;
-  const int64_t s0 = wave_global() * SPW;
-  if (s0 >= batch) return;
+  const int64_t sw = wave_global() * (SPW * G);
+  if (sw >= batch) return;
   const Frag f = dft16_frag(tw, lane);
-  c2 a[4];
-  {
-    const int64_t s = s0 + j % SPW;
+  c2 au[G][4];
+#pragma unroll
+  for (int u = 0; u < G; ++u) {
+    const int64_t s = sw + SPW * u + j % SPW;
     const int n2 = j / SPW;
 #pragma unroll
     for (int c = 0; c < 4; ++c)
-      a[c] = (s < batch) ? ld_in<INV>(in + s * N + M * (4 * c + g) + n2) : make_double2(0.0, 0.0);
+      au[u][c] =
+          (s < batch) ? ld_in<INV>(in + s * N + M * (4 * c + g) + n2) : make_double2(0.0, 0.0);
   }
+#pragma unroll
+  for (int u = 0; u < G; ++u) {
+  const int64_t s0 = sw + SPW * u;
   v4d yr, yi;
-  kgemm(a, f, yr, yi);
+  kgemm(au[u], f, yr, yi);
   c2 y[4];
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
@@ -247,6 +267,7 @@ __global__ __launch_bounds__(64 * kWaves) void tensor16xm_kernel(const c2 *in, c
       }
     }
   }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -274,31 +295,40 @@ __device__ __forceinline__ void core256(c2 (&a)[4], const Frag &f, const c2 *__r
 // element index with bits 4..7 (16 x 16-byte slots per 256 B bank window).
 __device__ __forceinline__ int swz16(int e) { return e ^ ((e >> 4) & 15); }
 
-// N = 256: one signal per wave, two MFMA stages, output re-ordered through
+// N = 256: one signal per unit, two MFMA stages, output re-ordered through
 // 4 KiB of LDS per wave so every store instruction writes 1 KiB contiguously.
-template <bool INV>
+template <int G, bool INV>
 __global__ __launch_bounds__(64 * kWaves) void tensor256_kernel(const c2 *in, c2 *out,
                                                                 int64_t batch,
                                                                 const c2 *__restrict__ tw,
                                                                 double scale) {
   __shared__ c2 lds[kWaves * 256];
   const int lane = lane_id(), g = lane >> 4, j = lane & 15;
-  const int64_t s = wave_global();
-  if (s >= batch) return;
+  const int64_t sw = wave_global() * G;
+  if (sw >= batch) return;
   c2 *buf = lds + (threadIdx.x >> 6) * 256;
   const Frag f = dft16_frag(tw, lane);
-  const c2 *x = in + s * 256;
-  c2 a[4];
+  c2 a[G][4];
 #pragma unroll
-  for (int c = 0; c < 4; ++c) a[c] = ld_in<INV>(x + 64 * c + lane);
-  c2 z[4];
-  core256(a, f, tw, lane, z);
+  for (int u = 0; u < G; ++u) {
+    const int64_t s = sw + u < batch ? sw + u : sw;  // a short tail re-reads signal sw
 #pragma unroll
-  for (int r = 0; r < 4; ++r) buf[swz16(4 * r + g + 16 * j)] = z[r];
-  wave_lds_sync();
-  c2 *y = out + s * 256;
+    for (int c = 0; c < 4; ++c) a[u][c] = ld_in<INV>(in + s * 256 + 64 * c + lane);
+  }
 #pragma unroll
-  for (int c = 0; c < 4; ++c) st_out<INV>(y + 64 * c + lane, buf[swz16(64 * c + lane)], scale);
+  for (int u = 0; u < G; ++u) {
+    c2 z[4];
+    core256(a[u], f, tw, lane, z);
+    if (u > 0) wave_lds_sync();  // the previous unit's LDS reads are done
+#pragma unroll
+    for (int r = 0; r < 4; ++r) buf[swz16(4 * r + g + 16 * j)] = z[r];
+    wave_lds_sync();
+    if (sw + u < batch) {
+      c2 *y = out + (sw + u) * 256;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) st_out<INV>(y + 64 * c + lane, buf[swz16(64 * c + lane)], scale);
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -308,27 +338,35 @@ __global__ __launch_bounds__(64 * kWaves) void tensor256_kernel(const c2 *in, c2
 // stage 2 (two groups k2a = 0, 1): DFT16 over n2' with the free transpose.
 // X[k1 + 16 k2a + 32 k2b]: lane l register r of group k2a holds
 // k1 = 4r + g, k2b = j.
-template <bool INV>
+template <int G, bool INV>
 __global__ __launch_bounds__(64 * kWaves) void tensor512_kernel(const c2 *in, c2 *out,
                                                                 int64_t batch,
                                                                 const c2 *__restrict__ tw,
                                                                 double scale) {
   __shared__ c2 lds[kWaves * 512];
   const int lane = lane_id(), g = lane >> 4, j = lane & 15;
-  const int64_t s = wave_global();
-  if (s >= batch) return;
+  const int64_t sw = wave_global() * G;
+  if (sw >= batch) return;
   c2 *buf = lds + (threadIdx.x >> 6) * 512;
   const Frag f = dft16_frag(tw, lane);
-  const c2 *x = in + s * 512;
+  c2 au[G][2][4];
+#pragma unroll
+  for (int u = 0; u < G; ++u) {
+    const int64_t s = sw + u < batch ? sw + u : sw;  // a short tail re-reads signal sw
+    // n1 = 4c + g, n2 = 16h + j
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) au[u][h][c] = ld_in<INV>(in + s * 512 + 32 * (4 * c + g) + 16 * h + j);
+  }
+#pragma unroll
+  for (int u = 0; u < G; ++u) {
+  const int64_t s = sw + u;
   c2 yh[2][4];
 #pragma unroll
   for (int h = 0; h < 2; ++h) {
-    c2 a[4];
-    // n1 = 4c + g, n2 = 16h + j
-#pragma unroll
-    for (int c = 0; c < 4; ++c) a[c] = ld_in<INV>(x + 32 * (4 * c + g) + 16 * h + j);
     v4d yr, yi;
-    kgemm(a, f, yr, yi);
+    kgemm(au[u][h], f, yr, yi);
     // reg r: n2' = 4r + g (n2 = 16h + n2'), k1 = j
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
@@ -344,6 +382,7 @@ __global__ __launch_bounds__(64 * kWaves) void tensor512_kernel(const c2 *in, c2
     b[0][r] = cadd(yh[0][r], yh[1][r]);
     b[1][r] = cmul(csub(yh[0][r], yh[1][r]), tw[n2p << 7]);  // W_32^{n2'}
   }
+  if (u > 0) wave_lds_sync();  // the previous unit's LDS reads are done
 #pragma unroll
   for (int k2a = 0; k2a < 2; ++k2a) {
     v4d zr, zi;
@@ -355,11 +394,14 @@ __global__ __launch_bounds__(64 * kWaves) void tensor512_kernel(const c2 *in, c2
     }
   }
   wave_lds_sync();
-  c2 *y = out + s * 512;
+  if (s < batch) {
+    c2 *y = out + s * 512;
 #pragma unroll
-  for (int c = 0; c < 8; ++c) {
-    const int e = 64 * c + lane;
-    st_out<INV>(y + e, buf[e ^ ((e >> 5) & 15)], scale);
+    for (int c = 0; c < 8; ++c) {
+      const int e = 64 * c + lane;
+      st_out<INV>(y + e, buf[e ^ ((e >> 5) & 15)], scale);
+    }
+  }
   }
 }
 
@@ -456,24 +498,50 @@ int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
     return hipGetLastError() == hipSuccess ? FFTOC_SUCCESS : FFTOC_ERROR_LAUNCH;               \
   }
 
-FFTOC_TENSOR_LAUNCHER(launch_t8, tensor8_kernel, 32 * kWaves, 64 * kWaves)
-FFTOC_TENSOR_LAUNCHER(launch_t16, tensor16_kernel, 16 * kWaves, 64 * kWaves)
-FFTOC_TENSOR_LAUNCHER(launch_t32, tensor16xm_kernel, 8 * kWaves, 64 * kWaves, 2)
-FFTOC_TENSOR_LAUNCHER(launch_t64, tensor16xm_kernel, 4 * kWaves, 64 * kWaves, 4)
-FFTOC_TENSOR_LAUNCHER(launch_t128, tensor16xm_kernel, 2 * kWaves, 64 * kWaves, 8)
-FFTOC_TENSOR_LAUNCHER(launch_t256, tensor256_kernel, kWaves, 64 * kWaves)
-FFTOC_TENSOR_LAUNCHER(launch_t512, tensor512_kernel, kWaves, 64 * kWaves)
+FFTOC_TENSOR_LAUNCHER(launch_t8, tensor8_kernel, 32 * kWaves, 64 * kWaves, 1)
+FFTOC_TENSOR_LAUNCHER(launch_t8g2, tensor8_kernel, 64 * kWaves, 64 * kWaves, 2)
+FFTOC_TENSOR_LAUNCHER(launch_t8g4, tensor8_kernel, 128 * kWaves, 64 * kWaves, 4)
+FFTOC_TENSOR_LAUNCHER(launch_t16, tensor16_kernel, 16 * kWaves, 64 * kWaves, 1)
+FFTOC_TENSOR_LAUNCHER(launch_t16g2, tensor16_kernel, 32 * kWaves, 64 * kWaves, 2)
+FFTOC_TENSOR_LAUNCHER(launch_t16g4, tensor16_kernel, 64 * kWaves, 64 * kWaves, 4)
+FFTOC_TENSOR_LAUNCHER(launch_t32, tensor16xm_kernel, 8 * kWaves, 64 * kWaves, 2, 1)
+FFTOC_TENSOR_LAUNCHER(launch_t32g2, tensor16xm_kernel, 16 * kWaves, 64 * kWaves, 2, 2)
+FFTOC_TENSOR_LAUNCHER(launch_t32g4, tensor16xm_kernel, 32 * kWaves, 64 * kWaves, 2, 4)
+FFTOC_TENSOR_LAUNCHER(launch_t64, tensor16xm_kernel, 4 * kWaves, 64 * kWaves, 4, 1)
+FFTOC_TENSOR_LAUNCHER(launch_t64g2, tensor16xm_kernel, 8 * kWaves, 64 * kWaves, 4, 2)
+FFTOC_TENSOR_LAUNCHER(launch_t64g4, tensor16xm_kernel, 16 * kWaves, 64 * kWaves, 4, 4)
+FFTOC_TENSOR_LAUNCHER(launch_t128, tensor16xm_kernel, 2 * kWaves, 64 * kWaves, 8, 1)
+FFTOC_TENSOR_LAUNCHER(launch_t128g2, tensor16xm_kernel, 4 * kWaves, 64 * kWaves, 8, 2)
+FFTOC_TENSOR_LAUNCHER(launch_t128g4, tensor16xm_kernel, 8 * kWaves, 64 * kWaves, 8, 4)
+FFTOC_TENSOR_LAUNCHER(launch_t256, tensor256_kernel, kWaves, 64 * kWaves, 1)
+FFTOC_TENSOR_LAUNCHER(launch_t256g2, tensor256_kernel, 2 * kWaves, 64 * kWaves, 2)
+FFTOC_TENSOR_LAUNCHER(launch_t256g4, tensor256_kernel, 4 * kWaves, 64 * kWaves, 4)
+FFTOC_TENSOR_LAUNCHER(launch_t512, tensor512_kernel, kWaves, 64 * kWaves, 1)
+FFTOC_TENSOR_LAUNCHER(launch_t512g2, tensor512_kernel, 2 * kWaves, 64 * kWaves, 2)
 FFTOC_TENSOR_LAUNCHER(launch_t4096, tensor4096_kernel, 1, 64 * k4096Waves)
 #undef FFTOC_TENSOR_LAUNCHER
 
 constexpr unsigned TF = FAM_TENSOR | FAM_ANY;
-const Variant kT8[] = {{"tensor:diag(F8,F8)/32 per wave", TF, &launch_t8}};
-const Variant kT16[] = {{"tensor:16/16 per wave", TF, &launch_t16}};
-const Variant kT32[] = {{"tensor:16x2/8 per wave", TF, &launch_t32}};
-const Variant kT64[] = {{"tensor:16x4/4 per wave", TF, &launch_t64}};
-const Variant kT128[] = {{"tensor:16x8/2 per wave", TF, &launch_t128}};
-const Variant kT256[] = {{"tensor:16x16/1 per wave", TF, &launch_t256}};
-const Variant kT512[] = {{"tensor:16x2x16/1 per wave", TF, &launch_t512}};
+const Variant kT8[] = {{"tensor:diag(F8,F8)/32 per wave", TF, &launch_t8},
+                       {"tensor:diag(F8,F8)/64 per wave", TF, &launch_t8g2},
+                       {"tensor:diag(F8,F8)/128 per wave", TF, &launch_t8g4}};
+const Variant kT16[] = {{"tensor:16/16 per wave", TF, &launch_t16},
+                        {"tensor:16/32 per wave", TF, &launch_t16g2},
+                        {"tensor:16/64 per wave", TF, &launch_t16g4}};
+const Variant kT32[] = {{"tensor:16x2/8 per wave", TF, &launch_t32},
+                        {"tensor:16x2/16 per wave", TF, &launch_t32g2},
+                        {"tensor:16x2/32 per wave", TF, &launch_t32g4}};
+const Variant kT64[] = {{"tensor:16x4/8 per wave", TF, &launch_t64g2},
+                        {"tensor:16x4/4 per wave", TF, &launch_t64},
+                        {"tensor:16x4/16 per wave", TF, &launch_t64g4}};
+const Variant kT128[] = {{"tensor:16x8/4 per wave", TF, &launch_t128g2},
+                         {"tensor:16x8/2 per wave", TF, &launch_t128},
+                         {"tensor:16x8/8 per wave", TF, &launch_t128g4}};
+const Variant kT256[] = {{"tensor:16x16/4 per wave", TF, &launch_t256g4},
+                         {"tensor:16x16/1 per wave", TF, &launch_t256},
+                         {"tensor:16x16/2 per wave", TF, &launch_t256g2}};
+const Variant kT512[] = {{"tensor:16x2x16/1 per wave", TF, &launch_t512},
+                         {"tensor:16x2x16/2 per wave", TF, &launch_t512g2}};
 const Variant kT4096[] = {{"tensor:16x16x16/8 waves", TF, &launch_t4096}};
 
 template <size_t K>

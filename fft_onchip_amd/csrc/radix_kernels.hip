@@ -49,33 +49,6 @@ void stockham_persistent_kernel(
   StockhamKernel<SP, S, INV, false>::run_persistent(in, out, batch, tw, scale);
 }
 
-// Resident workgroups per CU of a kernel, queried once per instantiation.
-template <class K>
-int blocks_per_cu(K kernel, int threads) {
-  static int cached = 0;
-  if (cached == 0) {
-    int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, threads, 0) != hipSuccess ||
-        nb <= 0)
-      nb = 1;
-    cached = nb;
-  }
-  return cached;
-}
-
-int device_cus() {
-  static int cus = 0;
-  if (cus == 0) {
-    int dev = 0, n = 0;
-    if (hipGetDevice(&dev) == hipSuccess &&
-        hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && n > 0)
-      cus = n;
-    else
-      cus = 256;
-  }
-  return cus;
-}
-
 enum class Form { OneTile, Staged, Persistent };
 
 template <class SP, int S, Form F, class CT = c2>
@@ -89,8 +62,7 @@ int launch_split(const LaunchArgs &a) {
   const real_t<CT> scale = (real_t<CT>)a.scale;
   if constexpr (F == Form::Persistent) {
     static_assert(sizeof(CT) == 16, "persistent form is FP64 only");
-    const int per_cu = blocks_per_cu(stockham_persistent_kernel<SP, S, false>, threads);
-    const int64_t resident = (int64_t)per_cu * device_cus();
+    const int64_t resident = resident_workgroups<stockham_persistent_kernel<SP, S, false>, threads>();
     const unsigned grid = (unsigned)(ntiles < resident ? ntiles : resident);
     if (a.inverse)
       hipLaunchKernelGGL((stockham_persistent_kernel<SP, S, true>), dim3(grid), dim3(threads), 0,
