@@ -257,6 +257,87 @@ def run_served(fo, seconds: float = 1.0, rate: float = 20000.0, saturated: int =
     return out
 
 
+def run_rocfft(wl: Workload, steps: int, warmup: int, host_x: np.ndarray):
+    """The vendor library on the same workload (the role cuFFTDx plays in the
+    reference's tables): rocFFT's C API through ctypes, out-of-place double
+    complex forward, same buffer rotation, clock ramp and timing as run_gpu.
+    Returns (result, sample of the output for the accuracy check)."""
+    import ctypes
+
+    from fft_onchip_amd import hip
+    try:
+        rf = ctypes.CDLL("librocfft.so.0")
+    except OSError as e:
+        return {"error": f"librocfft not loadable: {e}"}, None
+    vp, sz = ctypes.c_void_p, ctypes.c_size_t
+    rf.rocfft_setup.argtypes = []
+    rf.rocfft_plan_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                      sz, ctypes.POINTER(sz), sz, vp]
+    rf.rocfft_plan_get_work_buffer_size.argtypes = [vp, ctypes.POINTER(sz)]
+    rf.rocfft_execution_info_create.argtypes = [ctypes.POINTER(vp)]
+    rf.rocfft_execution_info_set_work_buffer.argtypes = [vp, vp, sz]
+    rf.rocfft_execution_info_set_stream.argtypes = [vp, vp]
+    rf.rocfft_execute.argtypes = [vp, ctypes.POINTER(vp), ctypes.POINTER(vp), vp]
+    rf.rocfft_plan_destroy.argtypes = [vp]
+    rf.rocfft_execution_info_destroy.argtypes = [vp]
+    if rf.rocfft_setup() != 0:
+        return {"error": "rocfft_setup failed"}, None
+    n, batch = wl.n, wl.batch
+    plan, info = vp(), vp()
+    lengths = (sz * 1)(n)
+    # placement 1 = not in place, transform 0 = complex forward, precision 1 = double
+    if rf.rocfft_plan_create(ctypes.byref(plan), 1, 0, 1, 1, lengths, batch, None) != 0:
+        return {"error": "rocfft_plan_create failed"}, None
+    wbytes = sz(0)
+    rf.rocfft_plan_get_work_buffer_size(plan, ctypes.byref(wbytes))
+    rf.rocfft_execution_info_create(ctypes.byref(info))
+    work = hip.DeviceBuffer(wbytes.value) if wbytes.value else None
+    if work is not None:
+        rf.rocfft_execution_info_set_work_buffer(info, vp(work.ptr), wbytes)
+    stream = hip.Stream()
+    rf.rocfft_execution_info_set_stream(info, vp(stream.handle))
+    nbytes = n * batch * 16
+    pairs = max(1, min(8, math.ceil(4 * MALL_BYTES / (2 * nbytes))))
+    ins = [hip.DeviceBuffer.from_array(host_x) for _ in range(pairs)]
+    outs = [hip.DeviceBuffer(nbytes) for _ in range(pairs)]
+    ptrs = [((vp * 1)(ins[i].ptr), (vp * 1)(outs[i].ptr)) for i in range(pairs)]
+
+    def execute(i):
+        ib, ob = ptrs[i % pairs]
+        if rf.rocfft_execute(plan, ib, ob, info) != 0:
+            raise RuntimeError("rocfft_execute failed")
+
+    t_ramp, i = time.perf_counter(), 0
+    while (time.perf_counter() - t_ramp) * 1e3 < RAMP_MS:
+        for _ in range(8):
+            execute(i)
+            i += 1
+        stream.synchronize()
+    for i in range(warmup):
+        execute(i)
+    stream.synchronize()
+    t0, t1 = hip.Event(), hip.Event()
+    t0.record(stream)
+    for i in range(steps):
+        execute(i)
+    t1.record(stream)
+    stream.synchronize()
+    ms = t0.elapsed_ms(t1) / steps
+    # accuracy sample from a fresh copy of the input (rocFFT may overwrite it)
+    ins[0].upload(host_x)
+    execute(0)
+    stream.synchronize()
+    sample = outs[0].download((min(batch, 256), n))
+    rf.rocfft_execution_info_destroy(info)
+    rf.rocfft_plan_destroy(plan)
+    for b in ins + outs + ([work] if work is not None else []):
+        b.free()
+    gbps = 32.0 * n * batch / (ms * 1e-3) / 1e9
+    return {"workload": wl.name, "n": n, "batch": batch, "path": "rocfft",
+            "ffts_per_s": batch / (ms * 1e-3), "ms_per_step": round(ms, 6), "GBps": round(gbps, 1),
+            "hbm_frac": round(gbps / HBM_PEAK_GBPS, 4), "work_buffer_bytes": wbytes.value}, sample
+
+
 def copy_bandwidth(nbytes: int = 1 << 30, reps: int = 10) -> float:
     """Device-to-device hipMemcpy of `nbytes` (read + write) in GB/s."""
     import ctypes
@@ -392,6 +473,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--quick", action="store_true", help="headline workload only")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--only", metavar="PATH:N[:PREC]",
+                    help="time one workload (path best|tensor|legacy8|legacy16, length N at its "
+                         "benchmark batch, precision fp64|fp32|bf16x3) and print its result; "
+                         "for counter runs")
     ap.add_argument("--traffic", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="write profiles/traffic.json from two rocprofv3 --pmc runs and exit")
     args = ap.parse_args()
@@ -409,6 +494,17 @@ def main():
         print("bench.py: no HIP device visible", file=sys.stderr)
         sys.exit(1)
     fo.init()
+
+    if args.only:
+        parts = args.only.split(":")
+        path, n = parts[0], int(parts[1])
+        prec = parts[2] if len(parts) > 2 else "fp64"
+        flag = {"fp64": 0, "fp32": fo.PREC_FP32, "bf16x3": fo.PREC_BF16X3}[prec]
+        batch = {4096: 16384, 64: 65536}.get(n, (1 << 24) // n)
+        wl = Workload(f"only_{path}_n{n}_{prec}", n, batch, 7, "normal", path, flag)
+        r, _, _ = run_gpu(fo, wl, args.steps, args.warmup)
+        print(json.dumps({k: (round(v, 6) if isinstance(v, float) else v) for k, v in r.items()}))
+        return
 
     head = Workload(HEADLINE, 4096, 16384, 7, "normal", "best")
     res, x, sample = run_gpu(fo, head, args.steps, args.warmup)
@@ -442,6 +538,20 @@ def main():
             extra[wl.name] = {k: (round(v, 6) if isinstance(v, float) else v) for k, v in r.items()}
             print(f"[bench] {wl.name}: {r['ffts_per_s']:.4g} FFT/s, {r['GBps']:.0f} GB/s", file=sys.stderr, flush=True)
     extra["hip_d2d_copy_GBps"] = round(copy_bandwidth(), 1)
+    rocfft_samples = {}
+    if not args.quick:
+        extra["rocfft_x4"] = {}
+        for wl in (Workload("fp64_n4096_b16384", 4096, 16384, 7, "normal", "rocfft"),
+                   Workload("fp64_n64_b65536", 64, 65536, 0, "uniform", "rocfft")):
+            hx = x if wl.n == head.n else gen_signals(wl.n, wl.batch, wl.seed, wl.kind)
+            r, smp = run_rocfft(wl, args.steps, args.warmup, hx)
+            ours = extra.get(f"fp64_n{wl.n}_b{wl.batch}_best") if wl.n != head.n else res
+            if ours and "ffts_per_s" in r:
+                r["fftoc_best_over_rocfft"] = round(ours["ffts_per_s"] / r["ffts_per_s"], 3)
+            extra["rocfft_x4"][wl.name] = r
+            if smp is not None:
+                rocfft_samples[wl.name] = (hx[:smp.shape[0]], smp)
+            print(f"[bench] rocfft {wl.name}: {r}", file=sys.stderr, flush=True)
     prec_samples = {}
     if not args.quick:
         extra["precision_c4"], prec_samples = run_precision(fo, args.steps, args.warmup)
@@ -456,6 +566,11 @@ def main():
         cpu = cpu_baseline(x, sample, threads)
         if prec_samples:
             extra["precision_c4_vs_oracle"] = precision_vs_oracle(prec_samples)
+        for name, (xin, got) in rocfft_samples.items():
+            import oracle  # the CPU leg checks the vendor library's output too
+            ref = oracle.reference(xin)
+            extra["rocfft_x4"][name]["max_rel_l2_vs_oracle"] = float(
+                np.max(oracle.rel_l2(got, ref, axis=-1)))
         print(f"[bench] cpu baseline: {cpu['value']:.4g} FFT/s on {threads} threads", file=sys.stderr, flush=True)
 
     traffic = load_traffic(head.name)

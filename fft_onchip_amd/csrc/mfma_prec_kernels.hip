@@ -59,13 +59,17 @@ struct Stage<kFP32> {
   struct Frag {
     float r[4], i[4], s[4];
   };
-  // B[k][col] = W_16^{k col}, K-chunk c, slot g -> k = 4g + c
+  // B[k][col] = W_16^{n col}, K-chunk c, slot g -> k = 4g + c; the input
+  // sample is n = k, or n = 4c + g when SAMPLES_4CG (a single-stage transform
+  // whose loads then cover 16 consecutive samples per signal)
+  template <bool SAMPLES_4CG = false>
   __device__ static Frag frag(const c2 *__restrict__ tw, int lane) {
     const int g = lane >> 4, j = lane & 15;
     Frag f;
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
-      const c2 w = tw[(((4 * g + c) * j) & 15) << 8];
+      const int n = SAMPLES_4CG ? 4 * c + g : 4 * g + c;
+      const c2 w = tw[((n * j) & 15) << 8];
       f.r[c] = (float)w.x;
       f.i[c] = (float)w.y;
       f.s[c] = (float)(w.x + w.y);
@@ -92,12 +96,15 @@ struct Stage<kBF16X3> {
   struct Frag {
     v4s r[3], i[3], s[3];  // [term]: B elements t = 0..3 <-> k = 4g + t
   };
+  // input sample n = k = 4g + t, or n = 4t + g when SAMPLES_4CG
+  template <bool SAMPLES_4CG = false>
   __device__ static Frag frag(const c2 *__restrict__ tw, int lane) {
     const int g = lane >> 4, j = lane & 15;
     Frag f;
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
-      const c2 w = tw[(((4 * g + t) * j) & 15) << 8];
+      const int n = SAMPLES_4CG ? 4 * t + g : 4 * g + t;
+      const c2 w = tw[((n * j) & 15) << 8];
       short a0, a1, a2;
       split3((float)w.x, a0, a1, a2);
       f.r[0][t] = a0; f.r[1][t] = a1; f.r[2][t] = a2;
@@ -159,10 +166,16 @@ __device__ __forceinline__ int64_t wave_global() {
 }
 
 constexpr int kWaves = 4;
+// Units per wave (all of a wave's HBM loads are issued before its first
+// product, as in the FP64 tensor kernels).
+constexpr int kG64 = 2;
+constexpr int kG256 = 4;
 
-// N = 64 = 16 x 4, four signals per wave; row i <-> (signal i / 4, n2 = i % 4),
-// K = n1 = 4g + c.  D register r of lane l: signal g, n2 = r, k1 = j; twiddle
-// W_64^{n2 k1} and DFT4 over r in FP32, output X_g[j + 16 k2].
+// N = 64 = 16 x 4, four signals per unit; row i <-> (signal i / 4, n2 = i % 4),
+// chunk / element c of lane group g holds n1 = 4c + g (16 consecutive samples
+// of a signal per load instruction).  D register r of lane l: signal g,
+// n2 = r, k1 = j; twiddle W_64^{n2 k1} and DFT4 over r in FP32, output
+// X_g[j + 16 k2].
 template <int P, bool INV>
 __global__ __launch_bounds__(64 * kWaves) void tensor64p_kernel(const c2 *in, c2 *out,
                                                                 int64_t batch,
@@ -170,19 +183,24 @@ __global__ __launch_bounds__(64 * kWaves) void tensor64p_kernel(const c2 *in, c2
                                                                 const f2 *__restrict__ tw32,
                                                                 double scale) {
   const int lane = lane_id(), g = lane >> 4, j = lane & 15;
-  const int64_t s0 = wave_global() * 4;
-  if (s0 >= batch) return;
-  const auto f = Stage<P>::frag(tw, lane);
-  f2 a[4];
-  {
-    const int64_t s = s0 + (j >> 2);
+  const int64_t sw = wave_global() * (4 * kG64);
+  if (sw >= batch) return;
+  const auto f = Stage<P>::template frag<true>(tw, lane);
+  f2 au[kG64][4];
+#pragma unroll
+  for (int u = 0; u < kG64; ++u) {
+    const int64_t s = sw + 4 * u + (j >> 2);
     const int n2 = j & 3;
 #pragma unroll
     for (int c = 0; c < 4; ++c)
-      a[c] = (s < batch) ? ld_in<INV>(in + s * 64 + 4 * (4 * g + c) + n2) : make_float2(0.f, 0.f);
+      au[u][c] =
+          (s < batch) ? ld_in<INV>(in + s * 64 + 4 * (4 * c + g) + n2) : make_float2(0.f, 0.f);
   }
+#pragma unroll
+  for (int u = 0; u < kG64; ++u) {
+  const int64_t s0 = sw + 4 * u;
   v4f yr, yi;
-  Stage<P>::gemm(a, f, yr, yi);
+  Stage<P>::gemm(au[u], f, yr, yi);
   f2 y[4];
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
@@ -195,6 +213,7 @@ __global__ __launch_bounds__(64 * kWaves) void tensor64p_kernel(const c2 *in, c2
   if (s < batch) {
 #pragma unroll
     for (int k2 = 0; k2 < 4; ++k2) st_out<INV>(out + s * 64 + j + 16 * k2, y[k2], scale);
+  }
   }
 }
 
@@ -230,22 +249,31 @@ __global__ __launch_bounds__(64 * kWaves) void tensor256p_kernel(const c2 *in, c
                                                                  double scale) {
   __shared__ f2 lds[kWaves * 256];
   const int lane = lane_id(), g = lane >> 4, j = lane & 15;
-  const int64_t s = wave_global();
-  if (s >= batch) return;
+  const int64_t sw = wave_global() * kG256;
+  if (sw >= batch) return;
   f2 *buf = lds + (threadIdx.x >> 6) * 256;
   const auto f = Stage<P>::frag(tw, lane);
-  const c2 *x = in + s * 256;
-  f2 a[4];
+  f2 au[kG256][4];
 #pragma unroll
-  for (int c = 0; c < 4; ++c) a[c] = ld_in<INV>(x + 64 * g + 16 * c + j);
-  f2 z[4];
-  core256p<P>(a, f, tw32, lane, z);
+  for (int u = 0; u < kG256; ++u) {
+    const int64_t s = sw + u < batch ? sw + u : sw;  // a short tail re-reads signal sw
 #pragma unroll
-  for (int r = 0; r < 4; ++r) buf[swz16(4 * g + r + 16 * j)] = z[r];
-  wave_lds_sync();
-  c2 *y = out + s * 256;
+    for (int c = 0; c < 4; ++c) au[u][c] = ld_in<INV>(in + s * 256 + 64 * g + 16 * c + j);
+  }
 #pragma unroll
-  for (int c = 0; c < 4; ++c) st_out<INV>(y + 64 * c + lane, buf[swz16(64 * c + lane)], scale);
+  for (int u = 0; u < kG256; ++u) {
+    f2 z[4];
+    core256p<P>(au[u], f, tw32, lane, z);
+    if (u > 0) wave_lds_sync();  // the previous unit's LDS reads are done
+#pragma unroll
+    for (int r = 0; r < 4; ++r) buf[swz16(4 * g + r + 16 * j)] = z[r];
+    wave_lds_sync();
+    if (sw + u < batch) {
+      c2 *y = out + (sw + u) * 256;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) st_out<INV>(y + 64 * c + lane, buf[swz16(64 * c + lane)], scale);
+    }
+  }
 }
 
 // N = 4096 = 16 x 256, one signal per 512-thread workgroup (see the FP64
@@ -321,8 +349,8 @@ int launch_p(int n, const LaunchArgs &a) {
   int64_t grid;
   int block;
   switch (n) {
-    case 64: grid = ceil_div(a.batch, 4 * kWaves); block = 64 * kWaves; break;
-    case 256: grid = ceil_div(a.batch, kWaves); block = 64 * kWaves; break;
+    case 64: grid = ceil_div(a.batch, 4 * kG64 * kWaves); block = 64 * kWaves; break;
+    case 256: grid = ceil_div(a.batch, kG256 * kWaves); block = 64 * kWaves; break;
     case 4096: grid = a.batch; block = 64 * k4096Waves; break;
     default: return FFTOC_ERROR_UNSUPPORTED;
   }
