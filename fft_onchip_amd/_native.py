@@ -23,6 +23,7 @@ SIGNATURES = {
     "fftoc_shutdown": ([], _int),
     "fftoc_strerror": ([_int], ctypes.c_char_p),
     "fftoc_version": ([], _int),
+    "fftoc_twiddle_table": ([ctypes.POINTER(_vp), ctypes.POINTER(_vp)], _int),
     "fftoc_tensor_fft": ([_vp, _vp, _i64, _int, _u32, _vp], _int),
     "fftoc_legacy8_fft": ([_vp, _vp, _i64, _int, _u32, _vp], _int),
     "fftoc_legacy16_fft": ([_vp, _vp, _i64, _int, _u32, _vp], _int),

@@ -342,9 +342,6 @@ int device_tables(const void **tw64, const void **tw32) {
   return FFTOC_SUCCESS;
 }
 
-namespace {
-
-}  // namespace
 }  // namespace fftoc
 
 using fftoc::Family;
@@ -354,6 +351,16 @@ extern "C" {
 int fftoc_init(void) {
   int dev = 0;
   return fftoc::init_current_device(&dev);
+}
+
+int fftoc_twiddle_table(const void **fp64, const void **fp32) {
+  if (!fp64 && !fp32) return FFTOC_ERROR_NULL_POINTER;
+  const void *a = nullptr, *b = nullptr;
+  const int rc = fftoc::device_tables(&a, &b);
+  if (rc != FFTOC_SUCCESS) return rc;
+  if (fp64) *fp64 = a;
+  if (fp32) *fp32 = b;
+  return FFTOC_SUCCESS;
 }
 
 int fftoc_shutdown(void) {

@@ -37,8 +37,10 @@ __device__ __forceinline__ int lds_index(int E) { return E + (E >> 4); }
 constexpr int lds_elems(int n) { return n + (n >> 4); }
 
 // INVM: 0 = forward, 1 = inverse, 2 = direction chosen at run time by the
-// `inv` argument (used by the mixed-size kernels).
-template <class SP, int S, int INVM, bool STAGED, class CT = c2>
+// `inv` argument (used by the mixed-size kernels).  PAD: LDS element E lives
+// at lds_index(E) (the library's own buffers) or at E (a caller's buffer in
+// natural order, see run_lds).
+template <class SP, int S, int INVM, bool STAGED, class CT = c2, bool PAD = true>
 struct StockhamKernel {
   using R_ = real_t<CT>;
   static constexpr int N = SP::N;
@@ -46,6 +48,11 @@ struct StockhamKernel {
   static constexpr int T = N / RM;
   static constexpr int THREADS = T * S;
   static constexpr bool WAVE_LOCAL = (T <= 64) && (64 % T == 0);
+
+  __device__ static int at(int E) {
+    if constexpr (PAD) return lds_index(E);
+    else return E;
+  }
 
   __device__ static bool swap_io(bool inv) {
     if constexpr (INVM == 0) return false;
@@ -91,7 +98,7 @@ struct StockhamKernel {
       for (int i = 0; i < NB; ++i) {
         const int j = t + i * T;
 #pragma unroll
-        for (int m = 0; m < R; ++m) reg[i * R + m] = lds[lds_index(sbase + j + m * (N / R))];
+        for (int m = 0; m < R; ++m) reg[i * R + m] = lds[at(sbase + j + m * (N / R))];
       }
       // all lanes have read this pass's inputs before anyone overwrites them
       sync();
@@ -123,7 +130,7 @@ struct StockhamKernel {
           if (swap_io(inv)) v = cswap(v);
           if (valid) st_stream(gout + d, v);
         } else {
-          lds[lds_index(sbase + d)] = v;
+          lds[at(sbase + d)] = v;
         }
       }
     }
@@ -158,7 +165,7 @@ struct StockhamKernel {
         const int E = tid + c * THREADS;
         CT v = (E < nvalid * N) ? ld_stream(src + E) : mk<CT>(0, 0);
         if (swap_io(inv)) v = cswap(v);
-        lds[lds_index(E)] = v;
+        lds[at(E)] = v;
       }
       __syncthreads();
       pass<0, false>(reg, lds, nullptr, t, sl * N, valid, tw, scale, inv);
@@ -176,11 +183,46 @@ struct StockhamKernel {
 #pragma unroll
       for (int c = 0; c < (S * N) / THREADS; ++c) {
         const int E = tid + c * THREADS;
-        CT v = lds[lds_index(E)];
+        CT v = lds[at(E)];
         if (scale != 1.0) v = cscale(v, scale);
         if (swap_io(inv)) v = cswap(v);
         if (E < nvalid * N) st_stream(dst + E, v);
       }
+    }
+  }
+
+  // In-LDS transform of the S signals of N elements at `lds` (index at(E)
+  // for element E = s*N + e), in place, by a workgroup of exactly THREADS
+  // threads, all of which call it.  Direction and scale are run-time
+  // arguments; `tw` is the library's W_4096 table (fftoc_twiddle_table).
+  __device__ static void run_lds(CT *lds, const CT *__restrict__ tw, R_ scale, bool inv) {
+    static_assert(STAGED, "the in-LDS form ends every pass in LDS");
+    const int tid = threadIdx.x;
+    const int t = tid % T;
+    const int sl = tid / T;
+    if (inv) {  // inverse = forward on swapped (re, im), swapped back
+#pragma unroll
+      for (int c = 0; c < (S * N) / THREADS; ++c) {
+        const int E = tid + c * THREADS;
+        lds[at(E)] = cswap(lds[at(E)]);
+      }
+      __syncthreads();
+    }
+    CT reg[RM];
+    pass<0, false>(reg, lds, nullptr, t, sl * N, true, tw, scale, false);
+    later_passes(std::make_integer_sequence<int, SP::P - 1>{}, reg, lds, nullptr, t, sl * N, true,
+                 tw, scale, false);
+    __syncthreads();  // the fix-up below crosses signals and waves
+    if (inv || scale != (R_)1) {
+#pragma unroll
+      for (int c = 0; c < (S * N) / THREADS; ++c) {
+        const int E = tid + c * THREADS;
+        CT v = lds[at(E)];
+        if (scale != (R_)1) v = cscale(v, scale);
+        if (inv) v = cswap(v);
+        lds[at(E)] = v;
+      }
+      __syncthreads();
     }
   }
 

@@ -88,6 +88,12 @@ int fftoc_shutdown(void);
 const char *fftoc_strerror(int status);
 /* Library version as MAJOR*10000 + MINOR*100 + PATCH. */
 int fftoc_version(void);
+/* Device addresses of the current device's twiddle tables W_4096^j,
+ * j = 0..4095 (double2 and float2), for kernels that call the in-LDS
+ * functors of include/fftoc_lds.hpp.  Either pointer may be NULL.  Replaces
+ * the per-functor twiddle setup of the reference (pow_theta,
+ * includes/tensor_fft_64.cuh:33-39; the table of includes/constants.cuh:7-110). */
+int fftoc_twiddle_table(const void **fp64, const void **fp32);
 
 /* ---- tensor path: FP64 MFMA Karatsuba DFT stages ------------------------ */
 /* Replaces fft::tensor_fft_N<CT,N,UPB,FPU> (includes/tensor_fft_8.cuh:12-66,

@@ -77,3 +77,18 @@ def test_python_mirror_raises():
     assert e.value.name == "INVALID_SIZE"
     for n in fo.TENSOR_SIZES:
         assert callable(getattr(fo, f"tensor_fft_{n}"))
+
+
+def test_fused_example_library_loads_and_checks_arguments():
+    """The in-LDS functor example library (include/fftoc_lds.hpp) is built and
+    rejects bad arguments before any launch."""
+    path = os.path.join(os.path.dirname(_native.LIB_PATH), "libfftoc_fused_example.so")
+    f = ctypes.CDLL(path).fftoc_example_windowed_fft
+    f.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_void_p,
+                  ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_void_p]
+    f.restype = ctypes.c_int
+    assert f(None, None, 0, 64, None, 0, 1.0, 0, None) == 0      # empty batch
+    assert f(None, None, -1, 64, None, 0, 1.0, 0, None) == 2     # FFTOC_ERROR_INVALID_BATCH
+    assert f(None, None, 4, 64, None, 0, 1.0, 0, None) == 3      # FFTOC_ERROR_NULL_POINTER
+    assert f(16, 16, 4, 64, 32, 0, 1.0, 0, None) == 5            # FFTOC_ERROR_OVERLAP
+    assert f(16, 4096, 4, 48, 32, 0, 1.0, 0, None) == 1          # FFTOC_ERROR_INVALID_SIZE
