@@ -63,11 +63,20 @@ constexpr int kNumRoutines = sizeof(kRoutines) / sizeof(kRoutines[0]);
 
 // Each tile routine is a separate (not inlined) function so that its
 // registers are allocated on their own; the kernel needs the maximum over the
-// routines, not their union.
+// routines, not their union.  Its pointer parameters carry their address
+// spaces (global, LDS): through generic pointers the routine would access HBM
+// with flat instructions, whose LDS counter makes every LDS wait also wait for
+// outstanding HBM traffic.
+template <class T>
+using global_ptr = __attribute__((address_space(1))) T *;
+template <class T>
+using lds_ptr = __attribute__((address_space(3))) T *;
+
 template <class K, class CT>
-__device__ __noinline__ void tile_call(const CT *in, CT *out, int64_t batch, int64_t tile, CT *lds,
-                                       const CT *__restrict__ tw, real_t<CT> scale, bool inv) {
-  K::run_tile(in, out, batch, tile, lds, tw, scale, inv);
+__device__ __noinline__ void tile_call(global_ptr<const CT> in, global_ptr<CT> out, int64_t batch,
+                                       int64_t tile, lds_ptr<CT> lds, global_ptr<const CT> tw,
+                                       real_t<CT> scale, bool inv) {
+  K::run_tile((const CT *)in, (CT *)out, batch, tile, (CT *)lds, (const CT *)tw, scale, inv);
 }
 
 template <class CT>
@@ -80,9 +89,9 @@ __device__ __forceinline__ void run_routine(int id, const JobDev &J, int64_t til
   switch (id) {
 #define FFTOC_CASE(ID, NN, NAME, SPLIT, S, STAGED)                                          \
   case ID:                                                                                \
-    tile_call<StockhamKernel<FFTOC_UNPAREN SPLIT, S, 2, STAGED, CT>, CT>(in, out, J.batch, \
-                                                                         tile, lds, tw,    \
-                                                                         scale, inv);      \
+    tile_call<StockhamKernel<FFTOC_UNPAREN SPLIT, S, 2, STAGED, CT>, CT>(                  \
+        (global_ptr<const CT>)in, (global_ptr<CT>)out, J.batch, tile, (lds_ptr<CT>)lds,     \
+        (global_ptr<const CT>)tw, scale, inv);                                               \
     break;
     FFTOC_ROUTINES(FFTOC_CASE)
 #undef FFTOC_CASE
