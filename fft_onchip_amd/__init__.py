@@ -242,6 +242,11 @@ class Server:
         if rc != 0:
             raise FftocError(rc, "fftoc_server_release")
 
+    def set_idle_timeout(self, seconds: float) -> None:
+        rc = lib().fftoc_server_set_idle_timeout(self.handle, float(seconds))
+        if rc != 0:
+            raise FftocError(rc, "fftoc_server_set_idle_timeout")
+
     def stats(self):
         used, inflight = ctypes.c_size_t(0), ctypes.c_int(0)
         lib().fftoc_server_stats(self.handle, ctypes.byref(used), ctypes.byref(inflight))

@@ -63,6 +63,7 @@ SIGNATURES.update({
                              ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)], _int),
     "fftoc_server_stats": ([_vp, ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(_int)], _int),
     "fftoc_server_destroy": ([_vp], _int),
+    "fftoc_server_set_idle_timeout": ([_vp, ctypes.c_double], _int),
 })
 TENSOR_SIZES = (8, 16, 64, 128, 256, 512, 4096)
 for _n in TENSOR_SIZES:

@@ -62,19 +62,18 @@ struct ServerDev {
   unsigned long long *wnext;     // next work ticket
   unsigned *dev_stop;      // the dispatcher has left: workers leave when idle
   Completion *comps;       // [nslots]
-  unsigned *claim;         // [max_requests] items handed out per request
-  unsigned *done;          // [max_requests] items finished per request
-  unsigned long long *head;  // lowest request that may still have items
+  unsigned *done;          // [nslots] items finished of the slot's request; the
+                           // worker finishing the last one resets it
   int nslots;
   unsigned *cu_seats;      // [kCuKeys] workers seated per CU in this launch
   int claim_backlog;       // unclaimed items above which secondary workers join
-  uint64_t max_requests;
   uint64_t idle_timeout_ticks;  // wall_clock64 ticks without work before exit
   const void *tw64;
   const void *tw32;
 };
 constexpr uint64_t kWorkRing = 1ull << 22;  // work-item ring entries
 constexpr int kItemBits = 24;                // items per request < 2^24
+constexpr uint64_t kMaxRequestId = 1ull << (64 - kItemBits);  // ids fit a work item
 constexpr int kCuKeys = 2048;                // (XCC, SE, SH, CU) of a workgroup
 int launch_server(const ServerDev &sd, int grid, hipStream_t stream);
 int server_grid(int max_per_cu);

@@ -385,8 +385,13 @@ void serve_kernel(ServerDev sd) {
       const uint64_t r = s_req;
       // this item's outputs reach memory before it is counted
       __threadfence();
-      const unsigned fin = atomicAdd(&sd.done[r], 1u);
+      unsigned *done = &sd.done[r % (uint64_t)sd.nslots];
+      const unsigned fin = atomicAdd(done, 1u);
       if ((int64_t)fin + 1 == J.item_begin) {
+        // every item is counted: the slot's counter starts from zero for the
+        // next request in it, which the host submits only after reading the
+        // completion record written below
+        __hip_atomic_store(done, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __threadfence_system();
         Completion *cp = &sd.comps[r % (uint64_t)sd.nslots];
         __hip_atomic_store(&cp->gpu_time, wall_clock64(), __ATOMIC_RELAXED,

@@ -76,9 +76,11 @@ struct Slot {
   int64_t items = 0;
 };
 
-// The device loop exits after this long without work (so it never outlives
-// a vanished host); submit / poll relaunch it when it is needed again.
-constexpr uint64_t kIdleTimeoutTicks = 2ull * 100000000ull;  // 2 s at 100 MHz
+// The device loop exits after this long without requests (so it never
+// outlives a vanished host, nor holds its hardware queue for long); submit /
+// poll relaunch it when it is needed again.
+constexpr double kTicksPerSecond = 1e8;  // wall_clock64 runs at 100 MHz
+constexpr uint64_t kIdleTimeoutTicks = (uint64_t)(0.020 * kTicksPerSecond);  // 20 ms
 
 // Output bytes in flight above which the device counts as busy (about four
 // 256 KiB items per worker).
@@ -118,9 +120,7 @@ int lg(int n) { return 31 - __builtin_clz((unsigned)n); }
 
 void free_server(fftoc_server_t *s) {
   if (s->d_pool) (void)hipFree(s->d_pool);
-  if (s->sd.claim) (void)hipFree(s->sd.claim);
   if (s->sd.done) (void)hipFree(s->sd.done);
-  if (s->sd.head) (void)hipFree(s->sd.head);
   if (s->sd.dtab) (void)hipFree(s->sd.dtab);
   if (s->sd.dev_pub) (void)hipFree(s->sd.dev_pub);
   if (s->sd.dev_stop) (void)hipFree(s->sd.dev_stop);
@@ -210,16 +210,13 @@ int fftoc_server_create(size_t hbm_budget_bytes, int max_inflight, fftoc_server 
   }
   s->nslots = max_inflight;
   s->slots.assign(max_inflight, Slot{});
-  s->sd.max_requests = 1ull << 22;
   s->sd.nslots = max_inflight;
   s->sd.idle_timeout_ticks = kIdleTimeoutTicks;
   s->sd.tw64 = tw64;
   s->sd.tw32 = tw32;
   rc = FFTOC_SUCCESS;
   if (hipMalloc(&s->d_pool, hbm_budget_bytes) != hipSuccess ||
-      hipMalloc(&s->sd.claim, s->sd.max_requests * sizeof(unsigned)) != hipSuccess ||
-      hipMalloc(&s->sd.done, s->sd.max_requests * sizeof(unsigned)) != hipSuccess ||
-      hipMalloc(&s->sd.head, sizeof(unsigned long long)) != hipSuccess ||
+      hipMalloc(&s->sd.done, (size_t)max_inflight * sizeof(unsigned)) != hipSuccess ||
       hipMalloc(&s->sd.dtab, (size_t)max_inflight * sizeof(ReqDesc)) != hipSuccess ||
       hipMalloc(&s->sd.dev_pub, sizeof(unsigned long long)) != hipSuccess ||
       hipMalloc(&s->sd.dev_stop, sizeof(unsigned)) != hipSuccess ||
@@ -229,9 +226,7 @@ int fftoc_server_create(size_t hbm_budget_bytes, int max_inflight, fftoc_server 
       hipMalloc(&s->sd.cu_seats, kCuKeys * sizeof(unsigned)) != hipSuccess)
     rc = FFTOC_ERROR_ALLOC;
   if (rc == FFTOC_SUCCESS &&
-      (hipMemset(s->sd.claim, 0, s->sd.max_requests * sizeof(unsigned)) != hipSuccess ||
-       hipMemset(s->sd.done, 0, s->sd.max_requests * sizeof(unsigned)) != hipSuccess ||
-       hipMemset(s->sd.head, 0, sizeof(unsigned long long)) != hipSuccess ||
+      (hipMemset(s->sd.done, 0, (size_t)max_inflight * sizeof(unsigned)) != hipSuccess ||
        hipMemset(s->sd.dev_pub, 0, sizeof(unsigned long long)) != hipSuccess ||
        hipMemset(s->sd.wpub, 0, sizeof(unsigned long long)) != hipSuccess ||
        hipMemset(s->sd.dtab, 0xff, (size_t)max_inflight * sizeof(ReqDesc)) != hipSuccess))
@@ -312,7 +307,7 @@ int fftoc_server_submit(fftoc_server s, const void *in, int64_t batch, int n, ui
   constexpr int64_t kRingItems = (int64_t)kWorkRing / 2;  // keep half the ring as margin
   if (bytes > s->pool.capacity() || nitems > kRingItems || nitems >= (1 << kItemBits))
     return FFTOC_ERROR_BUDGET;
-  if (s->next_seq >= s->sd.max_requests) return FFTOC_ERROR_QUEUE_FULL;
+  if (s->next_seq >= kMaxRequestId) return FFTOC_ERROR_QUEUE_FULL;  // 2^40 per server
   const int slot = (int)(s->next_seq % (uint64_t)s->nslots);
   if (s->slots[slot].busy) return FFTOC_ERROR_QUEUE_FULL;  // ring full
   if (s->inflight_items + nitems > kRingItems) return FFTOC_ERROR_QUEUE_FULL;
@@ -475,6 +470,14 @@ int fftoc_server_stats(fftoc_server s, size_t *budget_used, int *inflight) {
   std::lock_guard<std::mutex> lock(s->m);
   if (budget_used) *budget_used = s->pool.used();
   if (inflight) *inflight = (int)s->slot_of.size();
+  return FFTOC_SUCCESS;
+}
+
+int fftoc_server_set_idle_timeout(fftoc_server s, double seconds) {
+  if (!s) return FFTOC_ERROR_INVALID_HANDLE;
+  if (!(seconds >= 1e-4 && seconds <= 60.0)) return FFTOC_ERROR_INVALID_BATCH;
+  std::lock_guard<std::mutex> lock(s->m);
+  s->sd.idle_timeout_ticks = (uint64_t)(seconds * kTicksPerSecond);
   return FFTOC_SUCCESS;
 }
 

@@ -192,9 +192,14 @@ int fftoc_plan_destroy(fftoc_plan plan);
  * latency at light load and the throughput at saturation.  A request is
  * admitted when a ring slot is free and its output fits in the free part of
  * the HBM budget (the server's output pool); requests that arrive while others
- * run start on idle workgroups at once.  While a server runs,
- * hipDeviceSynchronize() waits for the server to stop: synchronise streams
- * instead. */
+ * run start on idle workgroups at once.
+ * The loop leaves after an idle timeout without requests (20 ms by default,
+ * fftoc_server_set_idle_timeout) and is relaunched by the next submit or
+ * poll.  While it runs, work that waits for the whole device waits for it to
+ * go idle and leave: hipDeviceSynchronize(), and any stream HIP places on the
+ * loop's hardware queue (HIP maps streams onto GPU_MAX_HW_QUEUES queues, 4 by
+ * default), including synchronous hipMemcpy on the default stream.  Such a
+ * wait therefore lasts until the idle timeout after the last request. */
 typedef struct fftoc_server_t *fftoc_server;
 
 /* hbm_budget_bytes: size of the output pool; max_inflight: ring slots. */
@@ -216,6 +221,9 @@ int fftoc_server_gpu_time(fftoc_server server, uint64_t request_id, uint64_t *ti
 int fftoc_server_replay(fftoc_server server, const fftoc_job *reqs, int count,
                         const double *arrival_s, double *latency_s, double *span_s);
 int fftoc_server_stats(fftoc_server server, size_t *budget_used, int *inflight);
+/* Idle time after which the device loop leaves (1e-4 .. 60 s); takes effect
+ * from the loop's next launch. */
+int fftoc_server_set_idle_timeout(fftoc_server server, double seconds);
 /* Stops the loop (waits for it to drain) and frees everything. */
 int fftoc_server_destroy(fftoc_server server);
 

@@ -82,6 +82,26 @@ def test_server_requests_match_oracle():
         check(r, got)
 
 
+def test_server_reuses_slots_many_times():
+    """Three ring slots for 120 requests, submitted as slots free up: every
+    slot (and its completion counter) serves 40 requests in turn."""
+    reqs = make_requests(5, 120, max_batch=64)
+    with fo.Server(64 << 20, 3) as srv:
+        print("[server] 120 requests through 3 slots", flush=True)
+        pending, outs = [], [None] * len(reqs)
+        for k, r in enumerate(reqs):
+            pending.append((k, submit_retry(srv, r["buf"], r["x"].shape[0], r["n"], r["flags"])))
+            if len(pending) == 3:
+                j, rid = pending.pop(0)
+                outs[j] = hip.download_ptr(srv.wait(rid), reqs[j]["x"].shape, reqs[j]["x"].dtype)
+                srv.release(rid)
+        for j, rid in pending:
+            outs[j] = hip.download_ptr(srv.wait(rid), reqs[j]["x"].shape, reqs[j]["x"].dtype)
+            srv.release(rid)
+    for r, got in zip(reqs, outs):
+        check(r, got)
+
+
 def test_server_result_same_alone_and_in_mix():
     reqs = make_requests(2, 40)
     with fo.Server(512 << 20, 128) as srv:
@@ -126,11 +146,16 @@ def test_server_relaunches_after_idle_exit():
     x = signals(256, 16, seed=5)
     buf = fo.DeviceBuffer.from_array(x)
     with fo.Server(16 << 20, 16) as srv:
+        srv.set_idle_timeout(0.05)                   # from the next launch on
         rid = srv.submit(buf, 16, 256, 0)
         srv.wait(rid)
         srv.release(rid)
-        print("[server] idle 2.6 s", flush=True)
-        time.sleep(2.6)                              # the device loop times out (2 s)
+        print("[server] idle 0.3 s", flush=True)
+        time.sleep(0.3)                              # the device loop times out (20 ms)
+        rid = srv.submit(buf, 16, 256, 0)
+        srv.wait(rid)
+        srv.release(rid)
+        time.sleep(0.3)                              # and again (50 ms)
         rid = srv.submit(buf, 16, 256, 0)
         got = hip.download_ptr(srv.wait(rid), x.shape)
         srv.release(rid)
