@@ -50,6 +50,40 @@ int validate_job(const fftoc_job &j) {
   return FFTOC_SUCCESS;
 }
 
+// Every job's output range must be disjoint from every other job's input and
+// output range (a job may be in place): the jobs run concurrently.
+int check_cross_overlap(const fftoc_job *jobs, int njobs) {
+  struct Range {
+    uintptr_t lo, hi;
+    int job;
+    bool written;
+  };
+  std::vector<Range> r;
+  r.reserve(2 * (size_t)njobs);
+  for (int i = 0; i < njobs; ++i) {
+    const fftoc_job &j = jobs[i];
+    if (j.batch == 0) continue;
+    const uintptr_t bytes = (uintptr_t)j.batch * (uintptr_t)j.n *
+                            ((j.flags & FFTOC_DATA_C64) ? 8u : 16u);
+    const uintptr_t in = reinterpret_cast<uintptr_t>(j.in);
+    const uintptr_t o = reinterpret_cast<uintptr_t>(j.out);
+    r.push_back({o, o + bytes, i, true});
+    if (in != o) r.push_back({in, in + bytes, i, false});
+  }
+  std::sort(r.begin(), r.end(), [](const Range &a, const Range &b) { return a.lo < b.lo; });
+  // sweep: compare each range with the earlier ones still open
+  std::vector<Range> open;
+  for (const Range &c : r) {
+    open.erase(std::remove_if(open.begin(), open.end(),
+                              [&](const Range &o) { return o.hi <= c.lo; }),
+               open.end());
+    for (const Range &o : open)
+      if (o.job != c.job && (o.written || c.written)) return FFTOC_ERROR_OVERLAP;
+    open.push_back(c);
+  }
+  return FFTOC_SUCCESS;
+}
+
 int first_routine(int n) {
   for (int id = 0; id < batch_routine_count(); ++id)
     if (batch_routine_n(id) == n) return id;
@@ -130,8 +164,10 @@ int fftoc_plan_create(const fftoc_job *jobs, int njobs, fftoc_plan *out) {
     const int rc = validate_job(jobs[i]);
     if (rc != FFTOC_SUCCESS) return rc;
   }
+  int rc = check_cross_overlap(jobs, njobs);
+  if (rc != FFTOC_SUCCESS) return rc;
   const void *tw64 = nullptr, *tw32 = nullptr;
-  int rc = device_tables(&tw64, &tw32);
+  rc = device_tables(&tw64, &tw32);
   if (rc != FFTOC_SUCCESS) return rc;
   fftoc_plan_t *p = new (std::nothrow) fftoc_plan_t;
   if (!p) return FFTOC_ERROR_ALLOC;

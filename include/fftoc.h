@@ -158,8 +158,10 @@ int fftoc_tuning_load(const char *path);
  * kernel (includes/testing.cuh:59-92).  Each job is one call's worth of work
  * (same conventions as fftoc_fft; flags: FFTOC_INVERSE, FFTOC_NORMALIZE,
  * FFTOC_DATA_C64).  A plan fixes the buffers; executing it transforms their
- * current contents.  Executions of one plan must not overlap (they share
- * the plan's work counter); different plans are independent. */
+ * current contents.  A job's output must not overlap any other job's input
+ * or output (FFTOC_ERROR_OVERLAP); a job may run in place.  Executions of one
+ * plan must not overlap (they share the plan's work counter); different
+ * plans are independent. */
 typedef struct fftoc_job {
   const void *in;
   void *out;

@@ -92,3 +92,18 @@ def test_fused_example_library_loads_and_checks_arguments():
     assert f(None, None, 4, 64, None, 0, 1.0, 0, None) == 3      # FFTOC_ERROR_NULL_POINTER
     assert f(16, 16, 4, 64, 32, 0, 1.0, 0, None) == 5            # FFTOC_ERROR_OVERLAP
     assert f(16, 4096, 4, 48, 32, 0, 1.0, 0, None) == 1          # FFTOC_ERROR_INVALID_SIZE
+
+
+def test_plan_rejects_jobs_writing_into_each_other():
+    """Checked before any device work: a job's output overlapping another
+    job's input or output is FFTOC_ERROR_OVERLAP; shared inputs and in-place
+    jobs are fine as far as validation goes."""
+    J = _native.Job
+    base = 1 << 20
+    size = 64 * 16 * 4  # 4 signals of 64 points
+    bad = [J(base, base + 8 * size, 4, 64, 0), J(base + 16 * size, base + 8 * size + 256, 4, 64, 0)]
+    bad_in = [J(base, base + 8 * size, 4, 64, 0), J(base + 8 * size + 512, base + 32 * size, 4, 64, 0)]
+    for jobs in (bad, bad_in):
+        arr = (J * len(jobs))(*jobs)
+        h = ctypes.c_void_p()
+        assert fo.lib().fftoc_plan_create(arr, len(jobs), ctypes.byref(h)) == 5
