@@ -99,9 +99,11 @@ __device__ __forceinline__ c2 ld_in(const c2 *p) {
   return v;
 }
 
-template <bool INV>
+// SC: the launch has a scale other than 1 (FFTOC_NORMALIZE).  A compile-time
+// choice: a run-time test costs two selects and two multiplies per value.
+template <bool INV, bool SC>
 __device__ __forceinline__ void st_out(c2 *p, c2 v, double scale) {
-  if (scale != 1.0) v = cscale(v, scale);
+  if constexpr (SC) v = cscale(v, scale);
   if constexpr (INV) v = cswap(v);
   st_stream(p, v);
 }
@@ -120,7 +122,7 @@ constexpr int kWaves = 4;  // waves per workgroup for the one-unit-per-wave kern
 
 // N = 8: 32 signals per unit.  A = [x_a | x_b] (16 rows, K = 16: chunks 0-1
 // hold signal i's 8 inputs, chunks 2-3 those of signal 16+i), B = diag(F8,F8).
-template <int G, bool INV>
+template <int G, bool INV, bool SC>
 __global__ __launch_bounds__(64 * kWaves) void tensor8_kernel(const c2 *in, c2 *out, int64_t batch,
                                                               const c2 *__restrict__ tw,
                                                               double scale) {
@@ -144,14 +146,14 @@ __global__ __launch_bounds__(64 * kWaves) void tensor8_kernel(const c2 *in, c2 *
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int64_t s = sw + 32 * u + (j < 8 ? 0 : 16) + 4 * r + g;
-      if (s < batch) st_out<INV>(out + s * 8 + (j & 7), make_double2(yr[r], yi[r]), scale);
+      if (s < batch) st_out<INV, SC>(out + s * 8 + (j & 7), make_double2(yr[r], yi[r]), scale);
     }
   }
 }
 
 // ---------------------------------------------------------------------------
 // N = 16: 16 signals per unit, one DFT16 per row.
-template <int G, bool INV>
+template <int G, bool INV, bool SC>
 __global__ __launch_bounds__(64 * kWaves) void tensor16_kernel(const c2 *in, c2 *out, int64_t batch,
                                                                const c2 *__restrict__ tw,
                                                                double scale) {
@@ -174,7 +176,7 @@ __global__ __launch_bounds__(64 * kWaves) void tensor16_kernel(const c2 *in, c2 
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int64_t s = sw + 16 * u + 4 * r + g;
-      if (s < batch) st_out<INV>(out + s * 16 + j, make_double2(yr[r], yi[r]), scale);
+      if (s < batch) st_out<INV, SC>(out + s * 16 + j, make_double2(yr[r], yi[r]), scale);
     }
   }
 }
@@ -188,7 +190,7 @@ __global__ __launch_bounds__(64 * kWaves) void tensor16_kernel(const c2 *in, c2 
 //   M = 4: signal g, n2 = r                 -> all four n2 in-lane
 //   M = 8: signal g % 2, n2 = 2r + g / 2    -> even n2 here, odd n2 in lane
 //          l ^ 32; DFT4 in-lane, then one radix-2 exchange across the pair.
-template <int M, int G, bool INV>
+template <int M, int G, bool INV, bool SC>
 __global__ __launch_bounds__(64 * kWaves) void tensor16xm_kernel(const c2 *in, c2 *out,
                                                                  int64_t batch,
                                                                  const c2 *__restrict__ tw,
@@ -230,8 +232,8 @@ __global__ __launch_bounds__(64 * kWaves) void tensor16xm_kernel(const c2 *in, c
       c2 u0 = y[h], u1 = y[h + 2];
       dft2(u0, u1);
       if (s < batch) {
-        st_out<INV>(out + s * N + j, u0, scale);
-        st_out<INV>(out + s * N + j + 16, u1, scale);
+        st_out<INV, SC>(out + s * N + j, u0, scale);
+        st_out<INV, SC>(out + s * N + j + 16, u1, scale);
       }
     }
   } else if constexpr (M == 4) {
@@ -239,7 +241,7 @@ __global__ __launch_bounds__(64 * kWaves) void tensor16xm_kernel(const c2 *in, c
     const int64_t s = s0 + g;
     if (s < batch) {
 #pragma unroll
-      for (int k2 = 0; k2 < 4; ++k2) st_out<INV>(out + s * N + j + 16 * k2, y[k2], scale);
+      for (int k2 = 0; k2 < 4; ++k2) st_out<INV, SC>(out + s * N + j + 16 * k2, y[k2], scale);
     }
   } else {
     static_assert(M == 8, "M must be 2, 4 or 8");
@@ -263,7 +265,7 @@ __global__ __launch_bounds__(64 * kWaves) void tensor16xm_kernel(const c2 *in, c
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
         const c2 v = (h == 0) ? cadd(y[k], p[k]) : csub(p[k], y[k]);
-        st_out<INV>(out + s * N + j + 16 * (4 * h + k), v, scale);
+        st_out<INV, SC>(out + s * N + j + 16 * (4 * h + k), v, scale);
       }
     }
   }
@@ -297,7 +299,7 @@ __device__ __forceinline__ int swz16(int e) { return e ^ ((e >> 4) & 15); }
 
 // N = 256: one signal per unit, two MFMA stages, output re-ordered through
 // 4 KiB of LDS per wave so every store instruction writes 1 KiB contiguously.
-template <int G, bool INV>
+template <int G, bool INV, bool SC>
 __global__ __launch_bounds__(64 * kWaves) void tensor256_kernel(const c2 *in, c2 *out,
                                                                 int64_t batch,
                                                                 const c2 *__restrict__ tw,
@@ -326,7 +328,7 @@ __global__ __launch_bounds__(64 * kWaves) void tensor256_kernel(const c2 *in, c2
     if (sw + u < batch) {
       c2 *y = out + (sw + u) * 256;
 #pragma unroll
-      for (int c = 0; c < 4; ++c) st_out<INV>(y + 64 * c + lane, buf[swz16(64 * c + lane)], scale);
+      for (int c = 0; c < 4; ++c) st_out<INV, SC>(y + 64 * c + lane, buf[swz16(64 * c + lane)], scale);
     }
   }
 }
@@ -338,7 +340,7 @@ __global__ __launch_bounds__(64 * kWaves) void tensor256_kernel(const c2 *in, c2
 // stage 2 (two groups k2a = 0, 1): DFT16 over n2' with the free transpose.
 // X[k1 + 16 k2a + 32 k2b]: lane l register r of group k2a holds
 // k1 = 4r + g, k2b = j.
-template <int G, bool INV>
+template <int G, bool INV, bool SC>
 __global__ __launch_bounds__(64 * kWaves) void tensor512_kernel(const c2 *in, c2 *out,
                                                                 int64_t batch,
                                                                 const c2 *__restrict__ tw,
@@ -399,7 +401,7 @@ __global__ __launch_bounds__(64 * kWaves) void tensor512_kernel(const c2 *in, c2
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
       const int e = 64 * c + lane;
-      st_out<INV>(y + e, buf[e ^ ((e >> 5) & 15)], scale);
+      st_out<INV, SC>(y + e, buf[e ^ ((e >> 5) & 15)], scale);
     }
   }
   }
@@ -418,7 +420,7 @@ constexpr int k4096Waves = 8;
 __device__ __forceinline__ int swz_z(int k1, int m) { return k1 * 256 + (m ^ k1); }
 __device__ __forceinline__ int swz_x(int e) { return e ^ ((e >> 8) & 15); }
 
-template <bool INV>
+template <bool INV, bool SC>
 __global__ __launch_bounds__(64 * k4096Waves) void tensor4096_kernel(const c2 *in, c2 *out,
                                                                      int64_t batch,
                                                                      const c2 *__restrict__ tw,
@@ -475,7 +477,7 @@ __global__ __launch_bounds__(64 * k4096Waves) void tensor4096_kernel(const c2 *i
 #pragma unroll
   for (int c = 0; c < 8; ++c) {
     const int e = 512 * c + threadIdx.x;
-    st_out<INV>(y + e, lds[swz_x(e)], scale);
+    st_out<INV, SC>(y + e, lds[swz_x(e)], scale);
   }
 }
 
@@ -489,12 +491,20 @@ int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
     const c2 *tw = static_cast<const c2 *>(a.twiddles);                                        \
     const int64_t grid = ceil_div(a.batch, (UNITS_PER_WG));                                    \
     if (grid > 0x7fffffffLL) return FFTOC_ERROR_INVALID_BATCH;                                 \
-    if (a.inverse)                                                                             \
-      hipLaunchKernelGGL((KERNEL<__VA_ARGS__ __VA_OPT__(, ) true>), dim3((unsigned)grid),      \
-                         dim3(BLOCK), 0, a.stream, in, out, a.batch, tw, a.scale);             \
+    const dim3 gd((unsigned)grid), bd(BLOCK);                                                \
+    const bool sc = a.scale != 1.0;                                                            \
+    if (a.inverse && sc)                                                                       \
+      hipLaunchKernelGGL((KERNEL<__VA_ARGS__ __VA_OPT__(, ) true, true>), gd, bd, 0, a.stream, \
+                         in, out, a.batch, tw, a.scale);                                       \
+    else if (a.inverse)                                                                        \
+      hipLaunchKernelGGL((KERNEL<__VA_ARGS__ __VA_OPT__(, ) true, false>), gd, bd, 0,          \
+                         a.stream, in, out, a.batch, tw, a.scale);                             \
+    else if (sc)                                                                               \
+      hipLaunchKernelGGL((KERNEL<__VA_ARGS__ __VA_OPT__(, ) false, true>), gd, bd, 0,          \
+                         a.stream, in, out, a.batch, tw, a.scale);                             \
     else                                                                                       \
-      hipLaunchKernelGGL((KERNEL<__VA_ARGS__ __VA_OPT__(, ) false>), dim3((unsigned)grid),     \
-                         dim3(BLOCK), 0, a.stream, in, out, a.batch, tw, a.scale);             \
+      hipLaunchKernelGGL((KERNEL<__VA_ARGS__ __VA_OPT__(, ) false, false>), gd, bd, 0,         \
+                         a.stream, in, out, a.batch, tw, a.scale);                             \
     return hipGetLastError() == hipSuccess ? FFTOC_SUCCESS : FFTOC_ERROR_LAUNCH;               \
   }
 
