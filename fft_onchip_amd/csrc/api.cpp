@@ -156,6 +156,13 @@ const char *default_best(int n, int bucket) {
   }
 }
 
+// Default choice of fftoc_tensor_fft where it depends on the batch size (else
+// the first tensor variant of the length).
+const char *default_tensor(int n, int bucket) {
+  if (n == 64 && bucket <= 16) return "tensor:16x4/8 per wave/persistent";
+  return nullptr;
+}
+
 void build_registry() {
   for (int k = 3; k <= 12; ++k) for (int c64 = 0; c64 < 2; ++c64) {
     const int n = 1 << k;
@@ -176,9 +183,13 @@ void build_registry() {
       }
       for (int b = 0; b < kBuckets; ++b) {
         int pick = first;
-        if (f == Family::Best && !c64) {
+        const char *want = c64 ? nullptr
+                           : f == Family::Best   ? default_best(n, b)
+                           : f == Family::Tensor ? default_tensor(n, b)
+                                                 : nullptr;
+        if (want) {
           for (size_t i = 0; i < r.v.size(); ++i)
-            if (std::strcmp(r.v[i].name, default_best(n, b)) == 0) pick = (int)i;
+            if (std::strcmp(r.v[i].name, want) == 0) pick = (int)i;
         }
         r.choice[family_index(f)][b].store(pick);
       }

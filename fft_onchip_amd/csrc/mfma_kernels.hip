@@ -190,32 +190,28 @@ __global__ __launch_bounds__(64 * kWaves) void tensor16_kernel(const c2 *in, c2 
 //   M = 4: signal g, n2 = r                 -> all four n2 in-lane
 //   M = 8: signal g % 2, n2 = 2r + g / 2    -> even n2 here, odd n2 in lane
 //          l ^ 32; DFT4 in-lane, then one radix-2 exchange across the pair.
-template <int M, int G, bool INV, bool SC>
-__global__ __launch_bounds__(64 * kWaves) void tensor16xm_kernel(const c2 *in, c2 *out,
-                                                                 int64_t batch,
-                                                                 const c2 *__restrict__ tw,
-                                                                 double scale) {
+// Loads the A operand of the unit whose first signal is s0.
+template <int M, bool INV>
+__device__ __forceinline__ void load16xm(const c2 *in, int64_t s0, int64_t batch, int g, int j,
+                                         c2 (&a)[4]) {
   constexpr int N = 16 * M;
-  constexpr int SPW = 16 / M;  // signals per unit
-  const int lane = lane_id(), g = lane >> 4, j = lane & 15;
-  const int64_t sw = wave_global() * (SPW * G);
-  if (sw >= batch) return;
-  const Frag f = dft16_frag(tw, lane);
-  c2 au[G][4];
+  constexpr int SPW = 16 / M;
+  const int64_t s = s0 + j % SPW;
+  const int n2 = j / SPW;
 #pragma unroll
-  for (int u = 0; u < G; ++u) {
-    const int64_t s = sw + SPW * u + j % SPW;
-    const int n2 = j / SPW;
-#pragma unroll
-    for (int c = 0; c < 4; ++c)
-      au[u][c] =
-          (s < batch) ? ld_in<INV>(in + s * N + M * (4 * c + g) + n2) : make_double2(0.0, 0.0);
-  }
-#pragma unroll
-  for (int u = 0; u < G; ++u) {
-  const int64_t s0 = sw + SPW * u;
+  for (int c = 0; c < 4; ++c)
+    a[c] = (s < batch) ? ld_in<INV>(in + s * N + M * (4 * c + g) + n2) : make_double2(0.0, 0.0);
+}
+
+// Transforms and stores the unit whose first signal is s0.
+template <int M, bool INV, bool SC>
+__device__ __forceinline__ void unit16xm(const c2 (&a)[4], const Frag &f,
+                                         const c2 *__restrict__ tw, c2 *out, int64_t s0,
+                                         int64_t batch, double scale, int g, int j) {
+  constexpr int N = 16 * M;
+  constexpr int SPW = 16 / M;
   v4d yr, yi;
-  kgemm(au[u], f, yr, yi);
+  kgemm(a, f, yr, yi);
   c2 y[4];
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
@@ -269,7 +265,89 @@ __global__ __launch_bounds__(64 * kWaves) void tensor16xm_kernel(const c2 *in, c
       }
     }
   }
+}
+
+template <int M, int G, bool INV, bool SC>
+__global__ __launch_bounds__(64 * kWaves) void tensor16xm_kernel(const c2 *in, c2 *out,
+                                                                 int64_t batch,
+                                                                 const c2 *__restrict__ tw,
+                                                                 double scale) {
+  constexpr int SPW = 16 / M;  // signals per unit
+  const int lane = lane_id(), g = lane >> 4, j = lane & 15;
+  const int64_t sw = wave_global() * (SPW * G);
+  if (sw >= batch) return;
+  const Frag f = dft16_frag(tw, lane);
+  c2 au[G][4];
+#pragma unroll
+  for (int u = 0; u < G; ++u) load16xm<M, INV>(in, sw + SPW * u, batch, g, j, au[u]);
+#pragma unroll
+  for (int u = 0; u < G; ++u) unit16xm<M, INV, SC>(au[u], f, tw, out, sw + SPW * u, batch, scale, g, j);
+}
+
+// Persistent form: wave w handles unit groups w, w + (waves in the grid), ...
+// of G units each; the next group's loads are issued before the current
+// group is transformed, so they are in flight during its matrix products and
+// stores.
+template <int M, int G, bool INV, bool SC>
+__global__ __launch_bounds__(64 * kWaves) void tensor16xm_persistent_kernel(
+    const c2 *in, c2 *out, int64_t batch, const c2 *__restrict__ tw, double scale) {
+  constexpr int SPG = (16 / M) * G;  // signals per group
+  const int lane = lane_id(), g = lane >> 4, j = lane & 15;
+  const int64_t groups = (batch + SPG - 1) / SPG;
+  const int64_t stride = (int64_t)gridDim.x * kWaves;
+  int64_t q = wave_global();
+  if (q >= groups) return;
+  const Frag f = dft16_frag(tw, lane);
+  c2 cur[G][4];
+#pragma unroll
+  for (int u = 0; u < G; ++u) load16xm<M, INV>(in, q * SPG + (16 / M) * u, batch, g, j, cur[u]);
+  for (;;) {
+    const int64_t qn = q + stride;
+    c2 nxt[G][4];
+    if (qn < groups) {
+#pragma unroll
+      for (int u = 0; u < G; ++u)
+        load16xm<M, INV>(in, qn * SPG + (16 / M) * u, batch, g, j, nxt[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < G; ++u)
+      unit16xm<M, INV, SC>(cur[u], f, tw, out, q * SPG + (16 / M) * u, batch, scale, g, j);
+    if (qn >= groups) break;
+#pragma unroll
+    for (int u = 0; u < G; ++u)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) cur[u][c] = nxt[u][c];
+    q = qn;
   }
+}
+
+template <int M, int G>
+int launch_t16xm_persistent(const LaunchArgs &a) {
+  if (a.precision != FFTOC_PREC_FP64) return FFTOC_ERROR_UNSUPPORTED;
+  if (a.batch <= 0) return FFTOC_SUCCESS;
+  constexpr int SPG = (16 / M) * G;
+  const int64_t groups = (a.batch + SPG - 1) / SPG;
+  const int64_t resident =
+      resident_workgroups<tensor16xm_persistent_kernel<M, G, false, false>, 64 * kWaves>();
+  const int64_t need = (groups + kWaves - 1) / kWaves;
+  const dim3 gd((unsigned)(need < resident ? need : resident)), bd(64 * kWaves);
+  const c2 *in = static_cast<const c2 *>(a.in);
+  c2 *out = static_cast<c2 *>(a.out);
+  const c2 *tw = static_cast<const c2 *>(a.twiddles);
+  const bool sc = a.scale != 1.0;
+  if (a.inverse && sc)
+    hipLaunchKernelGGL((tensor16xm_persistent_kernel<M, G, true, true>), gd, bd, 0, a.stream, in,
+                       out, a.batch, tw, a.scale);
+  else if (a.inverse)
+    hipLaunchKernelGGL((tensor16xm_persistent_kernel<M, G, true, false>), gd, bd, 0, a.stream, in,
+                       out, a.batch, tw, a.scale);
+  else if (sc)
+    hipLaunchKernelGGL((tensor16xm_persistent_kernel<M, G, false, true>), gd, bd, 0, a.stream, in,
+                       out, a.batch, tw, a.scale);
+  else
+    hipLaunchKernelGGL((tensor16xm_persistent_kernel<M, G, false, false>), gd, bd, 0, a.stream, in,
+                       out, a.batch, tw, a.scale);
+  return hipGetLastError() == hipSuccess ? FFTOC_SUCCESS : FFTOC_ERROR_LAUNCH;
 }
 
 // ---------------------------------------------------------------------------
@@ -542,9 +620,11 @@ const Variant kT32[] = {{"tensor:16x2/8 per wave", TF, &launch_t32},
                         {"tensor:16x2/16 per wave", TF, &launch_t32g2},
                         {"tensor:16x2/32 per wave", TF, &launch_t32g4}};
 const Variant kT64[] = {{"tensor:16x4/8 per wave", TF, &launch_t64g2},
+                        {"tensor:16x4/8 per wave/persistent", TF, &launch_t16xm_persistent<4, 2>},
                         {"tensor:16x4/4 per wave", TF, &launch_t64},
                         {"tensor:16x4/16 per wave", TF, &launch_t64g4}};
 const Variant kT128[] = {{"tensor:16x8/4 per wave", TF, &launch_t128g2},
+
                          {"tensor:16x8/2 per wave", TF, &launch_t128},
                          {"tensor:16x8/8 per wave", TF, &launch_t128g4}};
 const Variant kT256[] = {{"tensor:16x16/4 per wave", TF, &launch_t256g4},
