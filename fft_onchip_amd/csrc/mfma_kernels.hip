@@ -412,41 +412,34 @@ __global__ __launch_bounds__(64 * kWaves) void tensor256_kernel(const c2 *in, c2
 }
 
 // ---------------------------------------------------------------------------
-// N = 512 = 16 x 32, one signal per wave.  n = 32 n1 + n2, n2 = 16 h + n2'.
+// N = 512 = 16 x 32, one signal per unit.  n = 32 n1 + n2, n2 = 16 h + n2'.
 // Stage 1 (two MFMA groups h = 0, 1): DFT16 over n1 -> Y_h[n2'][k1];
 // twiddle W_512^{n2 k1}; radix-2 over h in registers with W_32^{n2' k2a};
 // stage 2 (two groups k2a = 0, 1): DFT16 over n2' with the free transpose.
 // X[k1 + 16 k2a + 32 k2b]: lane l register r of group k2a holds
 // k1 = 4r + g, k2b = j.
-template <int G, bool INV, bool SC>
-__global__ __launch_bounds__(64 * kWaves) void tensor512_kernel(const c2 *in, c2 *out,
-                                                                int64_t batch,
-                                                                const c2 *__restrict__ tw,
-                                                                double scale) {
-  __shared__ c2 lds[kWaves * 512];
-  const int lane = lane_id(), g = lane >> 4, j = lane & 15;
-  const int64_t sw = wave_global() * G;
-  if (sw >= batch) return;
-  c2 *buf = lds + (threadIdx.x >> 6) * 512;
-  const Frag f = dft16_frag(tw, lane);
-  c2 au[G][2][4];
+template <bool INV>
+__device__ __forceinline__ void load512(const c2 *in, int64_t s, int g, int j, c2 (&a)[2][4]) {
+  // n1 = 4c + g, n2 = 16h + j
 #pragma unroll
-  for (int u = 0; u < G; ++u) {
-    const int64_t s = sw + u < batch ? sw + u : sw;  // a short tail re-reads signal sw
-    // n1 = 4c + g, n2 = 16h + j
+  for (int h = 0; h < 2; ++h)
 #pragma unroll
-    for (int h = 0; h < 2; ++h)
-#pragma unroll
-      for (int c = 0; c < 4; ++c) au[u][h][c] = ld_in<INV>(in + s * 512 + 32 * (4 * c + g) + 16 * h + j);
-  }
-#pragma unroll
-  for (int u = 0; u < G; ++u) {
-  const int64_t s = sw + u;
+    for (int c = 0; c < 4; ++c) a[h][c] = ld_in<INV>(in + s * 512 + 32 * (4 * c + g) + 16 * h + j);
+}
+
+// Transforms signal s from its operands and stores it (if s < batch) through
+// the wave's 512-entry LDS buffer; `reuse`: the buffer still serves the
+// previous unit's stores.
+template <bool INV, bool SC>
+__device__ __forceinline__ void unit512(const c2 (&a)[2][4], const Frag &f,
+                                        const c2 *__restrict__ tw, c2 *buf, c2 *out, int64_t s,
+                                        int64_t batch, double scale, int lane, bool reuse) {
+  const int g = lane >> 4, j = lane & 15;
   c2 yh[2][4];
 #pragma unroll
   for (int h = 0; h < 2; ++h) {
     v4d yr, yi;
-    kgemm(au[u][h], f, yr, yi);
+    kgemm(a[h], f, yr, yi);
     // reg r: n2' = 4r + g (n2 = 16h + n2'), k1 = j
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
@@ -462,7 +455,7 @@ __global__ __launch_bounds__(64 * kWaves) void tensor512_kernel(const c2 *in, c2
     b[0][r] = cadd(yh[0][r], yh[1][r]);
     b[1][r] = cmul(csub(yh[0][r], yh[1][r]), tw[n2p << 7]);  // W_32^{n2'}
   }
-  if (u > 0) wave_lds_sync();  // the previous unit's LDS reads are done
+  if (reuse) wave_lds_sync();  // the previous unit's LDS reads are done
 #pragma unroll
   for (int k2a = 0; k2a < 2; ++k2a) {
     v4d zr, zi;
@@ -482,6 +475,57 @@ __global__ __launch_bounds__(64 * kWaves) void tensor512_kernel(const c2 *in, c2
       st_out<INV, SC>(y + e, buf[e ^ ((e >> 5) & 15)], scale);
     }
   }
+}
+
+template <int G, bool INV, bool SC>
+__global__ __launch_bounds__(64 * kWaves) void tensor512_kernel(const c2 *in, c2 *out,
+                                                                int64_t batch,
+                                                                const c2 *__restrict__ tw,
+                                                                double scale) {
+  __shared__ c2 lds[kWaves * 512];
+  const int lane = lane_id(), g = lane >> 4, j = lane & 15;
+  const int64_t sw = wave_global() * G;
+  if (sw >= batch) return;
+  c2 *buf = lds + (threadIdx.x >> 6) * 512;
+  const Frag f = dft16_frag(tw, lane);
+  c2 au[G][2][4];
+#pragma unroll
+  for (int u = 0; u < G; ++u)
+    load512<INV>(in, sw + u < batch ? sw + u : sw, g, j, au[u]);  // a short tail re-reads sw
+#pragma unroll
+  for (int u = 0; u < G; ++u) unit512<INV, SC>(au[u], f, tw, buf, out, sw + u, batch, scale, lane, u > 0);
+}
+
+// Persistent form: wave w handles signals w, w + (waves in the grid), ...;
+// the next signal's loads are issued before the current one is transformed.
+template <bool INV, bool SC>
+__global__ __launch_bounds__(64 * kWaves) void tensor512p_kernel(const c2 *in, c2 *out,
+                                                                 int64_t batch,
+                                                                 const c2 *__restrict__ tw,
+                                                                 double scale) {
+  __shared__ c2 lds[kWaves * 512];
+  const int lane = lane_id(), g = lane >> 4, j = lane & 15;
+  const int64_t stride = (int64_t)gridDim.x * kWaves;
+  int64_t s = wave_global();
+  if (s >= batch) return;
+  c2 *buf = lds + (threadIdx.x >> 6) * 512;
+  const Frag f = dft16_frag(tw, lane);
+  c2 cur[2][4];
+  load512<INV>(in, s, g, j, cur);
+  for (bool reuse = false;; reuse = true) {
+    // keeps the twiddle loads inside the loop (hoisted they would pin VGPRs)
+    const c2 *twl = tw;
+    asm volatile("" : "+s"(twl));
+    const int64_t sn = s + stride;
+    c2 nxt[2][4];
+    if (sn < batch) load512<INV>(in, sn, g, j, nxt);
+    unit512<INV, SC>(cur, f, twl, buf, out, s, batch, scale, lane, reuse);
+    if (sn >= batch) break;
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) cur[h][c] = nxt[h][c];
+    s = sn;
   }
 }
 
@@ -561,6 +605,37 @@ __global__ __launch_bounds__(64 * k4096Waves) void tensor4096_kernel(const c2 *i
 
 int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
+// Persistent launch of a one-signal-per-wave kernel: one wave per signal up
+// to the resident count.
+
+#define FFTOC_PERSISTENT_LAUNCHER(FN, KERNEL)                                                  \
+  int FN(const LaunchArgs &a) {                                                               \
+    if (a.precision != FFTOC_PREC_FP64) return FFTOC_ERROR_UNSUPPORTED;                       \
+    if (a.batch <= 0) return FFTOC_SUCCESS;                                                   \
+    const int64_t resident = resident_workgroups<KERNEL<false, false>, 64 * kWaves>();        \
+    const int64_t need = ceil_div(a.batch, kWaves);                                           \
+    const dim3 gd((unsigned)(need < resident ? need : resident)), bd(64 * kWaves);            \
+    const c2 *in = static_cast<const c2 *>(a.in);                                             \
+    c2 *out = static_cast<c2 *>(a.out);                                                       \
+    const c2 *tw = static_cast<const c2 *>(a.twiddles);                                       \
+    const bool sc = a.scale != 1.0;                                                           \
+    if (a.inverse && sc)                                                                      \
+      hipLaunchKernelGGL((KERNEL<true, true>), gd, bd, 0, a.stream, in, out, a.batch, tw,     \
+                         a.scale);                                                            \
+    else if (a.inverse)                                                                       \
+      hipLaunchKernelGGL((KERNEL<true, false>), gd, bd, 0, a.stream, in, out, a.batch, tw,    \
+                         a.scale);                                                            \
+    else if (sc)                                                                              \
+      hipLaunchKernelGGL((KERNEL<false, true>), gd, bd, 0, a.stream, in, out, a.batch, tw,    \
+                         a.scale);                                                            \
+    else                                                                                      \
+      hipLaunchKernelGGL((KERNEL<false, false>), gd, bd, 0, a.stream, in, out, a.batch, tw,   \
+                         a.scale);                                                            \
+    return hipGetLastError() == hipSuccess ? FFTOC_SUCCESS : FFTOC_ERROR_LAUNCH;              \
+  }
+FFTOC_PERSISTENT_LAUNCHER(launch_t512p, tensor512p_kernel)
+#undef FFTOC_PERSISTENT_LAUNCHER
+
 #define FFTOC_TENSOR_LAUNCHER(FN, KERNEL, UNITS_PER_WG, BLOCK, ...)                              \
   int FN(const LaunchArgs &a) {                                                                \
     if (a.precision != FFTOC_PREC_FP64) return FFTOC_ERROR_UNSUPPORTED;                        \
@@ -630,7 +705,8 @@ const Variant kT128[] = {{"tensor:16x8/4 per wave", TF, &launch_t128g2},
 const Variant kT256[] = {{"tensor:16x16/4 per wave", TF, &launch_t256g4},
                          {"tensor:16x16/1 per wave", TF, &launch_t256},
                          {"tensor:16x16/2 per wave", TF, &launch_t256g2}};
-const Variant kT512[] = {{"tensor:16x2x16/1 per wave", TF, &launch_t512},
+const Variant kT512[] = {{"tensor:16x2x16/1 per wave/persistent", TF, &launch_t512p},
+                         {"tensor:16x2x16/1 per wave", TF, &launch_t512},
                          {"tensor:16x2x16/2 per wave", TF, &launch_t512g2}};
 const Variant kT4096[] = {{"tensor:16x16x16/8 waves", TF, &launch_t4096}};
 
