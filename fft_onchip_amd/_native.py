@@ -37,6 +37,7 @@ SIGNATURES.update({
     "fftoc_set_variant": ([_int, _i64, _int], _int),
     "fftoc_autotune": ([_int, _i64, _int, _vp], _int),
     "fftoc_variant_time_us": ([_int, _int], ctypes.c_double),
+    "fftoc_variant_tile": ([_int, _int], _int),
     "fftoc_tuning_save": ([ctypes.c_char_p], _int),
     "fftoc_tuning_load": ([ctypes.c_char_p], _int),
 })

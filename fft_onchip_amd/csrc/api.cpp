@@ -262,10 +262,18 @@ int run(Family fam, const void *in, void *out, int64_t batch, int n, uint32_t fl
   }
   SizeRegistry *reg = registry(n, a.c64);
   if (!reg) return FFTOC_ERROR_INVALID_SIZE;
-  const int pick =
-      variant >= 0 ? variant : reg->choice[family_index(fam)][batch_bucket(batch)].load();
+  int pick = variant >= 0 ? variant : reg->choice[family_index(fam)][batch_bucket(batch)].load();
   if (pick < 0 || pick >= (int)reg->v.size())
     return a.c64 ? FFTOC_ERROR_UNSUPPORTED : FFTOC_ERROR_INVALID_SIZE;
+  if (a.tile > 0 && reg->v[pick].tile != a.tile) {
+    // FFTOC_TILE(s): the family's variant with s signals per workgroup (the
+    // tuned choice when it has that tile, else the first listed)
+    if (variant >= 0) return FFTOC_ERROR_INVALID_FLAGS;  // contradicts the named variant
+    pick = -1;
+    for (size_t i = 0; i < reg->v.size() && pick < 0; ++i)
+      if ((reg->v[i].families & family_bit(fam)) && reg->v[i].tile == a.tile) pick = (int)i;
+    if (pick < 0) return FFTOC_ERROR_UNSUPPORTED;
+  }
   if (batch == 0) return FFTOC_SUCCESS;
   int dev = 0;
   rc = init_current_device(&dev);
@@ -489,6 +497,12 @@ double fftoc_variant_time_us(int n, int variant) {
   fftoc::SizeRegistry *r = fftoc::registry(n);
   if (!r || variant < 0 || variant >= (int)r->v.size()) return -1.0;
   return r->tuned_us[variant];
+}
+
+int fftoc_variant_tile(int n, int variant) {
+  fftoc::SizeRegistry *r = fftoc::registry(n);
+  if (!r || variant < 0 || variant >= (int)r->v.size()) return -1;
+  return r->v[variant].tile;
 }
 
 // Text format, one line per (entry point, length, batch bucket):

@@ -60,6 +60,7 @@ struct Variant {
   const char *name;
   unsigned families;
   int (*launch)(const LaunchArgs &);
+  int tile;  // signals per workgroup (per step for persistent forms): FFTOC_TILE
 };
 
 // radix_kernels.hip / mfma_kernels.hip: variants available for length n,

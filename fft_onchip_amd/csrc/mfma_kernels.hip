@@ -685,30 +685,30 @@ FFTOC_TENSOR_LAUNCHER(launch_t4096, tensor4096_kernel, 1, 64 * k4096Waves)
 #undef FFTOC_TENSOR_LAUNCHER
 
 constexpr unsigned TF = FAM_TENSOR | FAM_ANY;
-const Variant kT8[] = {{"tensor:diag(F8,F8)/32 per wave", TF, &launch_t8},
-                       {"tensor:diag(F8,F8)/64 per wave", TF, &launch_t8g2},
-                       {"tensor:diag(F8,F8)/128 per wave", TF, &launch_t8g4}};
-const Variant kT16[] = {{"tensor:16/16 per wave", TF, &launch_t16},
-                        {"tensor:16/32 per wave", TF, &launch_t16g2},
-                        {"tensor:16/64 per wave", TF, &launch_t16g4}};
-const Variant kT32[] = {{"tensor:16x2/8 per wave", TF, &launch_t32},
-                        {"tensor:16x2/16 per wave", TF, &launch_t32g2},
-                        {"tensor:16x2/32 per wave", TF, &launch_t32g4}};
-const Variant kT64[] = {{"tensor:16x4/8 per wave", TF, &launch_t64g2},
-                        {"tensor:16x4/8 per wave/persistent", TF, &launch_t16xm_persistent<4, 2>},
-                        {"tensor:16x4/4 per wave", TF, &launch_t64},
-                        {"tensor:16x4/16 per wave", TF, &launch_t64g4}};
-const Variant kT128[] = {{"tensor:16x8/4 per wave", TF, &launch_t128g2},
-
-                         {"tensor:16x8/2 per wave", TF, &launch_t128},
-                         {"tensor:16x8/8 per wave", TF, &launch_t128g4}};
-const Variant kT256[] = {{"tensor:16x16/4 per wave", TF, &launch_t256g4},
-                         {"tensor:16x16/1 per wave", TF, &launch_t256},
-                         {"tensor:16x16/2 per wave", TF, &launch_t256g2}};
-const Variant kT512[] = {{"tensor:16x2x16/1 per wave/persistent", TF, &launch_t512p},
-                         {"tensor:16x2x16/1 per wave", TF, &launch_t512},
-                         {"tensor:16x2x16/2 per wave", TF, &launch_t512g2}};
-const Variant kT4096[] = {{"tensor:16x16x16/8 waves", TF, &launch_t4096}};
+// tile: signals per workgroup (4 waves x signals per wave)
+const Variant kT8[] = {{"tensor:diag(F8,F8)/32 per wave", TF, &launch_t8, 128},
+                       {"tensor:diag(F8,F8)/64 per wave", TF, &launch_t8g2, 256},
+                       {"tensor:diag(F8,F8)/128 per wave", TF, &launch_t8g4, 512}};
+const Variant kT16[] = {{"tensor:16/16 per wave", TF, &launch_t16, 64},
+                        {"tensor:16/32 per wave", TF, &launch_t16g2, 128},
+                        {"tensor:16/64 per wave", TF, &launch_t16g4, 256}};
+const Variant kT32[] = {{"tensor:16x2/8 per wave", TF, &launch_t32, 32},
+                        {"tensor:16x2/16 per wave", TF, &launch_t32g2, 64},
+                        {"tensor:16x2/32 per wave", TF, &launch_t32g4, 128}};
+const Variant kT64[] = {{"tensor:16x4/8 per wave", TF, &launch_t64g2, 32},
+                        {"tensor:16x4/8 per wave/persistent", TF, &launch_t16xm_persistent<4, 2>, 32},
+                        {"tensor:16x4/4 per wave", TF, &launch_t64, 16},
+                        {"tensor:16x4/16 per wave", TF, &launch_t64g4, 64}};
+const Variant kT128[] = {{"tensor:16x8/4 per wave", TF, &launch_t128g2, 16},
+                         {"tensor:16x8/2 per wave", TF, &launch_t128, 8},
+                         {"tensor:16x8/8 per wave", TF, &launch_t128g4, 32}};
+const Variant kT256[] = {{"tensor:16x16/4 per wave", TF, &launch_t256g4, 16},
+                         {"tensor:16x16/1 per wave", TF, &launch_t256, 4},
+                         {"tensor:16x16/2 per wave", TF, &launch_t256g2, 8}};
+const Variant kT512[] = {{"tensor:16x2x16/1 per wave/persistent", TF, &launch_t512p, 4},
+                         {"tensor:16x2x16/1 per wave", TF, &launch_t512, 4},
+                         {"tensor:16x2x16/2 per wave", TF, &launch_t512g2, 8}};
+const Variant kT4096[] = {{"tensor:16x16x16/8 waves", TF, &launch_t4096, 1}};
 
 template <size_t K>
 int set_out(const Variant (&v)[K], const Variant **out) {

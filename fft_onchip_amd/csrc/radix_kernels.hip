@@ -82,8 +82,13 @@ int launch_split(const LaunchArgs &a) {
   return hipGetLastError() == hipSuccess ? FFTOC_SUCCESS : FFTOC_ERROR_LAUNCH;
 }
 
+template <class SP, int S, Form F, class CT = c2>
+constexpr int tile_of() {
+  return S;
+}
+
 #define FFTOC_V(NAME, FAM, ...) \
-  Variant { NAME, FAM, &launch_split<__VA_ARGS__> }
+  Variant { NAME, FAM, &launch_split<__VA_ARGS__>, tile_of<__VA_ARGS__>() }
 
 constexpr unsigned R8 = FAM_RADIX8 | FAM_ANY;
 constexpr unsigned R16 = FAM_RADIX16 | FAM_ANY;
@@ -155,7 +160,8 @@ const Variant kV4096[] = {
 
 // complex64 data (FP32 arithmetic, float2 twiddles rounded from the FP64 table)
 #define FFTOC_V32(NAME, FAM, SPLIT_S_FORM) \
-  Variant { NAME, FAM, &launch_split<FFTOC_UNPAREN SPLIT_S_FORM, f2> }
+  Variant { NAME, FAM, &launch_split<FFTOC_UNPAREN SPLIT_S_FORM, f2>, \
+            tile_of<FFTOC_UNPAREN SPLIT_S_FORM, f2>() }
 #define FFTOC_UNPAREN(...) __VA_ARGS__
 const Variant kW8[] = {FFTOC_V32("radix:4x2/staged/S128", MIX, (Split<4, 2>, 128, Form::Staged)),
                        FFTOC_V32("radix:8/staged/S128", R8, (Split<8>, 128, Form::Staged))};

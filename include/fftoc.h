@@ -73,7 +73,9 @@ typedef enum fftoc_status {
 /* data type: complex64 (float2) input and output with FP32 arithmetic and
  * FP32 twiddles rounded from the FP64 table (default: complex128) */
 #define FFTOC_DATA_C64 0x100u
-/* signals per workgroup override (0 = library default / autotuned choice) */
+/* signals per workgroup (the reference's UPB x FPU): selects the entry
+ * point's kernel variant with that tile (FFTOC_ERROR_UNSUPPORTED if the
+ * length has none; fftoc_variant_tile lists them).  0 = the tuned choice. */
 #define FFTOC_TILE_SHIFT 16
 #define FFTOC_TILE_MASK (0xFFu << FFTOC_TILE_SHIFT)
 #define FFTOC_TILE(s) (((uint32_t)(s) & 0xFFu) << FFTOC_TILE_SHIFT)
@@ -149,6 +151,9 @@ int fftoc_set_variant(int n, int64_t batch, int variant);
 int fftoc_autotune(int n, int64_t batch, int reps, void *stream);
 /* Median time in microseconds of a variant in the last autotune (<0: none). */
 double fftoc_variant_time_us(int n, int variant);
+/* Signals per workgroup of a variant (the FFTOC_TILE value that selects it),
+ * or -1 for an unknown variant. */
+int fftoc_variant_tile(int n, int variant);
 /* Persisted tuning table (text: "<family> <n> <log2 batch> <variant>"). */
 int fftoc_tuning_save(const char *path);
 int fftoc_tuning_load(const char *path);

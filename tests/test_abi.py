@@ -107,3 +107,19 @@ def test_plan_rejects_jobs_writing_into_each_other():
         arr = (J * len(jobs))(*jobs)
         h = ctypes.c_void_p()
         assert fo.lib().fftoc_plan_create(arr, len(jobs), ctypes.byref(h)) == 5
+
+
+def test_tile_flag_selects_a_variant_or_is_unsupported():
+    """FFTOC_TILE(s) names the variant with s signals per workgroup; a tile
+    the length does not offer is FFTOC_ERROR_UNSUPPORTED (checked before any
+    device work), and a tile contradicting a named variant is rejected."""
+    lib = fo.lib()
+    n = 256
+    tiles = [lib.fftoc_variant_tile(n, v) for v in range(lib.fftoc_variant_count(n))]
+    assert all(t >= 1 for t in tiles) and lib.fftoc_variant_tile(n, 999) == -1
+    fake = 1 << 20  # aligned, never dereferenced: the call fails before any launch
+    assert 7 not in tiles
+    assert lib.fftoc_fft(fake, fake, 4, n, fo.make_flags(tile=7), None) == 6
+    v = 0
+    other = next(t for t in tiles if t != tiles[v])
+    assert lib.fftoc_fft_variant(fake, fake, 4, n, fo.make_flags(tile=other), v, None) == 4

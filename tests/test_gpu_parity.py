@@ -130,6 +130,33 @@ def test_every_variant_matches_oracle(n):
             assert err <= TOL, (n, name, inv, err)
 
 
+@pytest.mark.parametrize("n", [64, 256, 4096])
+def test_tile_flag_runs_a_variant_with_that_tile(n):
+    """fftoc_fft / fftoc_tensor_fft with FFTOC_TILE(s) run a variant with s
+    signals per workgroup: every tile the length offers matches the oracle,
+    and the tile of fftoc_fft's own choice reproduces its output exactly."""
+    lib = fo.lib()
+    x = signals(n, batch_for(n), seed=900 + n)
+    src = fo.DeviceBuffer.from_array(x)
+    dst = fo.DeviceBuffer(x.nbytes)
+    ref = oracle.reference(x)
+    for fam, fn in (("radix:", lib.fftoc_fft), ("tensor:", lib.fftoc_tensor_fft)):
+        tiles = sorted({lib.fftoc_variant_tile(n, v) for v, name in _variants(n)
+                        if name.startswith(fam) and lib.fftoc_variant_tile(n, v) <= 255})
+        for tile in tiles:
+            print(f"[tile] n={n} {fam} tile={tile}", flush=True)
+            assert fn(src.ptr, dst.ptr, x.shape[0], n, fo.make_flags(tile=tile), None) == 0
+            fo.synchronize()
+            assert np.max(oracle.rel_l2(dst.download(x.shape), ref, axis=-1)) <= TOL, (fam, tile)
+    chosen = lib.fftoc_get_variant(n, x.shape[0])
+    tile = lib.fftoc_variant_tile(n, chosen)
+    plain = fo.fft_host(x)
+    if tile <= 255:
+        assert lib.fftoc_fft(src.ptr, dst.ptr, x.shape[0], n, fo.make_flags(tile=tile), None) == 0
+        fo.synchronize()
+        assert np.array_equal(dst.download(x.shape), plain)
+
+
 def test_autotune_and_tuning_roundtrip(tmp_path):
     lib = fo.lib()
     n = 256
