@@ -68,8 +68,14 @@ def run_gpu(fo, wl: Workload, steps: int, warmup: int, host_x: np.ndarray | None
     ins = [hip.DeviceBuffer.from_array(x) for _ in range(pairs)]
     outs = [hip.DeviceBuffer(nbytes) for _ in range(pairs)]
     stream = hip.Stream()
-    fn = {"best": fo.fft, "tensor": fo.tensor_fft, "legacy8": fo.legacy_fft,
-          "legacy16": fo.legacy16_fft}[wl.path]
+    if wl.path.startswith("variant:"):  # one named kernel variant
+        vidx = fo.variant_index(n, wl.path[len("variant:"):])
+
+        def fn(x_, b_, n_, out=None, flags=0, stream=None):
+            fo.fft_variant(x_, b_, n_, vidx, out=out, flags=flags, stream=stream)
+    else:
+        fn = {"best": fo.fft, "tensor": fo.tensor_fft, "legacy8": fo.legacy_fft,
+              "legacy16": fo.legacy16_fft}[wl.path]
     # Clock ramp: MI355X needs ~100 ms of sustained load to reach its
     # clocks (longer from idle); launches continue for the ramp time before
     # the W warm-up steps.
@@ -513,6 +519,9 @@ def main():
     extra = {}
     if not args.quick:
         others = [Workload("fp64_n4096_b16384_tensor", 4096, 16384, 7, "normal", "tensor"),
+                  # the tensor path with all three DFT16 stages on the matrix cores
+                  Workload("fp64_n4096_b16384_tensor_3mfma", 4096, 16384, 7, "normal",
+                           "variant:tensor:16x16x16/8 waves"),
                   Workload("fp64_n4096_b16384_legacy16", 4096, 16384, 7, "normal", "legacy16"),
                   Workload("fp64_n4096_b16384_legacy8", 4096, 16384, 7, "normal", "legacy8")]
         for path in ("best", "tensor", "legacy8"):

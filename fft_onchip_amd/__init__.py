@@ -126,6 +126,25 @@ def fft(x, batch: int, n: int, out=None, flags: int = 0, stream=None) -> None:
     _call("fftoc_fft", x, out, batch, n, flags, stream)
 
 
+def variant_index(n: int, name: str) -> int:
+    """Index of the kernel variant called `name` for length n."""
+    for v in range(lib().fftoc_variant_count(int(n))):
+        if lib().fftoc_variant_name(int(n), v).decode() == name:
+            return v
+    raise KeyError(f"no variant {name!r} for n={n}")
+
+
+def fft_variant(x, batch: int, n: int, variant: int, out=None, flags: int = 0,
+                stream=None) -> None:
+    """One named kernel variant (fftoc_fft_variant)."""
+    ip = _ptr(x)
+    op = _ptr(out) if out is not None else ip
+    rc = lib().fftoc_fft_variant(ip, op, int(batch), int(n), int(flags), int(variant),
+                                 _stream(stream))
+    if rc != 0:
+        raise FftocError(rc, "fftoc_fft_variant")
+
+
 _PATHS = {"tensor": tensor_fft, "legacy8": legacy_fft, "legacy16": legacy16_fft, "best": fft}
 
 

@@ -603,6 +603,63 @@ __global__ __launch_bounds__(64 * k4096Waves) void tensor4096_kernel(const c2 *i
   }
 }
 
+// N = 4096 with stage A on the vector ALUs (threads 0..255; thread t owns
+// column m = t: a radix-16 butterfly over n1 in registers, twiddles
+// W_4096^{m k1}) and stage B, the 256-point core of each row, on the matrix
+// cores (wave w of WAVES: rows (16/WAVES) w + q).  Two thirds of the MFMA work of
+// tensor4096_kernel; the VALU stage runs while other waves use the matrix
+// cores.  Each wave writes its stage-B results X[k1 + 16 k2] back into the
+// LDS row k1 it alone read (k2 permuted within the row against bank
+// conflicts), so only two barriers are needed.
+__device__ __forceinline__ int swz_row(int k1, int k2) {
+  return k1 * 256 + (k2 ^ (((k2 >> 4) ^ k1) & 15));
+}
+
+template <int WAVES, bool INV, bool SC>
+__global__ __launch_bounds__(64 * WAVES) void tensor4096h_kernel(const c2 *in, c2 *out,
+                                                                 int64_t batch,
+                                                                 const c2 *__restrict__ tw,
+                                                                 double scale) {
+  constexpr int kRowsPerWave = 16 / WAVES;
+  __shared__ c2 lds[4096];
+  const int t = threadIdx.x;
+  const int lane = lane_id(), g = lane >> 4, j = lane & 15;
+  const int w = t >> 6;
+  const int64_t s = blockIdx.x;
+  if (s >= batch) return;  // uniform over the workgroup
+  const c2 *x = in + s * 4096;
+  if (t < 256) {
+    // ---- stage A (VALU): column m = t
+    c2 v[16];
+#pragma unroll
+    for (int n1 = 0; n1 < 16; ++n1) v[n1] = ld_in<INV>(x + 256 * n1 + t);
+    dft<16>(v);
+    apply_twiddles<16>(v, tw, t);  // v[k1] *= W_4096^{t k1}
+#pragma unroll
+    for (int k1 = 0; k1 < 16; ++k1) lds[swz_z(k1, t)] = v[k1];
+  }
+  __syncthreads();
+  // ---- stage B (MFMA): rows k1 = kRowsPerWave w + q, results back into the same row
+  const Frag f = dft16_frag(tw, lane);
+#pragma unroll
+  for (int q = 0; q < kRowsPerWave; ++q) {
+    const int k1 = kRowsPerWave * w + q;
+    c2 a[4], z[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) a[c] = lds[swz_z(k1, 64 * c + lane)];
+    core256(a, f, tw, lane, z);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) lds[swz_row(k1, 4 * r + g + 16 * j)] = z[r];
+  }
+  __syncthreads();
+  c2 *y = out + s * 4096;
+#pragma unroll
+  for (int c = 0; c < 4096 / (64 * WAVES); ++c) {
+    const int e = 64 * WAVES * c + t;
+    st_out<INV, SC>(y + e, lds[swz_row(e & 15, e >> 4)], scale);
+  }
+}
+
 int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
 // Persistent launch of a one-signal-per-wave kernel: one wave per signal up
@@ -682,6 +739,7 @@ FFTOC_TENSOR_LAUNCHER(launch_t256g4, tensor256_kernel, 4 * kWaves, 64 * kWaves, 
 FFTOC_TENSOR_LAUNCHER(launch_t512, tensor512_kernel, kWaves, 64 * kWaves, 1)
 FFTOC_TENSOR_LAUNCHER(launch_t512g2, tensor512_kernel, 2 * kWaves, 64 * kWaves, 2)
 FFTOC_TENSOR_LAUNCHER(launch_t4096, tensor4096_kernel, 1, 64 * k4096Waves)
+FFTOC_TENSOR_LAUNCHER(launch_t4096h, tensor4096h_kernel, 1, 256, 4)
 #undef FFTOC_TENSOR_LAUNCHER
 
 constexpr unsigned TF = FAM_TENSOR | FAM_ANY;
@@ -708,7 +766,10 @@ const Variant kT256[] = {{"tensor:16x16/4 per wave", TF, &launch_t256g4, 16},
 const Variant kT512[] = {{"tensor:16x2x16/1 per wave/persistent", TF, &launch_t512p, 4},
                          {"tensor:16x2x16/1 per wave", TF, &launch_t512, 4},
                          {"tensor:16x2x16/2 per wave", TF, &launch_t512g2, 8}};
-const Variant kT4096[] = {{"tensor:16x16x16/8 waves", TF, &launch_t4096, 1}};
+// The hybrid form leads: 0.64 of HBM against 0.51 for three MFMA stages
+// (bound by the matrix cores plus the vector work between them).
+const Variant kT4096[] = {{"tensor:16x16x16/VALU stage A, 4 waves", TF, &launch_t4096h, 1},
+                          {"tensor:16x16x16/8 waves", TF, &launch_t4096, 1}};
 
 template <size_t K>
 int set_out(const Variant (&v)[K], const Variant **out) {
