@@ -483,6 +483,8 @@ def main():
                     help="time one workload (path best|tensor|legacy8|legacy16, length N at its "
                          "benchmark batch, precision fp64|fp32|bf16x3) and print its result; "
                          "for counter runs")
+    ap.add_argument("--variant", metavar="NAME",
+                    help="with --only: run this named kernel variant of the length")
     ap.add_argument("--traffic", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="write profiles/traffic.json from two rocprofv3 --pmc runs and exit")
     args = ap.parse_args()
@@ -507,6 +509,8 @@ def main():
         prec = parts[2] if len(parts) > 2 else "fp64"
         flag = {"fp64": 0, "fp32": fo.PREC_FP32, "bf16x3": fo.PREC_BF16X3}[prec]
         batch = {4096: 16384, 64: 65536}.get(n, (1 << 24) // n)
+        if args.variant:
+            path = "variant:" + args.variant
         wl = Workload(f"only_{path}_n{n}_{prec}", n, batch, 7, "normal", path, flag)
         r, _, _ = run_gpu(fo, wl, args.steps, args.warmup)
         print(json.dumps({k: (round(v, 6) if isinstance(v, float) else v) for k, v in r.items()}))
