@@ -5,6 +5,9 @@
 // Data stay complex128 in HBM; they are rounded to FP32 on load, the
 // inter-stage twiddles are the FP64 table rounded to FP32, results are
 // widened to FP64 on store and the 1/N normalisation is applied in FP64.
+// At N = 4096 the first of the three 16-point stages is an FP64 butterfly on
+// the vector ALUs (rounded to FP32 into LDS) and the two GEMM stages of each
+// row's 256-point core run in the chosen precision.
 //
 // Operand layouts on gfx950 (wave64, lane l, g = l/16, j = l%16), measured:
 //  f32 16x16x4:   A[i][k] = lane(i=j, k=g); B[k][c] = lane(k=g, c=j);
@@ -276,65 +279,64 @@ __global__ __launch_bounds__(64 * kWaves) void tensor256p_kernel(const c2 *in, c
   }
 }
 
-// N = 4096 = 16 x 256, one signal per 512-thread workgroup (see the FP64
-// form); the LDS transposes hold FP32 values (32 KiB).
-constexpr int k4096Waves = 8;
+// LDS element of row k1, column m for the N = 4096 transpose (XOR swizzle).
 __device__ __forceinline__ int swz_z(int k1, int m) { return k1 * 256 + (m ^ k1); }
-__device__ __forceinline__ int swz_x(int e) { return e ^ ((e >> 8) & 15); }
+
+// N = 4096 with stage A as an FP64 radix-16 butterfly on the vector ALUs
+// (thread t of 256 owns column m = t; twiddles W_4096^{m k1} from the FP64
+// table; the result is rounded to FP32 into LDS) and stage B, the two DFT16
+// GEMM stages of each row's 256-point core, in precision P on the matrix
+// cores (wave w: rows 4w .. 4w + 3), as the FP64 default does.  Each wave
+// writes its results X[k1 + 16 k2] back into the LDS row k1 it alone read (k2
+// permuted within the row), so two barriers suffice.
+__device__ __forceinline__ int swz_row(int k1, int k2) {
+  return k1 * 256 + (k2 ^ (((k2 >> 4) ^ k1) & 15));
+}
 
 template <int P, bool INV>
-__global__ __launch_bounds__(64 * k4096Waves) void tensor4096p_kernel(const c2 *in, c2 *out,
-                                                                      int64_t batch,
-                                                                      const c2 *__restrict__ tw,
-                                                                      const f2 *__restrict__ tw32,
-                                                                      double scale) {
+__global__ __launch_bounds__(256) void tensor4096ph_kernel(const c2 *in, c2 *out, int64_t batch,
+                                                           const c2 *__restrict__ tw,
+                                                           const f2 *__restrict__ tw32,
+                                                           double scale) {
   __shared__ f2 lds[4096];
+  const int t = threadIdx.x;
   const int lane = lane_id(), g = lane >> 4, j = lane & 15;
-  const int w = threadIdx.x >> 6;
+  const int w = t >> 6;
   const int64_t s = blockIdx.x;
   if (s >= batch) return;  // uniform over the workgroup
   const c2 *x = in + s * 4096;
-  const auto f = Stage<P>::frag(tw, lane);
-  // stage A: rows m = m0 + j, K = n1 = 4g + c
+  {
+    c2 v[16];
 #pragma unroll
-  for (int q = 0; q < 2; ++q) {
-    const int m0 = 16 * (2 * w + q);
-    f2 a[4];
-#pragma unroll
-    for (int c = 0; c < 4; ++c) a[c] = ld_in<INV>(x + 256 * (4 * g + c) + m0 + j);
-    v4f yr, yi;
-    Stage<P>::gemm(a, f, yr, yi);
-    // D register r: m = m0 + 4g + r, k1 = j
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int m = m0 + 4 * g + r;
-      lds[swz_z(j, m)] = cmul(make_float2(yr[r], yi[r]), tw32[m * j]);
+    for (int n1 = 0; n1 < 16; ++n1) {
+      c2 e = ld_stream(x + 256 * n1 + t);
+      if constexpr (INV) e = cswap(e);
+      v[n1] = e;
     }
+    dft<16>(v);
+    apply_twiddles<16>(v, tw, t);  // v[k1] *= W_4096^{t k1}
+#pragma unroll
+    for (int k1 = 0; k1 < 16; ++k1) lds[swz_z(k1, t)] = make_float2((float)v[k1].x, (float)v[k1].y);
   }
   __syncthreads();
-  // stage B: rows k1 = 2w + q, a 256-point core over m
-  f2 z[2][4];
+  const auto f = Stage<P>::frag(tw, lane);
 #pragma unroll
-  for (int q = 0; q < 2; ++q) {
-    const int k1 = 2 * w + q;
-    f2 a[4];
+  for (int q = 0; q < 4; ++q) {
+    const int k1 = 4 * w + q;
+    f2 a[4], z[4];
 #pragma unroll
     for (int c = 0; c < 4; ++c) a[c] = lds[swz_z(k1, 64 * g + 16 * c + j)];
-    core256p<P>(a, f, tw32, lane, z[q]);
-  }
-  __syncthreads();
+    core256p<P>(a, f, tw32, lane, z);
+    // z[r] = X[k1 + 16 k2], k2 = 4g + r + 16j
 #pragma unroll
-  for (int q = 0; q < 2; ++q) {
-    const int k1 = 2 * w + q;
-#pragma unroll
-    for (int r = 0; r < 4; ++r) lds[swz_x(k1 + 16 * (4 * g + r + 16 * j))] = z[q][r];
+    for (int r = 0; r < 4; ++r) lds[swz_row(k1, 4 * g + r + 16 * j)] = z[r];
   }
   __syncthreads();
   c2 *y = out + s * 4096;
 #pragma unroll
-  for (int c = 0; c < 8; ++c) {
-    const int e = 512 * c + threadIdx.x;
-    st_out<INV>(y + e, lds[swz_x(e)], scale);
+  for (int c = 0; c < 16; ++c) {
+    const int e = 256 * c + t;
+    st_out<INV>(y + e, lds[swz_row(e & 15, e >> 4)], scale);
   }
 }
 
@@ -351,7 +353,7 @@ int launch_p(int n, const LaunchArgs &a) {
   switch (n) {
     case 64: grid = ceil_div(a.batch, 4 * kG64 * kWaves); block = 64 * kWaves; break;
     case 256: grid = ceil_div(a.batch, kG256 * kWaves); block = 64 * kWaves; break;
-    case 4096: grid = a.batch; block = 64 * k4096Waves; break;
+    case 4096: grid = a.batch; block = 256; break;
     default: return FFTOC_ERROR_UNSUPPORTED;
   }
   if (grid > 0x7fffffffLL) return FFTOC_ERROR_INVALID_BATCH;
@@ -365,7 +367,7 @@ int launch_p(int n, const LaunchArgs &a) {
   } while (0)
   if (n == 64) FFTOC_P_LAUNCH(tensor64p_kernel);
   else if (n == 256) FFTOC_P_LAUNCH(tensor256p_kernel);
-  else FFTOC_P_LAUNCH(tensor4096p_kernel);
+  else FFTOC_P_LAUNCH(tensor4096ph_kernel);
 #undef FFTOC_P_LAUNCH
   return hipGetLastError() == hipSuccess ? FFTOC_SUCCESS : FFTOC_ERROR_LAUNCH;
 }

@@ -64,8 +64,9 @@ typedef enum fftoc_status {
 #define FFTOC_NORMALIZE 0x2u      /* multiply the result by 1/N (exact)      */
 /* arithmetic of the DFT stages of fftoc_tensor_fft (n = 64, 256, 4096 for
  * the two split forms): data stay complex128 in memory, are rounded to FP32
- * for the stages, twiddles are the FP64 table rounded to FP32, results are
- * widened to FP64 and FFTOC_NORMALIZE is applied in FP64. */
+ * for the GEMM stages, twiddles are the FP64 table rounded to FP32, results
+ * are widened to FP64 and FFTOC_NORMALIZE is applied in FP64.  At n = 4096
+ * the first 16-point stage is an FP64 butterfly (as in the FP64 default). */
 #define FFTOC_PREC_FP64 0x00u     /* FP64 everywhere (default)               */
 #define FFTOC_PREC_FP32 0x10u     /* FP32 MFMA stage GEMMs, FP32 accumulate  */
 #define FFTOC_PREC_BF16X3 0x20u   /* 3-term split-BF16 MFMA, FP32 accumulate */
