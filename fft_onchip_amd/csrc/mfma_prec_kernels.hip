@@ -22,6 +22,8 @@
 // 3-term split BF16: x = x0 + x1 + x2 with x0 = bf16(x), x1 = bf16(x - x0),
 // x2 = bf16(x - x0 - x1) (24 significant bits); a product keeps the six terms
 // a_p b_q with p + q <= 2, smallest first, accumulated in FP32.
+#include <algorithm>
+
 #include "fftoc_device.hpp"
 #include "fftoc_internal.hpp"
 
@@ -180,30 +182,21 @@ constexpr int kG256 = 4;
 // n2 = r, k1 = j; twiddle W_64^{n2 k1} and DFT4 over r in FP32, output
 // X_g[j + 16 k2].
 template <int P, bool INV>
-__global__ __launch_bounds__(64 * kWaves) void tensor64p_kernel(const c2 *in, c2 *out,
-                                                                int64_t batch,
-                                                                const c2 *__restrict__ tw,
-                                                                const f2 *__restrict__ tw32,
-                                                                double scale) {
-  const int lane = lane_id(), g = lane >> 4, j = lane & 15;
-  const int64_t sw = wave_global() * (4 * kG64);
-  if (sw >= batch) return;
-  const auto f = Stage<P>::template frag<true>(tw, lane);
-  f2 au[kG64][4];
+__device__ __forceinline__ void load64p(const c2 *in, int64_t s0, int64_t batch, int g, int j,
+                                        f2 (&a)[4]) {
+  const int64_t s = s0 + (j >> 2);
+  const int n2 = j & 3;
 #pragma unroll
-  for (int u = 0; u < kG64; ++u) {
-    const int64_t s = sw + 4 * u + (j >> 2);
-    const int n2 = j & 3;
-#pragma unroll
-    for (int c = 0; c < 4; ++c)
-      au[u][c] =
-          (s < batch) ? ld_in<INV>(in + s * 64 + 4 * (4 * c + g) + n2) : make_float2(0.f, 0.f);
-  }
-#pragma unroll
-  for (int u = 0; u < kG64; ++u) {
-  const int64_t s0 = sw + 4 * u;
+  for (int c = 0; c < 4; ++c)
+    a[c] = (s < batch) ? ld_in<INV>(in + s * 64 + 4 * (4 * c + g) + n2) : make_float2(0.f, 0.f);
+}
+
+template <int P, bool INV>
+__device__ __forceinline__ void unit64p(const f2 (&a)[4], const typename Stage<P>::Frag &f,
+                                        const f2 *__restrict__ tw32, c2 *out, int64_t s0,
+                                        int64_t batch, double scale, int g, int j) {
   v4f yr, yi;
-  Stage<P>::gemm(au[u], f, yr, yi);
+  Stage<P>::gemm(a, f, yr, yi);
   f2 y[4];
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
@@ -217,6 +210,43 @@ __global__ __launch_bounds__(64 * kWaves) void tensor64p_kernel(const c2 *in, c2
 #pragma unroll
     for (int k2 = 0; k2 < 4; ++k2) st_out<INV>(out + s * 64 + j + 16 * k2, y[k2], scale);
   }
+}
+
+// Persistent: wave w handles groups of kG64 units w, w + (waves in the
+// grid), ...; the next group's loads are issued before the current group is
+// transformed (as the FP64 N = 64 kernel).
+template <int P, bool INV>
+__global__ __launch_bounds__(64 * kWaves) void tensor64p_kernel(const c2 *in, c2 *out,
+                                                                int64_t batch,
+                                                                const c2 *__restrict__ tw,
+                                                                const f2 *__restrict__ tw32,
+                                                                double scale) {
+  constexpr int SPG = 4 * kG64;  // signals per group
+  const int lane = lane_id(), g = lane >> 4, j = lane & 15;
+  const int64_t groups = (batch + SPG - 1) / SPG;
+  const int64_t stride = (int64_t)gridDim.x * kWaves;
+  int64_t q = wave_global();
+  if (q >= groups) return;
+  const auto f = Stage<P>::template frag<true>(tw, lane);
+  f2 cur[kG64][4];
+#pragma unroll
+  for (int u = 0; u < kG64; ++u) load64p<P, INV>(in, q * SPG + 4 * u, batch, g, j, cur[u]);
+  for (;;) {
+    const int64_t qn = q + stride;
+    f2 nxt[kG64][4];
+    if (qn < groups) {
+#pragma unroll
+      for (int u = 0; u < kG64; ++u) load64p<P, INV>(in, qn * SPG + 4 * u, batch, g, j, nxt[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < kG64; ++u)
+      unit64p<P, INV>(cur[u], f, tw32, out, q * SPG + 4 * u, batch, scale, g, j);
+    if (qn >= groups) break;
+#pragma unroll
+    for (int u = 0; u < kG64; ++u)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) cur[u][c] = nxt[u][c];
+    q = qn;
   }
 }
 
@@ -351,7 +381,12 @@ int launch_p(int n, const LaunchArgs &a) {
   int64_t grid;
   int block;
   switch (n) {
-    case 64: grid = ceil_div(a.batch, 4 * kG64 * kWaves); block = 64 * kWaves; break;
+    case 64: {  // persistent: up to the resident workgroups
+      const int64_t resident = resident_workgroups<tensor64p_kernel<P, false>, 64 * kWaves>();
+      grid = std::min<int64_t>(ceil_div(a.batch, 4 * kG64 * kWaves), resident);
+      block = 64 * kWaves;
+      break;
+    }
     case 256: grid = ceil_div(a.batch, kG256 * kWaves); block = 64 * kWaves; break;
     case 4096: grid = a.batch; block = 256; break;
     default: return FFTOC_ERROR_UNSUPPORTED;
