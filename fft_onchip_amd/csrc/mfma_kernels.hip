@@ -284,34 +284,53 @@ __global__ __launch_bounds__(64 * kWaves) void tensor16xm_kernel(const c2 *in, c
   for (int u = 0; u < G; ++u) unit16xm<M, INV, SC>(au[u], f, tw, out, sw + SPW * u, batch, scale, g, j);
 }
 
-// Persistent form: wave w handles unit groups w, w + (waves in the grid), ...
-// of G units each; the next group's loads are issued before the current
-// group is transformed, so they are in flight during its matrix products and
-// stores.
-template <int M, int G, bool INV, bool SC>
-__global__ __launch_bounds__(64 * kWaves) void tensor16xm_persistent_kernel(
+// A unit of a small-N kernel as a type, for the persistent form below:
+// kSignals signals per unit, frag() the B operand, load() the A operand of
+// the unit starting at signal s0, run() its transform and stores.  (Measured
+// persistent forms of N = 8, 16, 128 and 256 were no faster than their
+// one-pass kernels; N = 64 below 2^17 signals gains 6 %.)
+template <int M>
+struct Unit16xM {
+  static constexpr int kSignals = 16 / M;
+  __device__ static Frag frag(const c2 *__restrict__ tw, int lane) { return dft16_frag(tw, lane); }
+  template <bool INV>
+  __device__ static void load(const c2 *in, int64_t s0, int64_t batch, int g, int j, c2 (&a)[4]) {
+    load16xm<M, INV>(in, s0, batch, g, j, a);
+  }
+  template <bool INV, bool SC>
+  __device__ static void run(const c2 (&a)[4], const Frag &f, const c2 *__restrict__ tw, c2 *out,
+                             int64_t s0, int64_t batch, double scale, int g, int j) {
+    unit16xm<M, INV, SC>(a, f, tw, out, s0, batch, scale, g, j);
+  }
+};
+
+// Persistent form: wave w handles groups of G units w, w + (waves in the
+// grid), ...; the next group's loads are issued before the current group is
+// transformed, so they are in flight during its matrix products and stores.
+template <class U, int G, bool INV, bool SC>
+__global__ __launch_bounds__(64 * kWaves) void tensor_persistent_kernel(
     const c2 *in, c2 *out, int64_t batch, const c2 *__restrict__ tw, double scale) {
-  constexpr int SPG = (16 / M) * G;  // signals per group
+  constexpr int SPG = U::kSignals * G;  // signals per group
   const int lane = lane_id(), g = lane >> 4, j = lane & 15;
   const int64_t groups = (batch + SPG - 1) / SPG;
   const int64_t stride = (int64_t)gridDim.x * kWaves;
   int64_t q = wave_global();
   if (q >= groups) return;
-  const Frag f = dft16_frag(tw, lane);
+  const Frag f = U::frag(tw, lane);
   c2 cur[G][4];
 #pragma unroll
-  for (int u = 0; u < G; ++u) load16xm<M, INV>(in, q * SPG + (16 / M) * u, batch, g, j, cur[u]);
+  for (int u = 0; u < G; ++u) U::template load<INV>(in, q * SPG + U::kSignals * u, batch, g, j, cur[u]);
   for (;;) {
     const int64_t qn = q + stride;
     c2 nxt[G][4];
     if (qn < groups) {
 #pragma unroll
       for (int u = 0; u < G; ++u)
-        load16xm<M, INV>(in, qn * SPG + (16 / M) * u, batch, g, j, nxt[u]);
+        U::template load<INV>(in, qn * SPG + U::kSignals * u, batch, g, j, nxt[u]);
     }
 #pragma unroll
     for (int u = 0; u < G; ++u)
-      unit16xm<M, INV, SC>(cur[u], f, tw, out, q * SPG + (16 / M) * u, batch, scale, g, j);
+      U::template run<INV, SC>(cur[u], f, tw, out, q * SPG + U::kSignals * u, batch, scale, g, j);
     if (qn >= groups) break;
 #pragma unroll
     for (int u = 0; u < G; ++u)
@@ -321,14 +340,14 @@ __global__ __launch_bounds__(64 * kWaves) void tensor16xm_persistent_kernel(
   }
 }
 
-template <int M, int G>
-int launch_t16xm_persistent(const LaunchArgs &a) {
+template <class U, int G>
+int launch_tensor_persistent(const LaunchArgs &a) {
   if (a.precision != FFTOC_PREC_FP64) return FFTOC_ERROR_UNSUPPORTED;
   if (a.batch <= 0) return FFTOC_SUCCESS;
-  constexpr int SPG = (16 / M) * G;
+  constexpr int SPG = U::kSignals * G;
   const int64_t groups = (a.batch + SPG - 1) / SPG;
   const int64_t resident =
-      resident_workgroups<tensor16xm_persistent_kernel<M, G, false, false>, 64 * kWaves>();
+      resident_workgroups<tensor_persistent_kernel<U, G, false, false>, 64 * kWaves>();
   const int64_t need = (groups + kWaves - 1) / kWaves;
   const dim3 gd((unsigned)(need < resident ? need : resident)), bd(64 * kWaves);
   const c2 *in = static_cast<const c2 *>(a.in);
@@ -336,16 +355,16 @@ int launch_t16xm_persistent(const LaunchArgs &a) {
   const c2 *tw = static_cast<const c2 *>(a.twiddles);
   const bool sc = a.scale != 1.0;
   if (a.inverse && sc)
-    hipLaunchKernelGGL((tensor16xm_persistent_kernel<M, G, true, true>), gd, bd, 0, a.stream, in,
-                       out, a.batch, tw, a.scale);
+    hipLaunchKernelGGL((tensor_persistent_kernel<U, G, true, true>), gd, bd, 0, a.stream, in, out,
+                       a.batch, tw, a.scale);
   else if (a.inverse)
-    hipLaunchKernelGGL((tensor16xm_persistent_kernel<M, G, true, false>), gd, bd, 0, a.stream, in,
+    hipLaunchKernelGGL((tensor_persistent_kernel<U, G, true, false>), gd, bd, 0, a.stream, in,
                        out, a.batch, tw, a.scale);
   else if (sc)
-    hipLaunchKernelGGL((tensor16xm_persistent_kernel<M, G, false, true>), gd, bd, 0, a.stream, in,
+    hipLaunchKernelGGL((tensor_persistent_kernel<U, G, false, true>), gd, bd, 0, a.stream, in,
                        out, a.batch, tw, a.scale);
   else
-    hipLaunchKernelGGL((tensor16xm_persistent_kernel<M, G, false, false>), gd, bd, 0, a.stream, in,
+    hipLaunchKernelGGL((tensor_persistent_kernel<U, G, false, false>), gd, bd, 0, a.stream, in,
                        out, a.batch, tw, a.scale);
   return hipGetLastError() == hipSuccess ? FFTOC_SUCCESS : FFTOC_ERROR_LAUNCH;
 }
@@ -754,7 +773,7 @@ const Variant kT32[] = {{"tensor:16x2/8 per wave", TF, &launch_t32, 32},
                         {"tensor:16x2/16 per wave", TF, &launch_t32g2, 64},
                         {"tensor:16x2/32 per wave", TF, &launch_t32g4, 128}};
 const Variant kT64[] = {{"tensor:16x4/8 per wave", TF, &launch_t64g2, 32},
-                        {"tensor:16x4/8 per wave/persistent", TF, &launch_t16xm_persistent<4, 2>, 32},
+                        {"tensor:16x4/8 per wave/persistent", TF, &launch_tensor_persistent<Unit16xM<4>, 2>, 32},
                         {"tensor:16x4/4 per wave", TF, &launch_t64, 16},
                         {"tensor:16x4/16 per wave", TF, &launch_t64g4, 64}};
 const Variant kT128[] = {{"tensor:16x8/4 per wave", TF, &launch_t128g2, 16},
