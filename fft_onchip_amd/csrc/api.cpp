@@ -352,6 +352,10 @@ const char *kFamilyNames[4] = {"tensor", "legacy8", "legacy16", "fft"};
 
 }  // namespace
 
+std::atomic<int> g_table_holders{0};
+void hold_tables() { g_table_holders.fetch_add(1); }
+void drop_tables() { g_table_holders.fetch_sub(1); }
+
 int device_tables(const void **tw64, const void **tw32) {
   int dev = 0;
   const int rc = init_current_device(&dev);
@@ -384,6 +388,8 @@ int fftoc_twiddle_table(const void **fp64, const void **fp32) {
 
 int fftoc_shutdown(void) {
   std::lock_guard<std::mutex> lock(fftoc::g_mutex);
+  // a plan's or server's kernels read the tables
+  if (fftoc::g_table_holders.load() > 0) return FFTOC_ERROR_UNSUPPORTED;
   int cur = 0;
   (void)hipGetDevice(&cur);
   for (int d = 0; d < fftoc::kMaxDevices; ++d) {

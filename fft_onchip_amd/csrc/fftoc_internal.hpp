@@ -46,6 +46,10 @@ int launch_tensor_precision(int n, const LaunchArgs &a);
 
 // Per-device tables shared with the plan / serving code (api.cpp).
 int device_tables(const void **tw64, const void **tw32);
+// Plans and servers hold the tables while they exist: fftoc_shutdown refuses
+// to free them until every holder is destroyed.
+void hold_tables();
+void drop_tables();
 
 // A kernel variant for one transform length: name, the entry-point families
 // it may serve, and its launcher.

@@ -25,6 +25,7 @@ struct fftoc_plan_t {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   const void *tw64 = nullptr, *tw32 = nullptr;
+  bool holds_tables = false;
 };
 
 namespace fftoc {
@@ -193,6 +194,8 @@ int fftoc_plan_create(const fftoc_job *jobs, int njobs, fftoc_plan *out) {
     fftoc_plan_destroy(p);
     return rc;
   }
+  hold_tables();
+  p->holds_tables = true;
   *out = p;
   return FFTOC_SUCCESS;
 }
@@ -312,6 +315,7 @@ int fftoc_plan_destroy(fftoc_plan p) {
   if (p->d_jobs) (void)hipFree(p->d_jobs);
   if (p->d_ctr) (void)hipFree(p->d_ctr);
   (void)hipSetDevice(cur);
+  if (p->holds_tables) fftoc::drop_tables();
   delete p;
   return FFTOC_SUCCESS;
 }

@@ -116,6 +116,7 @@ struct fftoc_server_t {
   bool running = false;
   int routine[13][2];
   const void *tw64 = nullptr, *tw32 = nullptr;
+  bool holds_tables = false;
 };
 
 namespace fftoc {
@@ -139,6 +140,7 @@ void free_server(fftoc_server_t *s) {
   if (s->comps) (void)hipHostFree(s->comps);
   if (s->live) (void)hipHostFree(s->live);
   if (s->stream) (void)hipStreamDestroy(s->stream);
+  if (s->holds_tables) drop_tables();
   delete s;
 }
 
@@ -277,6 +279,8 @@ int fftoc_server_create(size_t hbm_budget_bytes, int max_inflight, fftoc_server 
     return rc;
   }
   s->running = true;
+  hold_tables();
+  s->holds_tables = true;
   *out = s;
   return FFTOC_SUCCESS;
 }

@@ -85,7 +85,10 @@ typedef enum fftoc_status {
 /* Builds the FP64 twiddle / DFT-matrix tables on the current device.
  * Optional: every entry point initialises its device on first use. */
 int fftoc_init(void);
-/* Frees the tables of every device.  Not thread-safe against running calls. */
+/* Frees the tables of every device.  Not thread-safe against running calls.
+ * FFTOC_ERROR_UNSUPPORTED while plans or servers exist (their kernels read the
+ * tables): destroy them first.  Tables from fftoc_twiddle_table are invalid
+ * afterwards. */
 int fftoc_shutdown(void);
 /* Static description of a status code (never NULL). */
 const char *fftoc_strerror(int status);

@@ -143,3 +143,22 @@ def test_plan_rejects_bad_jobs():
     empty.execute(graph=True)
     fo.synchronize()
     empty.close()
+
+
+def test_shutdown_waits_for_plans_and_servers():
+    """fftoc_shutdown refuses while a plan or a server holds the device tables
+    (their kernels read them), succeeds once they are gone, and the library
+    re-initialises on the next call."""
+    import gc
+    gc.collect()
+    x = np.zeros((4, 64), dtype=np.complex128)
+    buf = fo.DeviceBuffer.from_array(x)
+    plan = fo.Plan([(buf, buf, 4, 64, 0)])
+    assert fo.lib().fftoc_shutdown() == 6  # FFTOC_ERROR_UNSUPPORTED
+    plan.close()
+    with fo.Server(1 << 20, 4):
+        assert fo.lib().fftoc_shutdown() == 6
+    assert fo.lib().fftoc_shutdown() == 0
+    y = np.random.default_rng(1).standard_normal((8, 256)) + 0j
+    got = fo.fft_host(y)
+    assert np.max(oracle.rel_l2(got, oracle.reference(y), axis=-1)) <= 1e-13
