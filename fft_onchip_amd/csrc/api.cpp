@@ -131,10 +131,12 @@ int batch_bucket(int64_t batch) {
 struct SizeRegistry {
   std::vector<Variant> v;
   std::atomic<int> choice[4][kBuckets];  // [Family][batch bucket] -> index into v
-  std::vector<double> tuned_us;  // last autotune time per variant (<0: not run)
+  std::vector<double> tuned_us;  // last autotune time per variant (<0: not run),
+                                 // guarded by g_tune_mutex
 };
 
 SizeRegistry g_reg[13][2];  // [log2 n][complex64 data]
+std::mutex g_tune_mutex;
 std::once_flag g_reg_once;
 
 // Default choice of fftoc_fft per length, from MI355X measurements of the
@@ -336,7 +338,10 @@ int autotune(int n, int64_t batch, int reps, void *stream_p) {
   (void)hipFree(in);
   (void)hipFree(out);
   if (rc != FFTOC_SUCCESS) return rc;
-  reg->tuned_us = us;
+  {
+    std::lock_guard<std::mutex> lock(g_tune_mutex);
+    reg->tuned_us = us;
+  }
   const Family fams[4] = {Family::Tensor, Family::Radix8, Family::Radix16, Family::Best};
   for (Family f : fams) {
     int best = -1;
@@ -502,6 +507,7 @@ int fftoc_autotune(int n, int64_t batch, int reps, void *stream) {
 double fftoc_variant_time_us(int n, int variant) {
   fftoc::SizeRegistry *r = fftoc::registry(n);
   if (!r || variant < 0 || variant >= (int)r->v.size()) return -1.0;
+  std::lock_guard<std::mutex> lock(fftoc::g_tune_mutex);
   return r->tuned_us[variant];
 }
 
