@@ -79,6 +79,7 @@ int launch_server(const ServerDev &sd, int grid, hipStream_t stream);
 int server_grid(int max_per_cu);
 
 int batch_routine_count();
+int batch_routine_default(int n, bool c64);  // untuned routine of a length / data type
 int batch_routine_n(int id);
 const char *batch_routine_name(int id);
 int batch_routine_signals(int id);

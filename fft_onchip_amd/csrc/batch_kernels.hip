@@ -17,8 +17,9 @@ namespace {
 constexpr int kBatchThreads = 256;
 constexpr int kLdsElems = lds_elems(4096);
 
-// Routine table: id -> (n, split, signals per tile, staged).  Two candidates
-// per length; the plan autotuner picks between them.
+// Routine table: id -> (n, split, signals per tile, staged).  Two or three
+// candidates per length; the plan autotuner picks among them, and
+// batch_routine_default gives the untuned choice per data type.
 #define FFTOC_ROUTINES(X)                              \
   X(0, 8, "4x2/staged/S128", (Split<4, 2>), 128, true)  \
   X(1, 8, "8/staged/S256", (Split<8>), 256, true)        \
@@ -38,7 +39,8 @@ constexpr int kLdsElems = lds_elems(4096);
   X(15, 1024, "8x8x16/tile/S4", (Split<8, 8, 16>), 4, false) \
   X(16, 2048, "16x16x8/tile/S2", (Split<16, 16, 8>), 2, false) \
   X(17, 2048, "8x16x16/tile/S2", (Split<8, 16, 16>), 2, false) \
-  X(18, 4096, "16x16x16/tile/S1", (Split<16, 16, 16>), 1, false)
+  X(18, 4096, "16x16x16/tile/S1", (Split<16, 16, 16>), 1, false) \
+  X(19, 128, "8x4x4/tile/S16", (Split<8, 4, 4>), 16, false)
 
 #define FFTOC_UNPAREN(...) __VA_ARGS__
 
@@ -442,6 +444,13 @@ int launch_server(const ServerDev &sd, int grid, hipStream_t stream) {
 }
 
 int batch_routine_count() { return kNumRoutines; }
+int batch_routine_default(int n, bool c64) {
+  // complex64 N=128: more lanes per signal read longer runs of 8-byte values
+  if (c64 && n == 128) return 19;
+  for (int id = 0; id < kNumRoutines; ++id)
+    if (kRoutines[id].n == n) return id;
+  return -1;
+}
 int batch_routine_n(int id) { return (id >= 0 && id < kNumRoutines) ? kRoutines[id].n : 0; }
 const char *batch_routine_name(int id) {
   return (id >= 0 && id < kNumRoutines) ? kRoutines[id].name : nullptr;

@@ -85,11 +85,6 @@ int check_cross_overlap(const fftoc_job *jobs, int njobs) {
   return FFTOC_SUCCESS;
 }
 
-int first_routine(int n) {
-  for (int id = 0; id < batch_routine_count(); ++id)
-    if (batch_routine_n(id) == n) return id;
-  return -1;
-}
 
 // Lays the jobs out as work items and uploads the descriptors.
 int layout(fftoc_plan_t *p) {
@@ -183,7 +178,10 @@ int fftoc_plan_create(const fftoc_job *jobs, int njobs, fftoc_plan *out) {
     if (p->jobs[a].n != p->jobs[b].n) return p->jobs[a].n > p->jobs[b].n;
     return p->jobs[a].batch > p->jobs[b].batch;
   });
-  for (int k = 0; k < 13; ++k) p->routine[k][0] = p->routine[k][1] = first_routine(1 << k);
+  for (int k = 0; k < 13; ++k) {
+    p->routine[k][0] = batch_routine_default(1 << k, false);
+    p->routine[k][1] = batch_routine_default(1 << k, true);
+  }
   if (hipMalloc(&p->d_ctr, 2 * sizeof(unsigned long long)) != hipSuccess ||
       hipMemset(p->d_ctr, 0, 2 * sizeof(unsigned long long)) != hipSuccess) {
     fftoc_plan_destroy(p);
