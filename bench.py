@@ -251,6 +251,12 @@ def run_served(fo, seconds: float = 1.0, rate: float = 20000.0, saturated: int =
                           "p50_us": round(float(np.percentile(lat, 50)) * 1e6, 1),
                           "p99_us": round(float(np.percentile(lat, 99)) * 1e6, 1),
                           "mean_us": round(float(np.mean(lat)) * 1e6, 1)}
+        # the loop's own round trip: 1-signal N=8 requests at 2 k/s, one at a time
+        tiny = [(lake.ptr, 1, 8, 0)] * 1000
+        lat0, _ = srv.replay(tiny, (np.arange(1000) * 5e-4).tolist())
+        out["round_trip_n8_b1"] = {"requests": 1000,
+                                   "p50_us": round(float(np.percentile(lat0, 50)) * 1e6, 1),
+                                   "p99_us": round(float(np.percentile(lat0, 99)) * 1e6, 1)}
         sat = reqs[count:]
         lat2, span2 = srv.replay(sat, [0.0] * len(sat))
         sig2 = sum(r[1] for r in sat)
