@@ -146,8 +146,8 @@ const char *default_best(int n, int bucket) {
   switch (n) {
     case 64:  // up to 2^16 signals the persistent form hides the launch tail
       return bucket <= 16 ? "radix:8x8/persistent/S32" : "radix:8x8/tile/S16";
-    case 8: return "radix:4x2/staged/S128";
-    case 16: return "radix:4x4/staged/S64";
+    case 8: return "radix:8/lanes/S128";
+    case 16: return "radix:16/lanes/S64";
     case 32: return "tensor:16x2/8 per wave";
     case 128: return "radix:16x8/tile/S8";
     case 256: return "radix:16x16/tile/S4";

@@ -3,12 +3,14 @@
 //
 // Work items are chunks of tiles of one job; workgroups claim items in order
 // from a device counter (dynamic load balance over uneven lengths) and run the
-// job's tile routine.  Every routine is a Stockham tile on exactly 256
-// threads with at most 4096 complex values in LDS, so one workgroup shape and
-// one LDS allocation serve every length.  The last workgroup to finish resets
+// job's tile routine.  Every routine is a tile on exactly 256 threads with at
+// most 4096 complex values in LDS (Stockham tiles; for N = 8 / 16 also the
+// lane-shuffle tiles of lane_fft.hpp, which use none), so one workgroup shape
+// and one LDS allocation serve every length.  The last workgroup to finish resets
 // the counters, so the launch can be replayed from a hipGraph without any
 // memset node.
 #include "batch_internal.hpp"
+#include "lane_fft.hpp"
 #include "stockham.hpp"
 
 namespace fftoc {
@@ -17,37 +19,52 @@ namespace {
 constexpr int kBatchThreads = 256;
 constexpr int kLdsElems = lds_elems(4096);
 
-// Routine table: id -> (n, split, signals per tile, staged).  Two or three
+// Tile forms: a Stockham tile read from HBM directly (Tile) or staged
+// through LDS (Staged), or the lane-shuffle tile (Lanes, N = 8 / 16).
+enum class TForm { Tile, Staged, Lanes };
+
+template <class SP, int S, TForm F, class CT>
+struct RoutineKernel {
+  using type = StockhamKernel<SP, S, 2, F == TForm::Staged, CT>;
+};
+template <class SP, int S, class CT>
+struct RoutineKernel<SP, S, TForm::Lanes, CT> {
+  using type = LaneTile<SP::N, S * SP::N / kBatchThreads, 2, CT>;
+};
+
+// Routine table: id -> (n, split, signals per tile, form).  Two or three
 // candidates per length; the plan autotuner picks among them, and
 // batch_routine_default gives the untuned choice per data type.
 #define FFTOC_ROUTINES(X)                              \
-  X(0, 8, "4x2/staged/S128", (Split<4, 2>), 128, true)  \
-  X(1, 8, "8/staged/S256", (Split<8>), 256, true)        \
-  X(2, 16, "4x4/staged/S64", (Split<4, 4>), 64, true)    \
-  X(3, 16, "16/staged/S256", (Split<16>), 256, true)     \
-  X(4, 32, "8x4/staged/S64", (Split<8, 4>), 64, true)    \
-  X(5, 32, "4x4x2/staged/S32", (Split<4, 4, 2>), 32, true) \
-  X(6, 64, "8x8/tile/S32", (Split<8, 8>), 32, false)     \
-  X(7, 64, "4x4x4/tile/S16", (Split<4, 4, 4>), 16, false) \
-  X(8, 128, "16x8/tile/S32", (Split<16, 8>), 32, false)  \
-  X(9, 128, "8x16/tile/S32", (Split<8, 16>), 32, false)  \
-  X(10, 256, "16x16/tile/S16", (Split<16, 16>), 16, false) \
-  X(11, 256, "8x8x4/tile/S8", (Split<8, 8, 4>), 8, false) \
-  X(12, 512, "8x8x8/tile/S4", (Split<8, 8, 8>), 4, false) \
-  X(13, 512, "16x16x2/tile/S8", (Split<16, 16, 2>), 8, false) \
-  X(14, 1024, "16x16x4/tile/S4", (Split<16, 16, 4>), 4, false) \
-  X(15, 1024, "8x8x16/tile/S4", (Split<8, 8, 16>), 4, false) \
-  X(16, 2048, "16x16x8/tile/S2", (Split<16, 16, 8>), 2, false) \
-  X(17, 2048, "8x16x16/tile/S2", (Split<8, 16, 16>), 2, false) \
-  X(18, 4096, "16x16x16/tile/S1", (Split<16, 16, 16>), 1, false) \
-  X(19, 128, "8x4x4/tile/S16", (Split<8, 4, 4>), 16, false)
+  X(0, 8, "4x2/staged/S128", (Split<4, 2>), 128, TForm::Staged)  \
+  X(1, 8, "8/staged/S256", (Split<8>), 256, TForm::Staged)        \
+  X(2, 16, "4x4/staged/S64", (Split<4, 4>), 64, TForm::Staged)    \
+  X(3, 16, "16/staged/S256", (Split<16>), 256, TForm::Staged)     \
+  X(4, 32, "8x4/staged/S64", (Split<8, 4>), 64, TForm::Staged)    \
+  X(5, 32, "4x4x2/staged/S32", (Split<4, 4, 2>), 32, TForm::Staged) \
+  X(6, 64, "8x8/tile/S32", (Split<8, 8>), 32, TForm::Tile)     \
+  X(7, 64, "4x4x4/tile/S16", (Split<4, 4, 4>), 16, TForm::Tile) \
+  X(8, 128, "16x8/tile/S32", (Split<16, 8>), 32, TForm::Tile)  \
+  X(9, 128, "8x16/tile/S32", (Split<8, 16>), 32, TForm::Tile)  \
+  X(10, 256, "16x16/tile/S16", (Split<16, 16>), 16, TForm::Tile) \
+  X(11, 256, "8x8x4/tile/S8", (Split<8, 8, 4>), 8, TForm::Tile) \
+  X(12, 512, "8x8x8/tile/S4", (Split<8, 8, 8>), 4, TForm::Tile) \
+  X(13, 512, "16x16x2/tile/S8", (Split<16, 16, 2>), 8, TForm::Tile) \
+  X(14, 1024, "16x16x4/tile/S4", (Split<16, 16, 4>), 4, TForm::Tile) \
+  X(15, 1024, "8x8x16/tile/S4", (Split<8, 8, 16>), 4, TForm::Tile) \
+  X(16, 2048, "16x16x8/tile/S2", (Split<16, 16, 8>), 2, TForm::Tile) \
+  X(17, 2048, "8x16x16/tile/S2", (Split<8, 16, 16>), 2, TForm::Tile) \
+  X(18, 4096, "16x16x16/tile/S1", (Split<16, 16, 16>), 1, TForm::Tile) \
+  X(19, 128, "8x4x4/tile/S16", (Split<8, 4, 4>), 16, TForm::Tile) \
+  X(20, 8, "8/lanes/S128", (Split<8>), 128, TForm::Lanes) \
+  X(21, 16, "16/lanes/S64", (Split<16>), 64, TForm::Lanes)
 
 #define FFTOC_UNPAREN(...) __VA_ARGS__
 
-#define FFTOC_CHECK_THREADS(ID, NN, NAME, SPLIT, S, STAGED)                              \
-  static_assert((FFTOC_UNPAREN SPLIT::N / FFTOC_UNPAREN SPLIT::RM) * S == kBatchThreads, \
-                "tile routine " #ID " must use 256 threads");                            \
-  static_assert(FFTOC_UNPAREN SPLIT::N == NN, "tile routine " #ID " length");            \
+#define FFTOC_CHECK_THREADS(ID, NN, NAME, SPLIT, S, FORM)                                     \
+  static_assert(RoutineKernel<FFTOC_UNPAREN SPLIT, S, FORM, c2>::type::THREADS == kBatchThreads, \
+                "tile routine " #ID " must use 256 threads");                                 \
+  static_assert(FFTOC_UNPAREN SPLIT::N == NN, "tile routine " #ID " length");                 \
   static_assert(S * NN <= 4096, "tile routine " #ID " LDS");
 FFTOC_ROUTINES(FFTOC_CHECK_THREADS)
 #undef FFTOC_CHECK_THREADS
@@ -58,7 +75,7 @@ struct RoutineInfo {
   int signals_per_tile;
 };
 
-#define FFTOC_INFO(ID, NN, NAME, SPLIT, S, STAGED) {NN, NAME, S},
+#define FFTOC_INFO(ID, NN, NAME, SPLIT, S, FORM) {NN, NAME, S},
 constexpr RoutineInfo kRoutines[] = {FFTOC_ROUTINES(FFTOC_INFO)};
 #undef FFTOC_INFO
 constexpr int kNumRoutines = sizeof(kRoutines) / sizeof(kRoutines[0]);
@@ -89,9 +106,9 @@ __device__ __forceinline__ void run_routine(int id, const JobDev &J, int64_t til
   const real_t<CT> scale = (real_t<CT>)J.scale;
   const bool inv = J.inverse != 0;
   switch (id) {
-#define FFTOC_CASE(ID, NN, NAME, SPLIT, S, STAGED)                                          \
+#define FFTOC_CASE(ID, NN, NAME, SPLIT, S, FORM)                                            \
   case ID:                                                                                \
-    tile_call<StockhamKernel<FFTOC_UNPAREN SPLIT, S, 2, STAGED, CT>, CT>(                  \
+    tile_call<typename RoutineKernel<FFTOC_UNPAREN SPLIT, S, FORM, CT>::type, CT>(          \
         (global_ptr<const CT>)in, (global_ptr<CT>)out, J.batch, tile, (lds_ptr<CT>)lds,     \
         (global_ptr<const CT>)tw, scale, inv);                                               \
     break;
@@ -447,6 +464,8 @@ int batch_routine_count() { return kNumRoutines; }
 int batch_routine_default(int n, bool c64) {
   // complex64 N=128: more lanes per signal read longer runs of 8-byte values
   if (c64 && n == 128) return 19;
+  if (n == 8) return 20;   // lane-shuffle tiles: no LDS round trips
+  if (n == 16) return 21;
   for (int id = 0; id < kNumRoutines; ++id)
     if (kRoutines[id].n == n) return id;
   return -1;

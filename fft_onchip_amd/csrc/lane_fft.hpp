@@ -1,0 +1,168 @@
+// Radix-8 / radix-16 butterflies across lanes with DPP (gfx950 data-parallel
+// primitives), for the shortest lengths, N = 8 and 16, where one signal is
+// 128 / 256 bytes.
+//
+// Layout: a wave reads 64 consecutive complex values per load instruction
+// (1 KiB of FP64 data, fully contiguous), so lane l holds element l mod N of
+// signal l / N of that instruction's slice.  The N lanes of a signal lie in
+// one 16-lane DPP row, and the signal's N-point DFT is computed in place
+// across them as log2(N) radix-2 decimation-in-frequency steps.  Step D
+// (D = N/2, ..., 1) pairs lane p with lane p ^ D of the group:
+//   lower lane (p & D == 0):  y = x_p + x_{p+D}
+//   upper lane:               y = (x_{p-D} - x_p) * W_{2D}^{p mod D}
+// The partner's value arrives by DPP: quad_perm for D = 1, 2 (inside 4-lane
+// quads), row_ror for D = 4, 8 (inside 16-lane rows).  After the last step
+// lane p of the group holds X[bitrev(p)]; the store writes it straight to
+// that index, so each store instruction again covers the same contiguous
+// 1 KiB (a permutation inside every N-lane group).  No LDS is used, so
+// occupancy is bounded by VGPRs only.
+//
+// This is a radix-N pass of the reference's legacy kernels
+// (includes/legacy16_fft.cuh:99-172 with N = 16 a single pass; radix 8 in
+// includes/legacy8_fft.cuh:57-118), with the butterfly's data exchange done
+// by lane shuffles instead of per-thread registers.
+#pragma once
+
+#include "fftoc_device.hpp"
+
+namespace fftoc {
+
+// DPP control words (GFX9 encoding).  row_ror:n makes lane l of a 16-lane
+// row read lane (l - n) mod 16 of the same row.
+constexpr int kDppQuadXor1 = 0xB1;  // quad_perm [1,0,3,2]: lane l reads l ^ 1
+constexpr int kDppQuadXor2 = 0x4E;  // quad_perm [2,3,0,1]: lane l reads l ^ 2
+constexpr int kDppRowRor4 = 0x124;  // reads (l - 4) mod 16
+constexpr int kDppRowRor8 = 0x128;  // reads (l + 8) mod 16 = l ^ 8
+constexpr int kDppRowRor12 = 0x12C; // reads (l + 4) mod 16
+
+template <int CTRL>
+__device__ __forceinline__ float dpp(float v) {
+  return __builtin_bit_cast(
+      float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xF, 0xF, true));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp(double v) {
+  const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)u, CTRL, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(u >> 32), CTRL, 0xF, 0xF, true);
+  return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+template <int CTRL, class CT>
+__device__ __forceinline__ CT dpp_c(CT v) {
+  return mk<CT>(dpp<CTRL>(v.x), dpp<CTRL>(v.y));
+}
+
+constexpr int ilog2(int n) { return n <= 1 ? 0 : 1 + ilog2(n / 2); }
+
+template <int N>
+__device__ __forceinline__ int bit_reverse(int p) {
+  int r = 0;
+#pragma unroll
+  for (int b = 0; b < ilog2(N); ++b) r |= ((p >> b) & 1) << (ilog2(N) - 1 - b);
+  return r;
+}
+
+// N-point DFT of values spread one per lane over aligned groups of N lanes
+// (N = 8 or 16); every lane of the wave must call it (DPP reads other lanes).
+template <int N, class CT>
+struct LaneFft {
+  static_assert(N == 8 || N == 16, "groups must fit a 16-lane DPP row");
+  using R = real_t<CT>;
+  static constexpr int kSteps = ilog2(N);
+
+  R sgn[kSteps];      // +1 on lower lanes, -1 on upper lanes of each step
+  CT w[kSteps - 1];   // step twiddle (1 on lower lanes); the D = 1 step has none
+  bool upper4;        // lane takes the D = 4 partner from l - 4 (else l + 4)
+
+  // tw: W_4096^j, j = 0..4095; lane: lane index in the wave.
+  __device__ LaneFft(const CT *__restrict__ tw, int lane) {
+    const int p = lane & (N - 1);
+#pragma unroll
+    for (int s = 0; s < kSteps; ++s) {
+      const int d = (N / 2) >> s;
+      const bool up = (p & d) != 0;
+      sgn[s] = up ? (R)-1 : (R)1;
+      // lower lanes read tw[0] = 1 exactly
+      if (s < kSteps - 1) w[s] = tw[up ? (p & (d - 1)) * (kTwN / (2 * d)) : 0];
+    }
+    upper4 = (p & 4) != 0;
+  }
+
+  template <int D>
+  __device__ __forceinline__ CT partner(CT x) const {
+    if constexpr (D == 8) {
+      return dpp_c<kDppRowRor8>(x);
+    } else if constexpr (D == 4) {
+      const CT from_below = dpp_c<kDppRowRor4>(x);   // lane l - 4
+      const CT from_above = dpp_c<kDppRowRor12>(x);  // lane l + 4
+      return upper4 ? from_below : from_above;
+    } else if constexpr (D == 2) {
+      return dpp_c<kDppQuadXor2>(x);
+    } else {
+      return dpp_c<kDppQuadXor1>(x);
+    }
+  }
+
+  template <int S>
+  __device__ __forceinline__ CT step(CT x) const {
+    constexpr int D = (N / 2) >> S;
+    const CT q = partner<D>(x);
+    // lower: x + q; upper: q - x (q is the lower partner's value)
+    CT y = mk<CT>(fma(sgn[S], x.x, q.x), fma(sgn[S], x.y, q.y));
+    if constexpr (D > 1) y = cmul(y, w[S]);
+    return y;
+  }
+
+  // Returns X[bitrev(p)] on lane p of each group.
+  __device__ __forceinline__ CT operator()(CT x) const {
+    x = step<0>(x);
+    x = step<1>(x);
+    x = step<2>(x);
+    if constexpr (N == 16) x = step<3>(x);
+    return x;
+  }
+};
+
+// One tile of S = 256 * C / N consecutive signals on a 256-thread workgroup:
+// wave w transforms elements [tile * S * N + w * C * 64, + C * 64), C load
+// instructions of 64 consecutive values.  INVM: 0 forward, 1 inverse, 2 the
+// `inv` argument.  Same interface as StockhamKernel::run_tile (the LDS
+// argument is unused).
+template <int N, int C, int INVM, class CT>
+struct LaneTile {
+  static constexpr int THREADS = 256;
+  static constexpr int S = THREADS * C / N;
+  static_assert(S * N == THREADS * C, "whole signals per wave");
+
+  __device__ static void run_tile(const CT *in, CT *out, int64_t batch, int64_t tile, CT *,
+                                  const CT *__restrict__ tw, real_t<CT> scale, bool inv = false) {
+    const bool sw = INVM == 0 ? false : INVM == 1 ? true : inv;
+    const int lane = threadIdx.x & 63;
+    const int64_t total = batch * N;
+    const int64_t base = tile * (S * N) + (int64_t)(threadIdx.x >> 6) * (C * 64);
+    const LaneFft<N, CT> f(tw, lane);
+    const int p = lane & (N - 1);
+    const int to = bit_reverse<N>(p) - p;  // the group's output index minus its input index
+    CT v[C];
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      // past the end: read the last element (no branch) and use zero
+      const int64_t e = base + c * 64 + lane;
+      v[c] = ld_stream(in + (e < total ? e : total - 1));
+      if (e >= total) v[c] = mk<CT>(0, 0);
+      if (sw) v[c] = cswap(v[c]);
+    }
+#pragma unroll
+    for (int c = 0; c < C; ++c) v[c] = f(v[c]);
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      const int64_t e = base + c * 64 + lane;  // a whole N-lane group is valid or not
+      CT r = v[c];
+      if (scale != (real_t<CT>)1) r = cscale(r, scale);
+      if (sw) r = cswap(r);
+      if (e < total) st_stream(out + e + to, r);
+    }
+  }
+};
+
+}  // namespace fftoc

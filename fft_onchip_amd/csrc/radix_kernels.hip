@@ -28,6 +28,7 @@
 //    long double (L2/L1 resident); fixed roots inside the butterflies are
 //    exact-rounded constants.
 #include "fftoc_internal.hpp"
+#include "lane_fft.hpp"
 #include "stockham.hpp"
 
 namespace fftoc {
@@ -48,6 +49,38 @@ void stockham_persistent_kernel(
     const c2 *in, c2 *out, int64_t batch, const c2 *__restrict__ tw, double scale) {
   StockhamKernel<SP, S, INV, false>::run_persistent(in, out, batch, tw, scale);
 }
+
+// N = 8 / 16 with the butterflies across lanes (lane_fft.hpp): each wave
+// transforms C * 64 / N signals, C load instructions of 64 consecutive values.
+constexpr int kLaneThreads = 256;
+static_assert(LaneTile<16, 4, 0, c2>::THREADS == kLaneThreads);
+
+template <int N, int C, bool INV, class CT>
+__global__ __launch_bounds__(kLaneThreads) void lane_fft_kernel(
+    const CT *in, CT *out, int64_t batch, const CT *__restrict__ tw, real_t<CT> scale) {
+  LaneTile<N, C, INV ? 1 : 0, CT>::run_tile(in, out, batch, blockIdx.x, nullptr, tw, scale);
+}
+
+template <int N, int C, class CT>
+int launch_lane(const LaunchArgs &a) {
+  constexpr int64_t per_block = (int64_t)kLaneThreads * C;  // elements
+  const int64_t blocks = (a.batch * N + per_block - 1) / per_block;
+  if (blocks > 0x7fffffffLL) return FFTOC_ERROR_INVALID_BATCH;
+  const CT *in = static_cast<const CT *>(a.in);
+  CT *out = static_cast<CT *>(a.out);
+  const CT *tw = static_cast<const CT *>(a.twiddles);
+  const real_t<CT> scale = (real_t<CT>)a.scale;
+  if (a.inverse)
+    hipLaunchKernelGGL((lane_fft_kernel<N, C, true, CT>), dim3((unsigned)blocks),
+                       dim3(kLaneThreads), 0, a.stream, in, out, a.batch, tw, scale);
+  else
+    hipLaunchKernelGGL((lane_fft_kernel<N, C, false, CT>), dim3((unsigned)blocks),
+                       dim3(kLaneThreads), 0, a.stream, in, out, a.batch, tw, scale);
+  return hipGetLastError() == hipSuccess ? FFTOC_SUCCESS : FFTOC_ERROR_LAUNCH;
+}
+
+#define FFTOC_LANE(NAME, FAM, N, C, CT) \
+  Variant { NAME, FAM, &launch_lane<N, C, CT>, kLaneThreads * (C) / (N) }
 
 enum class Form { OneTile, Staged, Persistent };
 
@@ -95,11 +128,15 @@ constexpr unsigned R16 = FAM_RADIX16 | FAM_ANY;
 constexpr unsigned MIX = FAM_ANY;
 
 const Variant kV8[] = {
+    FFTOC_LANE("radix:8/lanes/S128", R8, 8, 4, c2),
+    FFTOC_LANE("radix:8/lanes/S256", R8, 8, 8, c2),
     FFTOC_V("radix:8/staged/S128", R8, Split<8>, 128, Form::Staged),
     FFTOC_V("radix:8/staged/S256", R8, Split<8>, 256, Form::Staged),
     FFTOC_V("radix:4x2/staged/S128", MIX, Split<4, 2>, 128, Form::Staged),
 };
 const Variant kV16[] = {
+    FFTOC_LANE("radix:16/lanes/S64", R16, 16, 4, c2),
+    FFTOC_LANE("radix:16/lanes/S128", R16, 16, 8, c2),
     FFTOC_V("radix:16/staged/S64", R16, Split<16>, 64, Form::Staged),
     FFTOC_V("radix:16/staged/S128", R16, Split<16>, 128, Form::Staged),
     FFTOC_V("radix:4x4/staged/S64", MIX, Split<4, 4>, 64, Form::Staged),
@@ -163,9 +200,13 @@ const Variant kV4096[] = {
   Variant { NAME, FAM, &launch_split<FFTOC_UNPAREN SPLIT_S_FORM, f2>, \
             tile_of<FFTOC_UNPAREN SPLIT_S_FORM, f2>() }
 #define FFTOC_UNPAREN(...) __VA_ARGS__
-const Variant kW8[] = {FFTOC_V32("radix:4x2/staged/S128", MIX, (Split<4, 2>, 128, Form::Staged)),
+const Variant kW8[] = {FFTOC_LANE("radix:8/lanes/S128", R8, 8, 4, f2),
+                       FFTOC_LANE("radix:8/lanes/S256", R8, 8, 8, f2),
+                       FFTOC_V32("radix:4x2/staged/S128", MIX, (Split<4, 2>, 128, Form::Staged)),
                        FFTOC_V32("radix:8/staged/S128", R8, (Split<8>, 128, Form::Staged))};
-const Variant kW16[] = {FFTOC_V32("radix:4x4/staged/S64", MIX, (Split<4, 4>, 64, Form::Staged)),
+const Variant kW16[] = {FFTOC_LANE("radix:16/lanes/S128", R16, 16, 8, f2),
+                        FFTOC_LANE("radix:16/lanes/S64", R16, 16, 4, f2),
+                        FFTOC_V32("radix:4x4/staged/S64", MIX, (Split<4, 4>, 64, Form::Staged)),
                         FFTOC_V32("radix:16/staged/S64", R16, (Split<16>, 64, Form::Staged))};
 const Variant kW32[] = {FFTOC_V32("radix:8x4/staged/S64", MIX, (Split<8, 4>, 64, Form::Staged)),
                         FFTOC_V32("radix:8x4/staged/S32", MIX, (Split<8, 4>, 32, Form::Staged))};
@@ -186,6 +227,7 @@ const Variant kW2048[] = {FFTOC_V32("radix:8x16x16/tile/S1", MIX, (Split<8, 16, 
 const Variant kW4096[] = {FFTOC_V32("radix:16x16x16/tile/S1", R16, (Split<16, 16, 16>, 1, Form::OneTile)),
                           FFTOC_V32("radix:8x8x8x8/tile/S1", R8, (Split<8, 8, 8, 8>, 1, Form::OneTile))};
 #undef FFTOC_V32
+#undef FFTOC_LANE
 #undef FFTOC_UNPAREN
 #undef FFTOC_V
 

@@ -41,6 +41,34 @@ def test_complex64_matches_oracle(n, path):
     assert np.max(oracle.rel_l2(goti, refi, axis=-1)) <= TOL32
 
 
+@pytest.mark.parametrize("n", fo.ALL_SIZES)
+def test_every_complex64_variant_matches_oracle(n):
+    """Each complex64 kernel variant (indexes into the complex64 list of the
+    length, until the library reports none), forward and inverse, on a batch
+    that leaves a partial last workgroup."""
+    lib = fo.lib()
+    batch = max(3, min(515, 65536 // n)) | 1
+    x = signals(n, batch, seed=700 + n, c64=True)
+    refs = {False: oracle.reference(x.astype(np.complex128)),
+            True: oracle.reference(x.astype(np.complex128), inverse=True)}
+    src = fo.DeviceBuffer.from_array(x)
+    dst = fo.DeviceBuffer(x.nbytes)
+    v = 0
+    while True:
+        rc = lib.fftoc_fft_variant(src.ptr, dst.ptr, batch, n, fo.DATA_C64, v, None)
+        if rc != 0:
+            assert rc == 13 and v > 0, (n, v, rc)  # INVALID_HANDLE past the last variant
+            break
+        for inv in (False, True):
+            print(f"[c64 variant] n={n} v={v} inverse={inv}", flush=True)
+            flags = fo.DATA_C64 | (fo.INVERSE if inv else 0)
+            assert lib.fftoc_fft_variant(src.ptr, dst.ptr, batch, n, flags, v, None) == 0
+            fo.synchronize()
+            got = dst.download(x.shape, np.complex64)
+            assert np.max(oracle.rel_l2(got, refs[inv], axis=-1)) <= TOL32, (n, v, inv)
+        v += 1
+
+
 def make_jobs(seed, c64_mix=True):
     rng = np.random.default_rng(seed)
     jobs = []
