@@ -148,7 +148,7 @@ const char *default_best(int n, int bucket) {
       return bucket <= 16 ? "radix:8x8/persistent/S32" : "radix:8x8/tile/S16";
     case 8: return "radix:8/lanes/S128";
     case 16: return "radix:16/lanes/S64";
-    case 32: return "tensor:16x2/8 per wave";
+    case 32: return "radix:32/lanes/S32";
     case 128: return "radix:16x8/tile/S8";
     case 256: return "radix:16x16/tile/S4";
     case 512: return "radix:8x8x8/tile/S1";

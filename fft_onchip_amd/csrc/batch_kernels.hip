@@ -4,7 +4,7 @@
 // Work items are chunks of tiles of one job; workgroups claim items in order
 // from a device counter (dynamic load balance over uneven lengths) and run the
 // job's tile routine.  Every routine is a tile on exactly 256 threads with at
-// most 4096 complex values in LDS (Stockham tiles; for N = 8 / 16 also the
+// most 4096 complex values in LDS (Stockham tiles; for N = 8 ... 64 also the
 // lane-shuffle tiles of lane_fft.hpp, which use none), so one workgroup shape
 // and one LDS allocation serve every length.  The last workgroup to finish resets
 // the counters, so the launch can be replayed from a hipGraph without any
@@ -20,7 +20,7 @@ constexpr int kBatchThreads = 256;
 constexpr int kLdsElems = lds_elems(4096);
 
 // Tile forms: a Stockham tile read from HBM directly (Tile) or staged
-// through LDS (Staged), or the lane-shuffle tile (Lanes, N = 8 / 16).
+// through LDS (Staged), or the lane-shuffle tile (Lanes, N = 8 ... 64).
 enum class TForm { Tile, Staged, Lanes };
 
 template <class SP, int S, TForm F, class CT>
@@ -57,7 +57,9 @@ struct RoutineKernel<SP, S, TForm::Lanes, CT> {
   X(18, 4096, "16x16x16/tile/S1", (Split<16, 16, 16>), 1, TForm::Tile) \
   X(19, 128, "8x4x4/tile/S16", (Split<8, 4, 4>), 16, TForm::Tile) \
   X(20, 8, "8/lanes/S128", (Split<8>), 128, TForm::Lanes) \
-  X(21, 16, "16/lanes/S64", (Split<16>), 64, TForm::Lanes)
+  X(21, 16, "16/lanes/S64", (Split<16>), 64, TForm::Lanes) \
+  X(22, 32, "32/lanes/S32", (Split<32>), 32, TForm::Lanes) \
+  X(23, 64, "64/lanes/S16", (Split<64>), 16, TForm::Lanes)
 
 #define FFTOC_UNPAREN(...) __VA_ARGS__
 
@@ -466,6 +468,7 @@ int batch_routine_default(int n, bool c64) {
   if (c64 && n == 128) return 19;
   if (n == 8) return 20;   // lane-shuffle tiles: no LDS round trips
   if (n == 16) return 21;
+  if (n == 32) return 22;
   for (int id = 0; id < kNumRoutines; ++id)
     if (kRoutines[id].n == n) return id;
   return -1;

@@ -1,21 +1,22 @@
 // Radix-8 / radix-16 butterflies across lanes with DPP (gfx950 data-parallel
-// primitives), for the shortest lengths, N = 8 and 16, where one signal is
-// 128 / 256 bytes.
+// primitives) and the gfx950 lane-swap instructions, for the shortest
+// lengths, N = 8 ... 64, where one signal is 128 B ... 1 KiB.
 //
 // Layout: a wave reads 64 consecutive complex values per load instruction
 // (1 KiB of FP64 data, fully contiguous), so lane l holds element l mod N of
-// signal l / N of that instruction's slice.  The N lanes of a signal lie in
-// one 16-lane DPP row, and the signal's N-point DFT is computed in place
-// across them as log2(N) radix-2 decimation-in-frequency steps.  Step D
-// (D = N/2, ..., 1) pairs lane p with lane p ^ D of the group:
+// signal l / N of that instruction's slice.  The signal's N-point DFT is
+// computed in place across its N lanes as log2(N) radix-2
+// decimation-in-frequency steps.  Step D (D = N/2, ..., 1) pairs lane p with
+// lane p ^ D of the group:
 //   lower lane (p & D == 0):  y = x_p + x_{p+D}
 //   upper lane:               y = (x_{p-D} - x_p) * W_{2D}^{p mod D}
-// The partner's value arrives by DPP: quad_perm for D = 1, 2 (inside 4-lane
-// quads), row_ror for D = 4, 8 (inside 16-lane rows).  After the last step
-// lane p of the group holds X[bitrev(p)]; the store writes it straight to
-// that index, so each store instruction again covers the same contiguous
-// 1 KiB (a permutation inside every N-lane group).  No LDS is used, so
-// occupancy is bounded by VGPRs only.
+// The partner's value arrives by DPP for D = 1, 2 (quad_perm inside 4-lane
+// quads) and D = 4, 8 (row_ror inside 16-lane rows), and by
+// v_permlane16_swap / v_permlane32_swap for D = 16 / 32 (across rows / wave
+// halves).  After the last step lane p of the group holds X[bitrev(p)]; the
+// store writes it straight to that index, so each store instruction again
+// covers the same contiguous 1 KiB (a permutation inside every N-lane
+// group).  No LDS is used, so occupancy is bounded by VGPRs only.
 //
 // This is a radix-N pass of the reference's legacy kernels
 // (includes/legacy16_fft.cuh:99-172 with N = 16 a single pass; radix 8 in
@@ -52,6 +53,37 @@ __device__ __forceinline__ CT dpp_c(CT v) {
   return mk<CT>(dpp<CTRL>(v.x), dpp<CTRL>(v.y));
 }
 
+// Value of lane l ^ 16 (ROWS = true) or l ^ 32 (ROWS = false) of a 32-bit
+// register.  permlane{16,32}_swap(x, x) returns (x with its odd rows / upper
+// half replaced by the even rows / lower half, x with its even rows / lower
+// half replaced by the odd rows / upper half): the partner is the second
+// result on even rows / the lower half and the first one elsewhere.
+template <bool ROWS>
+__device__ __forceinline__ unsigned swap_u32(unsigned v, bool high) {
+  if constexpr (ROWS) {
+    const auto r = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+    return high ? r[0] : r[1];
+  } else {
+    const auto r = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+    return high ? r[0] : r[1];
+  }
+}
+template <bool ROWS>
+__device__ __forceinline__ float swap_r(float v, bool high) {
+  return __builtin_bit_cast(float, swap_u32<ROWS>(__builtin_bit_cast(unsigned, v), high));
+}
+template <bool ROWS>
+__device__ __forceinline__ double swap_r(double v, bool high) {
+  const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+  const unsigned lo = swap_u32<ROWS>((unsigned)u, high);
+  const unsigned hi = swap_u32<ROWS>((unsigned)(u >> 32), high);
+  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+template <bool ROWS, class CT>
+__device__ __forceinline__ CT swap_c(CT v, bool high) {
+  return mk<CT>(swap_r<ROWS>(v.x, high), swap_r<ROWS>(v.y, high));
+}
+
 constexpr int ilog2(int n) { return n <= 1 ? 0 : 1 + ilog2(n / 2); }
 
 template <int N>
@@ -63,16 +95,19 @@ __device__ __forceinline__ int bit_reverse(int p) {
 }
 
 // N-point DFT of values spread one per lane over aligned groups of N lanes
-// (N = 8 or 16); every lane of the wave must call it (DPP reads other lanes).
+// (N = 8, 16, 32 or 64); every lane of the wave must call it (the exchanges
+// read other lanes).
 template <int N, class CT>
 struct LaneFft {
-  static_assert(N == 8 || N == 16, "groups must fit a 16-lane DPP row");
+  static_assert(N == 8 || N == 16 || N == 32 || N == 64, "N-lane groups inside one wave");
   using R = real_t<CT>;
   static constexpr int kSteps = ilog2(N);
 
   R sgn[kSteps];      // +1 on lower lanes, -1 on upper lanes of each step
   CT w[kSteps - 1];   // step twiddle (1 on lower lanes); the D = 1 step has none
   bool upper4;        // lane takes the D = 4 partner from l - 4 (else l + 4)
+  bool odd_row;       // lane bit 4: the D = 16 partner is in the row below
+  bool high_half;     // lane bit 5: the D = 32 partner is in the lower half
 
   // tw: W_4096^j, j = 0..4095; lane: lane index in the wave.
   __device__ LaneFft(const CT *__restrict__ tw, int lane) {
@@ -86,11 +121,17 @@ struct LaneFft {
       if (s < kSteps - 1) w[s] = tw[up ? (p & (d - 1)) * (kTwN / (2 * d)) : 0];
     }
     upper4 = (p & 4) != 0;
+    odd_row = (lane & 16) != 0;
+    high_half = (lane & 32) != 0;
   }
 
   template <int D>
   __device__ __forceinline__ CT partner(CT x) const {
-    if constexpr (D == 8) {
+    if constexpr (D == 32) {
+      return swap_c<false>(x, high_half);
+    } else if constexpr (D == 16) {
+      return swap_c<true>(x, odd_row);
+    } else if constexpr (D == 8) {
       return dpp_c<kDppRowRor8>(x);
     } else if constexpr (D == 4) {
       const CT from_below = dpp_c<kDppRowRor4>(x);   // lane l - 4
@@ -118,7 +159,9 @@ struct LaneFft {
     x = step<0>(x);
     x = step<1>(x);
     x = step<2>(x);
-    if constexpr (N == 16) x = step<3>(x);
+    if constexpr (kSteps > 3) x = step<3>(x);
+    if constexpr (kSteps > 4) x = step<4>(x);
+    if constexpr (kSteps > 5) x = step<5>(x);
     return x;
   }
 };

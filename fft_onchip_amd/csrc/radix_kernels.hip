@@ -147,12 +147,16 @@ const Variant kV32[] = {
     FFTOC_V("radix:8x4/tile/S64", MIX, Split<8, 4>, 64, Form::OneTile),
     FFTOC_V("radix:4x8/staged/S32", MIX, Split<4, 8>, 32, Form::Staged),
     FFTOC_V("radix:4x4x2/staged/S64", MIX, Split<4, 4, 2>, 64, Form::Staged),
+    FFTOC_LANE("radix:32/lanes/S32", MIX, 32, 4, c2),
+    FFTOC_LANE("radix:32/lanes/S64", MIX, 32, 8, c2),
 };
 const Variant kV64[] = {
     FFTOC_V("radix:8x8/tile/S16", R8, Split<8, 8>, 16, Form::OneTile),
     FFTOC_V("radix:8x8/tile/S32", R8, Split<8, 8>, 32, Form::OneTile),
     FFTOC_V("radix:8x8/persistent/S32", R8, Split<8, 8>, 32, Form::Persistent),
     FFTOC_V("radix:16x4/tile/S16", MIX, Split<16, 4>, 16, Form::OneTile),
+    FFTOC_LANE("radix:64/lanes/S16", MIX, 64, 4, c2),
+    FFTOC_LANE("radix:64/lanes/S32", MIX, 64, 8, c2),
 };
 const Variant kV128[] = {
     FFTOC_V("radix:16x8/tile/S16", MIX, Split<16, 8>, 16, Form::OneTile),
@@ -208,7 +212,9 @@ const Variant kW16[] = {FFTOC_LANE("radix:16/lanes/S128", R16, 16, 8, f2),
                         FFTOC_LANE("radix:16/lanes/S64", R16, 16, 4, f2),
                         FFTOC_V32("radix:4x4/staged/S64", MIX, (Split<4, 4>, 64, Form::Staged)),
                         FFTOC_V32("radix:16/staged/S64", R16, (Split<16>, 64, Form::Staged))};
-const Variant kW32[] = {FFTOC_V32("radix:8x4/staged/S64", MIX, (Split<8, 4>, 64, Form::Staged)),
+const Variant kW32[] = {FFTOC_LANE("radix:32/lanes/S64", MIX, 32, 8, f2),
+                        FFTOC_LANE("radix:32/lanes/S32", MIX, 32, 4, f2),
+                        FFTOC_V32("radix:8x4/staged/S64", MIX, (Split<8, 4>, 64, Form::Staged)),
                         FFTOC_V32("radix:8x4/staged/S32", MIX, (Split<8, 4>, 32, Form::Staged))};
 // complex64: with 8-byte values the T lanes of a signal read T*8 bytes per
 // instruction, so splits with a smaller largest radix (more lanes per signal)
@@ -216,7 +222,9 @@ const Variant kW32[] = {FFTOC_V32("radix:8x4/staged/S64", MIX, (Split<8, 4>, 64,
 // against 4.9 / 4.3 for 8x8 / 16x8 (T = 8).
 const Variant kW64[] = {FFTOC_V32("radix:4x4x4/tile/S16", MIX, (Split<4, 4, 4>, 16, Form::OneTile)),
                         FFTOC_V32("radix:8x8/staged/S32", R8, (Split<8, 8>, 32, Form::Staged)),
-                        FFTOC_V32("radix:8x8/tile/S16", R8, (Split<8, 8>, 16, Form::OneTile))};
+                        FFTOC_V32("radix:8x8/tile/S16", R8, (Split<8, 8>, 16, Form::OneTile)),
+                        FFTOC_LANE("radix:64/lanes/S16", MIX, 64, 4, f2),
+                        FFTOC_LANE("radix:64/lanes/S32", MIX, 64, 8, f2)};
 const Variant kW128[] = {FFTOC_V32("radix:8x4x4/tile/S16", MIX, (Split<8, 4, 4>, 16, Form::OneTile)),
                          FFTOC_V32("radix:16x8/tile/S8", MIX, (Split<16, 8>, 8, Form::OneTile)),
                          FFTOC_V32("radix:8x8x2/tile/S16", MIX, (Split<8, 8, 2>, 16, Form::OneTile))};
