@@ -441,10 +441,12 @@ def load_traffic(workload: str):
         return None
 
 
-def traffic_from_pmc(fetch_csv: str, write_csv: str) -> dict:
+def traffic_from_pmc(fetch_csv: str, write_csv: str, out_path: str = "",
+                     workload: str = "") -> dict:
     """Scales FETCH_SIZE / WRITE_SIZE (KiB per dispatch) by what the same
     counters read for the 1 GiB device-to-device copy this script runs
-    (known: 1 GiB read, 1 GiB written) and writes profiles/traffic.json."""
+    (known: 1 GiB read, 1 GiB written) and writes the result as JSON
+    (default profiles/traffic.json, the headline workload's)."""
     import csv
     import collections
 
@@ -465,7 +467,7 @@ def traffic_from_pmc(fetch_csv: str, write_csv: str) -> dict:
     f_scale, w_scale = kib / max(copy_f), kib / max(copy_w)
     out = {"fetch_scale": f_scale, "write_scale": w_scale,
            "calibration": "1 GiB hipMemcpy device-to-device (read 1 GiB + write 1 GiB)",
-           "workload": HEADLINE, "kernels": {}}
+           "workload": workload or HEADLINE, "kernels": {}}
     for k in fetch:
         if COPY_KERNEL in k or k not in write:
             continue
@@ -473,7 +475,7 @@ def traffic_from_pmc(fetch_csv: str, write_csv: str) -> dict:
         out["kernels"][key] = {"fetch_bytes": fetch[k] * 1024 * f_scale,
                                "write_bytes": write[k] * 1024 * w_scale,
                                "bytes": (fetch[k] * f_scale + write[k] * w_scale) * 1024}
-    with open(TRAFFIC_JSON, "w") as f:
+    with open(out_path or TRAFFIC_JSON, "w") as f:
         json.dump(out, f, indent=1)
     return out
 
@@ -493,9 +495,14 @@ def main():
                     help="with --only: run this named kernel variant of the length")
     ap.add_argument("--traffic", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="write profiles/traffic.json from two rocprofv3 --pmc runs and exit")
+    ap.add_argument("--traffic-out", default="", metavar="JSON",
+                    help="with --traffic: write here instead (counter runs of other workloads)")
+    ap.add_argument("--traffic-workload", default="", metavar="NAME",
+                    help="with --traffic: the workload the counter runs measured")
     args = ap.parse_args()
     if args.traffic:
-        print(json.dumps(traffic_from_pmc(*args.traffic), indent=1))
+        print(json.dumps(traffic_from_pmc(*args.traffic, out_path=args.traffic_out,
+                                          workload=args.traffic_workload), indent=1))
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one GPU; the batched FFT shards by contiguous "
@@ -519,6 +526,8 @@ def main():
             path = "variant:" + args.variant
         wl = Workload(f"only_{path}_n{n}_{prec}", n, batch, 7, "normal", path, flag)
         r, _, _ = run_gpu(fo, wl, args.steps, args.warmup)
+        # the 1 GiB copies that calibrate FETCH_SIZE / WRITE_SIZE in counter runs
+        r["hip_d2d_copy_GBps"] = copy_bandwidth()
         print(json.dumps({k: (round(v, 6) if isinstance(v, float) else v) for k, v in r.items()}))
         return
 
@@ -546,6 +555,11 @@ def main():
             if n in (16, 256):
                 others.append(Workload(f"fp64_n{n}_b{(1 << 24) // n}_legacy16", n, (1 << 24) // n,
                                        1 + i, "uniform", "legacy16"))
+        # the lengths outside C2, at the same 2^24 points
+        for n, seed in ((32, 21), (1024, 22), (2048, 23)):
+            for path in ("best", "tensor") if n == 32 else ("best",):
+                others.append(Workload(f"fp64_n{n}_b{(1 << 24) // n}_{path}", n, (1 << 24) // n,
+                                       seed, "uniform", path))
         cache = {}
         for wl in others:
             key = (wl.n, wl.batch, wl.seed, wl.kind)
