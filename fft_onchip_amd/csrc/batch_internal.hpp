@@ -27,7 +27,9 @@ struct JobDev {
 
 // ---- serving loop -----------------------------------------------------------
 // Request descriptor in fine-grained (coherent) pinned host memory, written by
-// the host under a sequence lock: seq = kSeqInvalid, fields, seq = request id.
+// the host under a sequence lock: seq = kSeqInvalid, fields, seq = request id,
+// then `published` advanced past it; the device reads it only after acquiring
+// that count.
 constexpr uint64_t kSeqInvalid = ~0ull;
 struct alignas(64) ReqDesc {
   uint64_t seq;

@@ -160,42 +160,45 @@ __global__ __launch_bounds__(kBatchThreads) __attribute__((amdgpu_waves_per_eu(2
 
 // ---- device-resident serving loop -------------------------------------------
 
-__device__ __forceinline__ uint64_t ld_sys(const uint64_t *p) {
-  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
+// Polling reads are relaxed (they bypass non-coherent cache lines but
+// invalidate nothing); a poll that finds new work is followed by one acquire
+// fence.  An acquire load in the loop itself would invalidate the L2 of the
+// wave's XCD on every iteration.
+__device__ __forceinline__ uint64_t ld_sys_relaxed(const uint64_t *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
-__device__ __forceinline__ uint32_t ld_sys32(const uint32_t *p) {
-  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
+__device__ __forceinline__ uint32_t ld_sys32_relaxed(const uint32_t *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
+__device__ __forceinline__ void acquire_agent() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent"); }
+__device__ __forceinline__ void release_agent() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent"); }
 
-// Copies descriptor `c` from host memory under its sequence lock; false if
-// the host slot does not hold request c.
+// Copies descriptor `c` from host memory in one round trip (all words loaded
+// together); false if the slot does not hold request c.  The caller has
+// acquired `published` > c, and the host rewrites a descriptor slot only after
+// the request it held has completed, which needs this copy first: the words
+// are stable, and the sequence check only confirms the slot.
 __device__ bool read_host_desc(const ReqDesc *slot, uint64_t c, ReqDesc &d) {
-  if (ld_sys(&slot->seq) != c) return false;
   const uint64_t *src = reinterpret_cast<const uint64_t *>(slot);
   uint64_t *dst = reinterpret_cast<uint64_t *>(&d);
 #pragma unroll
-  for (int w = 1; w < (int)(sizeof(ReqDesc) / 8); ++w)
+  for (int w = 0; w < (int)(sizeof(ReqDesc) / 8); ++w)
     dst[w] = __hip_atomic_load(src + w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-  __atomic_thread_fence(__ATOMIC_ACQUIRE);
-  if (ld_sys(&slot->seq) != c) return false;
-  d.seq = c;
-  return true;
+  return d.seq == c;
 }
 
-// Device descriptor table: the dispatcher writes an entry as seq = invalid,
-// fields, seq = c (release between the steps); workers read it under the
-// same sequence lock at agent scope.
+// Device descriptor table entry of request c.  The dispatcher writes it before
+// releasing the request's work items, and the caller has acquired a published
+// item of request c, so the entry is complete; seq confirms the slot.
 __device__ bool read_dtab(const ReqDesc *e, uint64_t c, JobDev &J) {
-  if (__hip_atomic_load(&e->seq, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != c) return false;
   uint64_t w[sizeof(ReqDesc) / 8];
   const uint64_t *src = reinterpret_cast<const uint64_t *>(e);
 #pragma unroll
-  for (int k = 1; k < (int)(sizeof(ReqDesc) / 8); ++k)
+  for (int k = 0; k < (int)(sizeof(ReqDesc) / 8); ++k)
     w[k] = __hip_atomic_load(src + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  __atomic_thread_fence(__ATOMIC_ACQUIRE);
-  if (__hip_atomic_load(&e->seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != c) return false;
   ReqDesc d;
   __builtin_memcpy(&d, w, sizeof(d));
+  if (d.seq != c) return false;
   J.in = d.in;
   J.out = d.out;
   J.batch = d.batch;
@@ -226,10 +229,11 @@ __device__ void dispatcher(const ServerDev &sd) {
   uint64_t idle_since = wall_clock64();
   const uint64_t ns = (uint64_t)sd.nslots;
   for (;;) {
-    uint64_t pub = ld_sys(&sd.ring->published);
+    uint64_t pub = ld_sys_relaxed(&sd.ring->published);
     pub = ((uint64_t)__builtin_amdgcn_readfirstlane((unsigned)(pub >> 32)) << 32) |
           (uint64_t)__builtin_amdgcn_readfirstlane((unsigned)pub);
     if (pub > dpub) {
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "");  // system scope: the host's descriptors
       const uint64_t cnt = (pub - dpub) < 64 ? (pub - dpub) : 64;
       const uint64_t c = dpub + (uint64_t)lane;
       ReqDesc d;
@@ -240,15 +244,14 @@ __device__ void dispatcher(const ServerDev &sd) {
         ReqDesc *e = &sd.dtab[c % ns];
         uint64_t *dst = reinterpret_cast<uint64_t *>(e);
         const uint64_t *src = reinterpret_cast<const uint64_t *>(&d);
-        if (lane < k) __hip_atomic_store(&e->seq, kSeqInvalid, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __atomic_thread_fence(__ATOMIC_RELEASE);
+        // the entry (fields, then seq) and the work items become visible to
+        // the workers together, through the release before wpub below
         if (lane < k) {
 #pragma unroll
           for (int w = 1; w < (int)(sizeof(ReqDesc) / 8); ++w)
             __hip_atomic_store(dst + w, src[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          __hip_atomic_store(&e->seq, c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
-        __atomic_thread_fence(__ATOMIC_RELEASE);
-        if (lane < k) __hip_atomic_store(&e->seq, c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         // work items of these k requests: exclusive prefix sum of nitems
         uint64_t ni = (lane < k) ? (uint64_t)d.nitems : 0;
         uint64_t incl = ni;
@@ -263,7 +266,7 @@ __device__ void dispatcher(const ServerDev &sd) {
           __hip_atomic_store(&sd.witems[(first + i) & (kWorkRing - 1)],
                              (unsigned long long)((c << kItemBits) | i), __ATOMIC_RELAXED,
                              __HIP_MEMORY_SCOPE_AGENT);
-        __atomic_thread_fence(__ATOMIC_RELEASE);
+        release_agent();
         wp += total;
         dpub += (uint64_t)k;
         if (lane == 0) {
@@ -276,7 +279,8 @@ __device__ void dispatcher(const ServerDev &sd) {
         continue;
       }
     }
-    const unsigned stop = __builtin_amdgcn_readfirstlane(ld_sys32(&sd.ring->stop));
+    // a flag only: nothing is read on its strength
+    const unsigned stop = __builtin_amdgcn_readfirstlane(ld_sys32_relaxed(&sd.ring->stop));
     const uint64_t now = wall_clock64();
     const bool timeout = __builtin_amdgcn_readfirstlane((unsigned)(now - idle_since > sd.idle_timeout_ticks));
     if (stop || timeout) break;
@@ -346,7 +350,8 @@ void serve_kernel(ServerDev sd) {
       int state = kIdle;
       if (!has_ticket) has_ticket = claim_ticket(sd, secondary, ticket);
       if (has_ticket &&
-          ticket < __hip_atomic_load(sd.wpub, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)) {
+          ticket < __hip_atomic_load(sd.wpub, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+        acquire_agent();  // the item and its request's table entry
         const uint64_t e = __hip_atomic_load(&sd.witems[ticket & (kWorkRing - 1)], __ATOMIC_RELAXED,
                                              __HIP_MEMORY_SCOPE_AGENT);
         const uint64_t c = e >> kItemBits;
@@ -413,11 +418,13 @@ void serve_kernel(ServerDev sd) {
         // next request in it, which the host submits only after reading the
         // completion record written below
         __hip_atomic_store(done, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __threadfence_system();
         Completion *cp = &sd.comps[r % (uint64_t)sd.nslots];
         __hip_atomic_store(&cp->gpu_time, wall_clock64(), __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_SYSTEM);
-        __hip_atomic_store(&cp->seq, r, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+        __threadfence_system();
+        // ordered after the outputs, the counter reset and gpu_time by the
+        // system fence above
+        __hip_atomic_store(&cp->seq, r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
       }
     }
     __syncthreads();
