@@ -409,8 +409,10 @@ void serve_kernel(ServerDev sd) {
     }
     if (threadIdx.x == 0) {
       const uint64_t r = s_req;
-      // this item's outputs reach memory before it is counted
-      __threadfence();
+      // this item's outputs reach memory before it is counted (release only:
+      // the worker that counts the last item takes the acquire side from the
+      // system fence below)
+      release_agent();
       unsigned *done = &sd.done[r % (uint64_t)sd.nslots];
       const unsigned fin = atomicAdd(done, 1u);
       if ((int64_t)fin + 1 == J.item_begin) {
