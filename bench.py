@@ -185,11 +185,12 @@ def c5_requests(count: int, seed: int, max_batch: int = 4096):
     return rng, ns, batches, c64
 
 
-def run_batched(fo, steps: int, warmup: int, njobs: int = 64):
-    """Mixed-size plan (C5 job mix) as ONE persistent launch from a hipGraph,
-    against the same jobs issued as one fftoc_fft call each."""
+def run_batched(fo, steps: int, warmup: int, njobs: int = 64, max_batch: int = 4096):
+    """Mixed-size plan (C5 job mix, batch 1..max_batch) as ONE persistent launch
+    from a hipGraph, against the same jobs issued as one fftoc_fft call each
+    (from this Python process, so each call also carries its host overhead)."""
     from fft_onchip_amd import hip
-    rng, ns, batches, c64 = c5_requests(njobs, seed=13)
+    rng, ns, batches, c64 = c5_requests(njobs, seed=13, max_batch=max_batch)
     bufs, jobs, nsig, nbytes = [], [], 0, 0
     for n, b, c in zip(ns, batches, c64):
         es = 8 if c else 16
@@ -220,7 +221,7 @@ def run_batched(fo, steps: int, warmup: int, njobs: int = 64):
     plan.close()
     for b in bufs:
         b.free()
-    return {"jobs": njobs, "signals": nsig, "bytes": nbytes,
+    return {"jobs": njobs, "max_batch": max_batch, "signals": nsig, "bytes": nbytes,
             "graph_ms": round(ms_graph, 4), "graph_ffts_per_s": nsig / (ms_graph * 1e-3),
             "graph_GBps": round(nbytes / (ms_graph * 1e-3) / 1e9, 1),
             "per_call_ms": round(ms_plain, 4), "per_call_ffts_per_s": nsig / (ms_plain * 1e-3),
@@ -589,6 +590,9 @@ def main():
     if not args.quick:
         extra["precision_c4"], prec_samples = run_precision(fo, args.steps, args.warmup)
         extra["batched_mixed_c5"] = run_batched(fo, args.steps, args.warmup)
+        # many small jobs: where one launch saves the per-call overheads
+        extra["batched_small_jobs"] = run_batched(fo, args.steps, args.warmup, njobs=1024,
+                                                  max_batch=16)
         print(f"[bench] batched plan: {extra['batched_mixed_c5']}", file=sys.stderr, flush=True)
         extra["served_c5"] = run_served(fo)
         print(f"[bench] served: {extra['served_c5']}", file=sys.stderr, flush=True)

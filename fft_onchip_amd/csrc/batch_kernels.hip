@@ -129,16 +129,23 @@ __global__ __launch_bounds__(kBatchThreads) __attribute__((amdgpu_waves_per_eu(2
   __shared__ c2 lds[kLdsElems];
   __shared__ long long s_item;
   int jcur = 0;
+  // Thread 0 claims the next item while the current one runs, so the claim's
+  // round trip to the counter is hidden behind the tile work.
+  long long next = 0;
+  if (threadIdx.x == 0) next = (long long)atomicAdd(&ctr[0], 1ull);
   for (;;) {
-    if (threadIdx.x == 0) s_item = (long long)atomicAdd(&ctr[0], 1ull);
+    if (threadIdx.x == 0) s_item = next;
     __syncthreads();
     const int64_t item = s_item;
-    __syncthreads();  // s_item is read by all before thread 0 claims again
+    __syncthreads();  // s_item is read by all before thread 0 writes it again
     if (item >= total_items) break;
     while (jcur + 1 < njobs && jobs[jcur + 1].item_begin <= item) ++jcur;
     const JobDev &J = jobs[jcur];
     const int64_t t0 = (item - J.item_begin) * J.tiles_per_item;
     const int64_t t1 = (t0 + J.tiles_per_item < J.ntiles) ? t0 + J.tiles_per_item : J.ntiles;
+    // issued after the job lookup: a wait for that lookup's loads would also
+    // wait for this older atomic
+    if (threadIdx.x == 0) next = (long long)atomicAdd(&ctr[0], 1ull);
     for (int64_t t = t0; t < t1; ++t) {
       if (J.c64)
         run_routine<f2>(J.routine, J, t, reinterpret_cast<f2 *>(lds), tw32);
