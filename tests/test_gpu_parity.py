@@ -1,7 +1,7 @@
 """GPU parity: every path and size against the long-double oracle.
 
-Tolerance (FP64): relative L2 error per signal <= 1e-13, the bar of the
-north-star accuracy target; observed errors are around 1e-16.
+Tolerance (FP64): relative L2 error per signal <= 1e-13, this path's
+accuracy target; observed errors are around 1e-16.
 """
 import numpy as np
 import pytest
