@@ -237,8 +237,7 @@ __device__ void dispatcher(const ServerDev &sd) {
   const uint64_t ns = (uint64_t)sd.nslots;
   for (;;) {
     uint64_t pub = ld_sys_relaxed(&sd.ring->published);
-    pub = ((uint64_t)__builtin_amdgcn_readfirstlane((unsigned)(pub >> 32)) << 32) |
-          (uint64_t)__builtin_amdgcn_readfirstlane((unsigned)pub);
+    pub = uniform_u64(pub);
     if (pub > dpub) {
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "");  // system scope: the host's descriptors
       const uint64_t cnt = (pub - dpub) < 64 ? (pub - dpub) : 64;

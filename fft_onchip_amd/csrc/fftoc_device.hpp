@@ -211,6 +211,15 @@ __device__ __forceinline__ void st_stream(CT *p, CT v) {
   __builtin_nontemporal_store(t, reinterpret_cast<vec2 *>(p));
 }
 
+// A wave-uniform 64-bit value in scalar registers.  readfirstlane returns a
+// signed 32-bit int: each half goes through uint32_t, or the low half would
+// sign-extend into the high one when its bit 31 is set.
+__device__ __forceinline__ uint64_t uniform_u64(uint64_t v) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((unsigned)v);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+  return ((uint64_t)hi << 32) | lo;
+}
+
 // Wave-scope ordering point for LDS exchanges among the lanes of one wave:
 // LDS operations of a wave execute in issue order, so only compiler
 // reordering has to be prevented.
