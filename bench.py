@@ -426,6 +426,12 @@ COPY_KERNEL = "__amd_rocclr_copyBuffer"
 HEADLINE = "fp64_c2c_fwd_n4096_b16384"
 
 
+def headline_variant(fo, wl) -> str:
+    """Name of the kernel variant fftoc_fft runs for the workload."""
+    lib = fo.lib()
+    return lib.fftoc_variant_name(wl.n, lib.fftoc_get_variant(wl.n, wl.batch)).decode()
+
+
 def load_traffic(workload: str):
     """HBM bytes per launch of the headline kernel, from the committed
     rocprofv3 --pmc summary (profiles/traffic.json, written by --traffic from
@@ -615,7 +621,7 @@ def main():
             "frac": round(res["hbm_frac"], 4),
             "traffic": round(traffic["bytes"]) if traffic else None,
             "traffic_detail": traffic,
-            "kernel": "fftoc_fft N=4096 = radix:16x16x16/tile/S1 (3 radix-16 Stockham passes, one signal per 256-thread workgroup)",
+            "kernel": "fftoc_fft N=4096 = " + headline_variant(fo, head),
             "algorithmic_bytes_per_launch": 32 * head.n * head.batch,
             "launch_ms_mean": round(res["launch_ms_mean"], 5)}
     line = {

@@ -154,7 +154,7 @@ const char *default_best(int n, int bucket) {
     case 512: return "radix:8x8x8/tile/S1";
     case 1024: return "radix:8x8x16/tile/S2";
     case 2048: return "radix:8x16x16/tile/S1";
-    default: return "radix:16x16x16/tile/S1";
+    default: return "radix:8x8x8x8/pipelined/S1";
   }
 }
 

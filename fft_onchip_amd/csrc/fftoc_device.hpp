@@ -158,28 +158,40 @@ __device__ __forceinline__ void dft(CT *a) {
   }
 }
 
-// Multiplies a[m] (m = 1..R-1) by W^{m k}, where tw[idx * m] = W^{m k}: only
-// W^k, W^2k, W^4k, W^8k are read from the table, the other powers are
-// products of at most four table values (error a few ulp).
+// The table values one radix-R butterfly's twiddles are built from: W^k,
+// W^2k, W^4k, W^8k (as many as R needs), read at idx, 2 idx, 4 idx, 8 idx.
 template <int R, class CT>
-__device__ __forceinline__ void apply_twiddles(CT *a, const CT *__restrict__ tw, int idx) {
+struct TwBase {
+  static constexpr int K = R == 2 ? 1 : R == 4 ? 2 : R == 8 ? 3 : 4;
+  CT w[K];
+  __device__ __forceinline__ void fetch(const CT *__restrict__ tw, int idx) {
+#pragma unroll
+    for (int b = 0; b < K; ++b) w[b] = tw[idx << b];
+  }
+};
+
+// Multiplies a[m] (m = 1..R-1) by W^{m k} given the base values: the powers
+// that are not in the table are products of at most four of them (error a
+// few ulp).
+template <int R, class CT>
+__device__ __forceinline__ void apply_twiddle_base(CT *a, const TwBase<R, CT> &b) {
   if constexpr (R == 2) {
-    a[1] = cmul(a[1], tw[idx]);
+    a[1] = cmul(a[1], b.w[0]);
   } else {
-    const CT w1 = tw[idx], w2 = tw[2 * idx];
+    const CT w1 = b.w[0], w2 = b.w[1];
     const CT w3 = cmul(w2, w1);
     a[1] = cmul(a[1], w1);
     a[2] = cmul(a[2], w2);
     a[3] = cmul(a[3], w3);
     if constexpr (R >= 8) {
-      const CT w4 = tw[4 * idx];
+      const CT w4 = b.w[2];
       a[4] = cmul(a[4], w4);
       a[5] = cmul(a[5], cmul(w4, w1));
       a[6] = cmul(a[6], cmul(w4, w2));
       const CT w7 = cmul(w4, w3);
       a[7] = cmul(a[7], w7);
       if constexpr (R == 16) {
-        const CT w8 = tw[8 * idx];
+        const CT w8 = b.w[3];
         a[8] = cmul(a[8], w8);
         a[9] = cmul(a[9], cmul(w8, w1));
         a[10] = cmul(a[10], cmul(w8, w2));
@@ -191,6 +203,14 @@ __device__ __forceinline__ void apply_twiddles(CT *a, const CT *__restrict__ tw,
       }
     }
   }
+}
+
+// Multiplies a[m] (m = 1..R-1) by W^{m k}, where tw[idx * m] = W^{m k}.
+template <int R, class CT>
+__device__ __forceinline__ void apply_twiddles(CT *a, const CT *__restrict__ tw, int idx) {
+  TwBase<R, CT> b;
+  b.fetch(tw, idx);
+  apply_twiddle_base<R>(a, b);
 }
 
 // Streaming (non-temporal) global access for data touched exactly once:

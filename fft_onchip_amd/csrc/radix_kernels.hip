@@ -19,8 +19,12 @@
 //  * Streaming data uses non-temporal loads and stores.
 //  * Persistent variant: a workgroup loops over tiles and issues the HBM
 //    loads of its next tile right after the current tile's first pass has
-//    gone to LDS, so the next tile's reads are in flight while the current
-//    tile runs its LDS passes (workgroup barriers only wait for LDS traffic).
+//    gone to LDS.  Pipelined variant (StockhamKernel::run_pipelined_persistent):
+//    the same, with each lane's base twiddles loaded once before the loop, so
+//    the later passes issue no vector-memory load (the vector-memory counter
+//    retires in order: a twiddle load issued after the prefetch would make
+//    its pass wait for the prefetch), and the next tile's first pass runs
+//    before the current tile's stores; the N = 4096 default.
 //  * Between passes the data crosses lanes through LDS.  When a signal lives
 //    inside one wave (T <= 64) no workgroup barrier is needed: the LDS
 //    operations of a wave are executed in order.
@@ -48,6 +52,13 @@ __global__ __launch_bounds__((kThreads<SP, S>)) __attribute__((amdgpu_waves_per_
 void stockham_persistent_kernel(
     const c2 *in, c2 *out, int64_t batch, const c2 *__restrict__ tw, double scale) {
   StockhamKernel<SP, S, INV, false>::run_persistent(in, out, batch, tw, scale);
+}
+
+template <class SP, int S, bool INV>
+__global__ __launch_bounds__((kThreads<SP, S>)) __attribute__((amdgpu_waves_per_eu(2)))
+void stockham_pipelined_kernel(
+    const c2 *in, c2 *out, int64_t batch, const c2 *__restrict__ tw, double scale) {
+  StockhamKernel<SP, S, INV, false>::run_pipelined_persistent(in, out, batch, tw, scale);
 }
 
 // N = 8 / 16 with the butterflies across lanes (lane_fft.hpp): each wave
@@ -82,7 +93,7 @@ int launch_lane(const LaunchArgs &a) {
 #define FFTOC_LANE(NAME, FAM, N, C, CT) \
   Variant { NAME, FAM, &launch_lane<N, C, CT>, kLaneThreads * (C) / (N) }
 
-enum class Form { OneTile, Staged, Persistent };
+enum class Form { OneTile, Staged, Persistent, Pipelined };
 
 template <class SP, int S, Form F, class CT = c2>
 int launch_split(const LaunchArgs &a) {
@@ -102,6 +113,16 @@ int launch_split(const LaunchArgs &a) {
                          a.stream, in, out, a.batch, tw, a.scale);
     else
       hipLaunchKernelGGL((stockham_persistent_kernel<SP, S, false>), dim3(grid), dim3(threads), 0,
+                         a.stream, in, out, a.batch, tw, a.scale);
+  } else if constexpr (F == Form::Pipelined) {
+    static_assert(sizeof(CT) == 16, "pipelined form is FP64 only");
+    const int64_t resident = resident_workgroups<stockham_pipelined_kernel<SP, S, false>, threads>();
+    const unsigned grid = (unsigned)(ntiles < resident ? ntiles : resident);
+    if (a.inverse)
+      hipLaunchKernelGGL((stockham_pipelined_kernel<SP, S, true>), dim3(grid), dim3(threads), 0,
+                         a.stream, in, out, a.batch, tw, a.scale);
+    else
+      hipLaunchKernelGGL((stockham_pipelined_kernel<SP, S, false>), dim3(grid), dim3(threads), 0,
                          a.stream, in, out, a.batch, tw, a.scale);
   } else {
     constexpr bool staged = (F == Form::Staged);
@@ -157,18 +178,21 @@ const Variant kV64[] = {
     FFTOC_V("radix:16x4/tile/S16", MIX, Split<16, 4>, 16, Form::OneTile),
     FFTOC_LANE("radix:64/lanes/S16", MIX, 64, 4, c2),
     FFTOC_LANE("radix:64/lanes/S32", MIX, 64, 8, c2),
+    FFTOC_V("radix:8x8/pipelined/S32", R8, Split<8, 8>, 32, Form::Pipelined),
 };
 const Variant kV128[] = {
     FFTOC_V("radix:16x8/tile/S16", MIX, Split<16, 8>, 16, Form::OneTile),
     FFTOC_V("radix:16x8/tile/S8", MIX, Split<16, 8>, 8, Form::OneTile),
     FFTOC_V("radix:16x8/persistent/S16", MIX, Split<16, 8>, 16, Form::Persistent),
     FFTOC_V("radix:8x16/tile/S16", MIX, Split<8, 16>, 16, Form::OneTile),
+    FFTOC_V("radix:8x4x4/pipelined/S16", MIX, Split<8, 4, 4>, 16, Form::Pipelined),
 };
 const Variant kV256[] = {
     FFTOC_V("radix:16x16/tile/S4", R16, Split<16, 16>, 4, Form::OneTile),
     FFTOC_V("radix:16x16/tile/S8", R16, Split<16, 16>, 8, Form::OneTile),
     FFTOC_V("radix:16x16/persistent/S8", R16, Split<16, 16>, 8, Form::Persistent),
     FFTOC_V("radix:8x8x4/tile/S8", MIX, Split<8, 8, 4>, 8, Form::OneTile),
+    FFTOC_V("radix:8x8x4/pipelined/S8", MIX, Split<8, 8, 4>, 8, Form::Pipelined),
 };
 const Variant kV512[] = {
     FFTOC_V("radix:8x8x8/tile/S1", R8, Split<8, 8, 8>, 1, Form::OneTile),
@@ -178,6 +202,7 @@ const Variant kV512[] = {
     FFTOC_V("radix:16x16x2/tile/S4", MIX, Split<16, 16, 2>, 4, Form::OneTile),
     FFTOC_V("radix:16x16x2/persistent/S4", MIX, Split<16, 16, 2>, 4, Form::Persistent),
     FFTOC_V("radix:16x16x2/tile/S2", MIX, Split<16, 16, 2>, 2, Form::OneTile),
+    FFTOC_V("radix:8x8x8/pipelined/S4", R8, Split<8, 8, 8>, 4, Form::Pipelined),
 };
 const Variant kV1024[] = {
     FFTOC_V("radix:16x16x4/persistent/S2", MIX, Split<16, 16, 4>, 2, Form::Persistent),
@@ -185,16 +210,19 @@ const Variant kV1024[] = {
     FFTOC_V("radix:16x16x4/tile/S1", MIX, Split<16, 16, 4>, 1, Form::OneTile),
     FFTOC_V("radix:8x8x16/tile/S2", MIX, Split<8, 8, 16>, 2, Form::OneTile),
     FFTOC_V("radix:4x16x16/tile/S2", MIX, Split<4, 16, 16>, 2, Form::OneTile),
+    FFTOC_V("radix:8x8x8x2/pipelined/S1", MIX, Split<8, 8, 8, 2>, 1, Form::Pipelined),
 };
 const Variant kV2048[] = {
     FFTOC_V("radix:16x16x8/persistent/S1", MIX, Split<16, 16, 8>, 1, Form::Persistent),
     FFTOC_V("radix:16x16x8/tile/S1", MIX, Split<16, 16, 8>, 1, Form::OneTile),
     FFTOC_V("radix:16x8x16/persistent/S1", MIX, Split<16, 8, 16>, 1, Form::Persistent),
     FFTOC_V("radix:8x16x16/tile/S1", MIX, Split<8, 16, 16>, 1, Form::OneTile),
+    FFTOC_V("radix:8x8x8x4/pipelined/S1", MIX, Split<8, 8, 8, 4>, 1, Form::Pipelined),
 };
 const Variant kV4096[] = {
     FFTOC_V("radix:16x16x16/tile/S1", R16, Split<16, 16, 16>, 1, Form::OneTile),
     FFTOC_V("radix:16x16x16/persistent/S1", R16, Split<16, 16, 16>, 1, Form::Persistent),
+    FFTOC_V("radix:8x8x8x8/pipelined/S1", R8, Split<8, 8, 8, 8>, 1, Form::Pipelined),
     FFTOC_V("radix:8x8x8x8/tile/S1", R8, Split<8, 8, 8, 8>, 1, Form::OneTile),
     FFTOC_V("radix:8x8x8x8/persistent/S1", R8, Split<8, 8, 8, 8>, 1, Form::Persistent),
 };

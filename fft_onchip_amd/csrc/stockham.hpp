@@ -233,6 +233,156 @@ struct StockhamKernel {
     run_tile(in, out, batch, blockIdx.x, lds, tw, scale);
   }
 
+  // ---- pieces of a pass, for the pipelined persistent form below --------
+
+  static constexpr int kK(int r) { return r == 2 ? 1 : r == 4 ? 2 : r == 8 ? 3 : 4; }
+  // base twiddles of passes 1 .. P-1, per pass and group, in one array
+  static constexpr int kTwOff(int p) {
+    int o = 0;
+    for (int q = 1; q < p; ++q) o += (RM / SP::R[q]) * kK(SP::R[q]);
+    return o;
+  }
+  static constexpr int kTwCount = kTwOff(SP::P);
+
+  template <int p>
+  __device__ __forceinline__ static void fetch_pass_tw(CT *hw, const CT *__restrict__ tw, int t) {
+    constexpr int R = SP::R[p];
+    constexpr int L = SP::L(p);
+    constexpr int NB = RM / R;
+    constexpr int step = kTwN / (R * L);
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+      TwBase<R, CT> b;
+      b.fetch(tw, ((t + i * T) & (L - 1)) * step);
+#pragma unroll
+      for (int w = 0; w < kK(R); ++w) hw[kTwOff(p) + i * kK(R) + w] = b.w[w];
+    }
+  }
+  template <int... ps>
+  __device__ __forceinline__ static void fetch_all_tw(std::integer_sequence<int, ps...>, CT *hw,
+                                      const CT *__restrict__ tw, int t) {
+    (fetch_pass_tw<ps + 1>(hw, tw, t), ...);
+  }
+
+  // LDS -> registers (the pass's inputs), then an ordering point
+  template <int p>
+  __device__ __forceinline__ static void gather(CT (&reg)[RM], const CT *lds, int t, int sbase) {
+    constexpr int R = SP::R[p];
+    constexpr int NB = RM / R;
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+      const int j = t + i * T;
+#pragma unroll
+      for (int m = 0; m < R; ++m) reg[i * R + m] = lds[at(sbase + j + m * (N / R))];
+    }
+    sync();  // every lane has read before anyone overwrites
+  }
+  // twiddles (base values from hw; none in pass 0) and butterflies
+  template <int p>
+  __device__ __forceinline__ static void butterflies(CT (&reg)[RM], const CT *hw) {
+    constexpr int R = SP::R[p];
+    constexpr int NB = RM / R;
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+      if constexpr (p > 0) {
+        TwBase<R, CT> b;
+#pragma unroll
+        for (int w = 0; w < kK(R); ++w) b.w[w] = hw[kTwOff(p) + i * kK(R) + w];
+        apply_twiddle_base<R>(&reg[i * R], b);
+      }
+      dft<R>(&reg[i * R]);
+    }
+  }
+  template <int p>
+  __device__ __forceinline__ static void scatter_lds(const CT (&reg)[RM], CT *lds, int t, int sbase) {
+    constexpr int R = SP::R[p];
+    constexpr int L = SP::L(p);
+    constexpr int NB = RM / R;
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+      const int j = t + i * T;
+      const int k = j & (L - 1);
+#pragma unroll
+      for (int q = 0; q < R; ++q) lds[at(sbase + (j - k) * R + k + q * L)] = reg[i * R + q];
+    }
+    sync();
+  }
+  __device__ __forceinline__ static void scatter_out(const CT (&reg)[RM], CT *gout, int t, bool valid, R_ scale) {
+    constexpr int p = SP::P - 1;
+    constexpr int R = SP::R[p];
+    constexpr int L = SP::L(p);
+    constexpr int NB = RM / R;
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+      const int j = t + i * T;
+      const int k = j & (L - 1);
+#pragma unroll
+      for (int q = 0; q < R; ++q) {
+        CT v = reg[i * R + q];
+        if (scale != 1.0) v = cscale(v, scale);
+        if (swap_io(false)) v = cswap(v);
+        if (valid) st_stream(gout + (j - k) * R + k + q * L, v);
+      }
+    }
+  }
+  template <int... ps>
+  __device__ __forceinline__ static void middle_passes(std::integer_sequence<int, ps...>, CT (&reg)[RM], CT *lds,
+                                       int t, int sbase, const CT *hw) {
+    ((gather<ps + 1>(reg, lds, t, sbase), butterflies<ps + 1>(reg, hw),
+      scatter_lds<ps + 1>(reg, lds, t, sbase)),
+     ...);
+  }
+
+  // Persistent, pipelined: tiles blockIdx.x, blockIdx.x + gridDim.x, ...  The
+  // base twiddles are read once, before the loop, so the passes after the
+  // first issue no vector-memory loads: the next tile's loads, issued right
+  // after the current tile's first pass, then overlap all of its later passes
+  // (the vector-memory counter retires in order, so any load issued after
+  // them -- a twiddle read -- would wait for them).  The next tile's first
+  // pass runs before the current tile's stores are issued, so its wait for
+  // those loads does not wait for these stores either.
+  __device__ __forceinline__ static void run_pipelined_persistent(const CT *in, CT *out, int64_t batch,
+                                                  const CT *__restrict__ tw, R_ scale) {
+    static_assert(!STAGED && SP::P >= 2 && INVM != 2, "pipelined form needs an LDS pass");
+    __shared__ CT lds[lds_elems(S * N)];
+    const int tid = threadIdx.x;
+    const int t = tid % T;
+    const int sl = tid / T;
+    const int sbase = sl * N;
+    const int64_t ntiles = (batch + S - 1) / S;
+    int64_t tile = blockIdx.x;
+    if (tile >= ntiles) return;  // uniform over the workgroup
+    CT hw[kTwCount];
+    fetch_all_tw(std::make_integer_sequence<int, SP::P - 1>{}, hw, tw, t);
+    CT reg[RM];
+    {
+      const int64_t sig = tile * S + sl;
+      load_first(reg, in + sig * N, t, sig < batch, false);
+    }
+    butterflies<0>(reg, hw);
+    scatter_lds<0>(reg, lds, t, sbase);
+    for (;;) {
+      const int64_t sig = tile * S + sl;
+      const int64_t next = tile + gridDim.x;
+      const bool more = next < ntiles;  // uniform over the workgroup
+      CT nxt[RM];
+      if (more) {
+        const int64_t nsig = next * S + sl;
+        load_first(nxt, in + nsig * N, t, nsig < batch, false);
+      }
+      middle_passes(std::make_integer_sequence<int, SP::P - 2>{}, reg, lds, t, sbase, hw);
+      gather<SP::P - 1>(reg, lds, t, sbase);  // LDS free for the next tile after this
+      butterflies<SP::P - 1>(reg, hw);
+      if (more) {
+        butterflies<0>(nxt, hw);
+        scatter_lds<0>(nxt, lds, t, sbase);
+      }
+      scatter_out(reg, out + sig * N, t, sig < batch, scale);
+      if (!more) break;
+      tile = next;
+    }
+  }
+
   // Persistent workgroups: tiles blockIdx.x, blockIdx.x + gridDim.x, ...;
   // the next tile's HBM loads overlap the current tile's LDS passes.
   __device__ static void run_persistent(const CT *in, CT *out, int64_t batch,
