@@ -134,6 +134,34 @@ def test_every_variant_matches_oracle(n):
             assert err <= TOL, (n, name, inv, err)
 
 
+@pytest.mark.parametrize("n", fo.ALL_SIZES)
+def test_persistent_variants_over_several_grids(n):
+    """Persistent and pipelined variants at a batch several times what their
+    grid holds at once, so every workgroup loops and prefetches; out of place
+    and in place (the prefetch must read each next signal before any of it is
+    overwritten)."""
+    loops = [(v, name) for v, name in _variants(n) if "persistent" in name or "pipelined" in name]
+    if not loops:
+        pytest.skip(f"no persistent variant at n={n}")
+    x = signals(n, (1 << 23) // n + 7, seed=900 + n)
+    ref_f = oracle.reference(x)
+    ref_i = oracle.reference(x, inverse=True)
+    src = fo.DeviceBuffer.from_array(x)
+    dst = fo.DeviceBuffer(x.nbytes)
+    for v, name in loops:
+        for inv, ref in ((False, ref_f), (True, ref_i)):
+            for in_place in (False, True):
+                print(f"[persistent] n={n} v={v} {name} inverse={inv} in_place={in_place}", flush=True)
+                src.upload(x)  # an earlier in-place call overwrote it
+                out = src if in_place else dst
+                rc = fo.lib().fftoc_fft_variant(src.ptr, out.ptr, x.shape[0], n,
+                                                fo.INVERSE if inv else 0, v, None)
+                assert rc == 0, (name, rc)
+                fo.synchronize()
+                err = np.max(oracle.rel_l2(out.download(x.shape), ref, axis=-1))
+                assert err <= TOL, (n, name, inv, in_place, err)
+
+
 @pytest.mark.parametrize("n", [64, 256, 4096])
 def test_tile_flag_runs_a_variant_with_that_tile(n):
     """fftoc_fft / fftoc_tensor_fft with FFTOC_TILE(s) run a variant with s
