@@ -268,10 +268,17 @@ __device__ void dispatcher(const ServerDev &sd) {
         }
         const uint64_t total = __shfl(incl, 63);
         const uint64_t first = wp + incl - ni;
-        for (uint64_t i = 0; i < ni; ++i)
-          __hip_atomic_store(&sd.witems[(first + i) & (kWorkRing - 1)],
-                             (unsigned long long)((c << kItemBits) | i), __ATOMIC_RELAXED,
-                             __HIP_MEMORY_SCOPE_AGENT);
+        // the items of each request in turn, written by all lanes (a request
+        // of thousands of items takes tens of store instructions, not thousands)
+        for (int r = 0; r < k; ++r) {
+          const uint64_t cr = dpub + (uint64_t)r;
+          const uint64_t nr = __shfl((unsigned long long)ni, r);
+          const uint64_t fr = __shfl((unsigned long long)first, r);
+          for (uint64_t i = (uint64_t)lane; i < nr; i += 64)
+            __hip_atomic_store(&sd.witems[(fr + i) & (kWorkRing - 1)],
+                               (unsigned long long)((cr << kItemBits) | i), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        }
         release_agent();
         wp += total;
         dpub += (uint64_t)k;
