@@ -68,7 +68,7 @@ struct ServerDev {
                            // worker finishing the last one resets it
   int nslots;
   unsigned *cu_seats;      // [kCuKeys] workers seated per CU in this launch
-  int claim_backlog;       // unclaimed items above which secondary workers join
+  int claim_backlog;       // unassigned items above which secondary workers join
   uint64_t idle_timeout_ticks;  // wall_clock64 ticks without work before exit
   const void *tw64;
   const void *tw32;

@@ -313,10 +313,13 @@ __device__ inline unsigned cu_key() {
 
 // The first worker seated on a CU is its primary and takes work tickets in
 // order, holding one while it waits.  Any other worker on the CU is secondary:
-// it takes a ticket only while more published items wait unclaimed than there
-// are primaries, so at low load items run alone on their CUs and few workers
-// poll the queue, and under a backlog every slot works.  (Claiming single
-// published items by compare-and-swap instead starves the primaries' atomics.)
+// it takes a ticket only while published items wait unclaimed beyond the
+// tickets idle primaries hold, so at low load items run alone on their CUs and
+// few workers poll the queue, and under a backlog every slot works.  A worker
+// claims its next ticket only when it has finished an item: a ticket claimed
+// ahead would hold an item back from an idle worker until the current item is
+// done.  (Claiming single published items by compare-and-swap instead starves
+// the primaries' atomics.)
 __device__ inline bool claim_ticket(const ServerDev &sd, bool secondary, uint64_t &ticket) {
   if (secondary) {
     const uint64_t p = __hip_atomic_load(sd.wpub, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -411,8 +414,6 @@ void serve_kernel(ServerDev sd) {
     const int64_t item = s_item;
     const int64_t t0 = item * J.tiles_per_item;
     const int64_t t1 = (t0 + J.tiles_per_item < J.ntiles) ? t0 + J.tiles_per_item : J.ntiles;
-    if (threadIdx.x == 0 && !has_ticket)  // claimed before the work to hide its latency
-      has_ticket = claim_ticket(sd, secondary, ticket);
     for (int64_t t = t0; t < t1; ++t) {
       if (J.c64)
         run_routine<f2>(J.routine, J, t, reinterpret_cast<f2 *>(lds), static_cast<const f2 *>(sd.tw32));

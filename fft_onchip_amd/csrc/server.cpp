@@ -247,7 +247,9 @@ int fftoc_server_create(size_t hbm_budget_bytes, int max_inflight, fftoc_server 
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
       cus = 0;
     s->spread_workers = std::max(1, std::min(cus, s->grid - 1));
-    s->sd.claim_backlog = s->spread_workers;
+    // secondaries join as soon as any published item is left beyond the
+    // tickets idle primaries hold (a margin of 64 items measured worse)
+    s->sd.claim_backlog = 0;
   }
   if (rc == FFTOC_SUCCESS) rc = host_alloc<uint32_t>(&s->live, nullptr, &s->sd.live, s->grid);
   if (rc == FFTOC_SUCCESS) rc = host_alloc<HostRing>(&s->ring, nullptr, &ring_d, 1);
