@@ -360,6 +360,7 @@ void serve_kernel(ServerDev sd) {
   uint64_t ticket = 0;
   bool have = false;       // s_job holds request cached_c
   uint64_t cached_c = 0;
+  unsigned pending = 0;    // finished items of s_req not yet counted
   uint64_t idle_since = wall_clock64();
   for (;;) {
     if (threadIdx.x == 0) {
@@ -423,24 +424,45 @@ void serve_kernel(ServerDev sd) {
     }
     if (threadIdx.x == 0) {
       const uint64_t r = s_req;
-      // this item's outputs reach memory before it is counted (release only:
-      // the worker that counts the last item takes the acquire side from the
-      // system fence below)
-      release_agent();
-      unsigned *done = &sd.done[r % (uint64_t)sd.nslots];
-      const unsigned fin = atomicAdd(done, 1u);
-      if ((int64_t)fin + 1 == J.item_begin) {
-        // every item is counted: the slot's counter starts from zero for the
-        // next request in it, which the host submits only after reading the
-        // completion record written below
-        __hip_atomic_store(done, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        Completion *cp = &sd.comps[r % (uint64_t)sd.nslots];
-        __hip_atomic_store(&cp->gpu_time, wall_clock64(), __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_SYSTEM);
-        __threadfence_system();
-        // ordered after the outputs, the counter reset and gpu_time by the
-        // system fence above
-        __hip_atomic_store(&cp->seq, r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+      ++pending;
+      // A request with more items than there are workers: the next ticket is
+      // claimed now, and if its item is published and of the same request,
+      // this item is counted with that one (one release and one count for
+      // both: the release writes back this XCD's L2); the worker runs that
+      // item next, so the count is only delayed, never left.  (Smaller
+      // requests are counted at once: the claim's round trip would delay
+      // their completion.)
+      bool defer = false;
+      if (J.item_begin > (int64_t)gridDim.x - 1 && !has_ticket)
+        has_ticket = claim_ticket(sd, secondary, ticket);
+      if (J.item_begin > (int64_t)gridDim.x - 1 && has_ticket &&
+          ticket < __hip_atomic_load(sd.wpub, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+        acquire_agent();  // the item word
+        const uint64_t e = __hip_atomic_load(&sd.witems[ticket & (kWorkRing - 1)], __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT);
+        defer = (e >> kItemBits) == r;
+      }
+      if (!defer) {
+        // the counted items' outputs reach memory before they are counted
+        // (release only: the worker that counts the last item takes the
+        // acquire side from the system fence below)
+        release_agent();
+        unsigned *done = &sd.done[r % (uint64_t)sd.nslots];
+        const unsigned fin = atomicAdd(done, pending);
+        if ((int64_t)fin + pending == J.item_begin) {
+          // every item is counted: the slot's counter starts from zero for the
+          // next request in it, which the host submits only after reading the
+          // completion record written below
+          __hip_atomic_store(done, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          Completion *cp = &sd.comps[r % (uint64_t)sd.nslots];
+          __hip_atomic_store(&cp->gpu_time, wall_clock64(), __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_SYSTEM);
+          __threadfence_system();
+          // ordered after the outputs, the counter reset and gpu_time by the
+          // system fence above
+          __hip_atomic_store(&cp->seq, r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        }
+        pending = 0;
       }
     }
     __syncthreads();
