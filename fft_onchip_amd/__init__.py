@@ -205,6 +205,13 @@ class Plan:
             raise FftocError(rc, "fftoc_plan_routine")
         return name.value.decode()
 
+    def set_routine(self, n: int, name: str, c64: bool = False) -> None:
+        """Pins the tile routine of length n (one of batch_routines(n))."""
+        rc = lib().fftoc_plan_set_routine(self.handle, int(n), DATA_C64 if c64 else 0,
+                                          name.encode())
+        if rc != 0:
+            raise FftocError(rc, "fftoc_plan_set_routine")
+
     def close(self) -> None:
         if getattr(self, "handle", None):
             lib().fftoc_plan_destroy(self.handle)
@@ -215,6 +222,20 @@ class Plan:
             self.close()
         except Exception:
             pass
+
+
+def batch_routines(n: int) -> list:
+    """Names of the tile routines plans and servers can use for length n."""
+    names, i = [], 0
+    while True:
+        name = ctypes.c_char_p()
+        rc = lib().fftoc_batch_routine_name(int(n), i, ctypes.byref(name))
+        if rc == 13:  # FFTOC_ERROR_INVALID_HANDLE: past the last routine
+            return names
+        if rc != 0:
+            raise FftocError(rc, "fftoc_batch_routine_name")
+        names.append(name.value.decode())
+        i += 1
 
 
 class Server:

@@ -303,6 +303,37 @@ int fftoc_plan_routine(fftoc_plan p, int n, uint32_t flags, const char **name) {
   return FFTOC_SUCCESS;
 }
 
+int fftoc_plan_set_routine(fftoc_plan p, int n, uint32_t flags, const char *name) {
+  using namespace fftoc;
+  if (!p) return FFTOC_ERROR_INVALID_HANDLE;
+  if (!name) return FFTOC_ERROR_NULL_POINTER;
+  if (!pow2(n) || n < 8 || n > 4096) return FFTOC_ERROR_INVALID_SIZE;
+  const int rc = check_device(p);
+  if (rc != FFTOC_SUCCESS) return rc;
+  for (int id = 0; id < batch_routine_count(); ++id) {
+    if (batch_routine_n(id) == n && std::strcmp(batch_routine_name(id), name) == 0) {
+      p->routine[lg(n)][(flags & FFTOC_DATA_C64) ? 1 : 0] = id;
+      return layout(p);
+    }
+  }
+  return FFTOC_ERROR_UNSUPPORTED;
+}
+
+int fftoc_batch_routine_name(int n, int index, const char **name) {
+  using namespace fftoc;
+  if (!name) return FFTOC_ERROR_NULL_POINTER;
+  if (!pow2(n) || n < 8 || n > 4096) return FFTOC_ERROR_INVALID_SIZE;
+  int k = 0;
+  for (int id = 0; id < batch_routine_count(); ++id) {
+    if (batch_routine_n(id) != n) continue;
+    if (k++ == index) {
+      *name = batch_routine_name(id);
+      return FFTOC_SUCCESS;
+    }
+  }
+  return FFTOC_ERROR_INVALID_HANDLE;
+}
+
 int fftoc_plan_destroy(fftoc_plan p) {
   if (!p) return FFTOC_ERROR_INVALID_HANDLE;
   int cur = 0;

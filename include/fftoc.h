@@ -191,6 +191,15 @@ int fftoc_plan_execute_graph(fftoc_plan plan, void *stream);
 int fftoc_plan_autotune(fftoc_plan plan, int reps, void *stream);
 /* Name of the tile routine the plan uses for (n, data type of flags). */
 int fftoc_plan_routine(fftoc_plan plan, int n, uint32_t flags, const char **name);
+/* Pins the plan's tile routine for (n, data type of flags) to the named one
+ * (FFTOC_ERROR_UNSUPPORTED for a name that is not a routine of length n), as
+ * a persisted choice of fftoc_plan_autotune would be restored.  The
+ * reference fixes one configuration per compiled kernel
+ * (scripts/generate_bench.py:93-107); here the choice is made at run time. */
+int fftoc_plan_set_routine(fftoc_plan plan, int n, uint32_t flags, const char *name);
+/* index-th tile routine of length n, in table order (FFTOC_ERROR_INVALID_HANDLE
+ * past the last one). */
+int fftoc_batch_routine_name(int n, int index, const char **name);
 int fftoc_plan_destroy(fftoc_plan plan);
 
 /* ---- serving: device-resident request loop with HBM-budget admission -----

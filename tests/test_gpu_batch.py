@@ -158,6 +158,39 @@ def test_plan_autotune_keeps_results_correct():
         check_job(j, got)
 
 
+@pytest.mark.parametrize("n", fo.ALL_SIZES)
+def test_every_plan_routine_matches_oracle(n):
+    """Each tile routine of length n, pinned with fftoc_plan_set_routine, for
+    both data types: a forward job and a normalised inverse job of several
+    work items each, the last tile partial."""
+    routines = fo.batch_routines(n)
+    assert routines, n
+    batch = (1 << 20) // n + 3
+    for c64 in (False, True):
+        x = signals(n, batch, seed=700 + n, c64=c64)
+        ref_f = oracle.reference(x.astype(np.complex128))
+        ref_i = oracle.reference(x.astype(np.complex128), inverse=True)
+        ref_i = oracle.DD(ref_i.hi / n, ref_i.lo / n)
+        src = fo.DeviceBuffer.from_array(x)
+        outs = (fo.DeviceBuffer(x.nbytes), fo.DeviceBuffer(x.nbytes))
+        dt = fo.DATA_C64 if c64 else 0
+        plan = fo.Plan([(src, outs[0], batch, n, dt),
+                        (src, outs[1], batch, n, dt | fo.INVERSE | fo.NORMALIZE)])
+        tol = TOL32 if c64 else TOL64
+        for name in routines:
+            print(f"[routine] n={n} c64={c64} {name}", flush=True)
+            plan.set_routine(n, name, c64=c64)
+            assert plan.routine(n, c64=c64) == name
+            plan.execute()
+            fo.synchronize()
+            for o, ref in zip(outs, (ref_f, ref_i)):
+                got = o.download(x.shape, dtype=x.dtype)
+                assert np.max(oracle.rel_l2(got, ref, axis=-1)) <= tol, (n, c64, name)
+        with pytest.raises(fo.FftocError):
+            plan.set_routine(n, "no such routine", c64=c64)
+        plan.close()
+
+
 def test_plan_rejects_bad_jobs():
     a = fo.DeviceBuffer(64 * 16)
     with pytest.raises(fo.FftocError) as e:
