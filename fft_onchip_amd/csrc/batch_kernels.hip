@@ -136,21 +136,29 @@ __global__ __launch_bounds__(kBatchThreads) __attribute__((amdgpu_waves_per_eu(2
   for (;;) {
     if (threadIdx.x == 0) s_item = next;
     __syncthreads();
-    const int64_t item = s_item;
+    // readfirstlane: the value is the same in every lane, and the compiler
+    // then knows it, so the loop exit below is a uniform branch (a loop it
+    // took to exit per lane would run this loop's barriers on split lanes)
+    const int64_t item = (int64_t)uniform_u64((uint64_t)s_item);
     __syncthreads();  // s_item is read by all before thread 0 writes it again
     if (item >= total_items) break;
     while (jcur + 1 < njobs && jobs[jcur + 1].item_begin <= item) ++jcur;
     const JobDev &J = jobs[jcur];
-    const int64_t t0 = (item - J.item_begin) * J.tiles_per_item;
-    const int64_t t1 = (t0 + J.tiles_per_item < J.ntiles) ? t0 + J.tiles_per_item : J.ntiles;
+    // the tile loop holds barriers: its bounds and the routine made uniform
+    const int64_t tpi = __builtin_amdgcn_readfirstlane(J.tiles_per_item);
+    const int64_t ntiles = (int64_t)uniform_u64((uint64_t)J.ntiles);
+    const int64_t t0 = (item - (int64_t)uniform_u64((uint64_t)J.item_begin)) * tpi;
+    const int64_t t1 = (t0 + tpi < ntiles) ? t0 + tpi : ntiles;
+    const int routine = __builtin_amdgcn_readfirstlane(J.routine);
+    const bool c64 = __builtin_amdgcn_readfirstlane(J.c64) != 0;
     // issued after the job lookup: a wait for that lookup's loads would also
     // wait for this older atomic
     if (threadIdx.x == 0) next = (long long)atomicAdd(&ctr[0], 1ull);
     for (int64_t t = t0; t < t1; ++t) {
-      if (J.c64)
-        run_routine<f2>(J.routine, J, t, reinterpret_cast<f2 *>(lds), tw32);
+      if (c64)
+        run_routine<f2>(routine, J, t, reinterpret_cast<f2 *>(lds), tw32);
       else
-        run_routine<c2>(J.routine, J, t, lds, tw64);
+        run_routine<c2>(routine, J, t, lds, tw64);
       __syncthreads();  // LDS is reused by the next tile
     }
   }
@@ -401,7 +409,7 @@ void serve_kernel(ServerDev sd) {
       s_state = state;
     }
     __syncthreads();
-    const int state = s_state;
+    const int state = __builtin_amdgcn_readfirstlane(s_state);  // uniform branches below
     if (state == kExit) break;
     if (state != kWork) {
       if (state == kIdleNoTicket)  // a secondary looks for unclaimed items less often
@@ -412,14 +420,20 @@ void serve_kernel(ServerDev sd) {
       continue;
     }
     const JobDev J = s_job;
-    const int64_t item = s_item;
-    const int64_t t0 = item * J.tiles_per_item;
-    const int64_t t1 = (t0 + J.tiles_per_item < J.ntiles) ? t0 + J.tiles_per_item : J.ntiles;
+    // the tile loop's bounds and the routine, read from LDS, made uniform:
+    // the loop holds barriers, so its branches must be the same in every lane
+    const int64_t item = (int64_t)uniform_u64((uint64_t)s_item);
+    const int64_t tpi = __builtin_amdgcn_readfirstlane(J.tiles_per_item);
+    const int64_t ntiles = (int64_t)uniform_u64((uint64_t)J.ntiles);
+    const int routine = __builtin_amdgcn_readfirstlane(J.routine);
+    const bool c64 = __builtin_amdgcn_readfirstlane(J.c64) != 0;
+    const int64_t t0 = item * tpi;
+    const int64_t t1 = (t0 + tpi < ntiles) ? t0 + tpi : ntiles;
     for (int64_t t = t0; t < t1; ++t) {
-      if (J.c64)
-        run_routine<f2>(J.routine, J, t, reinterpret_cast<f2 *>(lds), static_cast<const f2 *>(sd.tw32));
+      if (c64)
+        run_routine<f2>(routine, J, t, reinterpret_cast<f2 *>(lds), static_cast<const f2 *>(sd.tw32));
       else
-        run_routine<c2>(J.routine, J, t, lds, static_cast<const c2 *>(sd.tw64));
+        run_routine<c2>(routine, J, t, lds, static_cast<const c2 *>(sd.tw64));
       __syncthreads();
     }
     if (threadIdx.x == 0) {
