@@ -449,10 +449,29 @@ __device__ __forceinline__ void load512(const c2 *in, int64_t s, int g, int j, c
 // Transforms signal s from its operands and stores it (if s < batch) through
 // the wave's 512-entry LDS buffer; `reuse`: the buffer still serves the
 // previous unit's stores.
-template <bool INV, bool SC>
-__device__ __forceinline__ void unit512(const c2 (&a)[2][4], const Frag &f,
-                                        const c2 *__restrict__ tw, c2 *buf, c2 *out, int64_t s,
-                                        int64_t batch, double scale, int lane, bool reuse) {
+// Twiddles of unit512 from the 4096-entry table in HBM.
+struct Tw512Global {
+  const c2 *__restrict__ tw;
+  __device__ c2 w512(int k) const { return tw[k << 3]; }   // W_512^k
+  __device__ c2 w32(int m) const { return tw[m << 7]; }    // W_32^m
+};
+// The same values copied into LDS (fill_tw512_lds): their reads count on the
+// LDS counter only, so waiting for one does not wait for HBM loads in flight.
+struct Tw512Lds {
+  const c2 *t;  // [512] W_512^k, then [32] W_32^m
+  __device__ c2 w512(int k) const { return t[k]; }
+  __device__ c2 w32(int m) const { return t[512 + m]; }
+};
+constexpr int kTw512LdsElems = 512 + 32;
+__device__ __forceinline__ void fill_tw512_lds(c2 *t, const c2 *__restrict__ tw) {
+  for (int k = threadIdx.x; k < kTw512LdsElems; k += blockDim.x)
+    t[k] = k < 512 ? tw[k << 3] : tw[(k - 512) << 7];
+}
+
+template <bool INV, bool SC, class TW>
+__device__ __forceinline__ void unit512(const c2 (&a)[2][4], const Frag &f, const TW &tw, c2 *buf,
+                                        c2 *out, int64_t s, int64_t batch, double scale, int lane,
+                                        bool reuse) {
   const int g = lane >> 4, j = lane & 15;
   c2 yh[2][4];
 #pragma unroll
@@ -463,7 +482,7 @@ __device__ __forceinline__ void unit512(const c2 (&a)[2][4], const Frag &f,
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int n2 = 16 * h + 4 * r + g;
-      yh[h][r] = cmul(make_double2(yr[r], yi[r]), tw[(n2 * j) << 3]);  // W_512
+      yh[h][r] = cmul(make_double2(yr[r], yi[r]), tw.w512(n2 * j));
     }
   }
   // radix-2 over h, then W_32^{n2' k2a}
@@ -472,7 +491,7 @@ __device__ __forceinline__ void unit512(const c2 (&a)[2][4], const Frag &f,
   for (int r = 0; r < 4; ++r) {
     const int n2p = 4 * r + g;
     b[0][r] = cadd(yh[0][r], yh[1][r]);
-    b[1][r] = cmul(csub(yh[0][r], yh[1][r]), tw[n2p << 7]);  // W_32^{n2'}
+    b[1][r] = cmul(csub(yh[0][r], yh[1][r]), tw.w32(n2p));
   }
   if (reuse) wave_lds_sync();  // the previous unit's LDS reads are done
 #pragma unroll
@@ -512,33 +531,38 @@ __global__ __launch_bounds__(64 * kWaves) void tensor512_kernel(const c2 *in, c2
   for (int u = 0; u < G; ++u)
     load512<INV>(in, sw + u < batch ? sw + u : sw, g, j, au[u]);  // a short tail re-reads sw
 #pragma unroll
-  for (int u = 0; u < G; ++u) unit512<INV, SC>(au[u], f, tw, buf, out, sw + u, batch, scale, lane, u > 0);
+  for (int u = 0; u < G; ++u)
+    unit512<INV, SC>(au[u], f, Tw512Global{tw}, buf, out, sw + u, batch, scale, lane, u > 0);
 }
 
 // Persistent form: wave w handles signals w, w + (waves in the grid), ...;
 // the next signal's loads are issued before the current one is transformed.
+// The twiddles are read from a copy in LDS: read from HBM after those loads,
+// each twiddle's wait (the vector-memory counter retires in order) would
+// also wait for the whole prefetch.
 template <bool INV, bool SC>
 __global__ __launch_bounds__(64 * kWaves) void tensor512p_kernel(const c2 *in, c2 *out,
                                                                  int64_t batch,
                                                                  const c2 *__restrict__ tw,
                                                                  double scale) {
   __shared__ c2 lds[kWaves * 512];
+  __shared__ c2 twl[kTw512LdsElems];
+  fill_tw512_lds(twl, tw);
+  __syncthreads();  // before any wave leaves
   const int lane = lane_id(), g = lane >> 4, j = lane & 15;
   const int64_t stride = (int64_t)gridDim.x * kWaves;
   int64_t s = wave_global();
   if (s >= batch) return;
   c2 *buf = lds + (threadIdx.x >> 6) * 512;
   const Frag f = dft16_frag(tw, lane);
+  const Tw512Lds twt{twl};
   c2 cur[2][4];
   load512<INV>(in, s, g, j, cur);
   for (bool reuse = false;; reuse = true) {
-    // keeps the twiddle loads inside the loop (hoisted they would pin VGPRs)
-    const c2 *twl = tw;
-    asm volatile("" : "+s"(twl));
     const int64_t sn = s + stride;
     c2 nxt[2][4];
     if (sn < batch) load512<INV>(in, sn, g, j, nxt);
-    unit512<INV, SC>(cur, f, twl, buf, out, s, batch, scale, lane, reuse);
+    unit512<INV, SC>(cur, f, twt, buf, out, s, batch, scale, lane, reuse);
     if (sn >= batch) break;
 #pragma unroll
     for (int h = 0; h < 2; ++h)
@@ -547,7 +571,6 @@ __global__ __launch_bounds__(64 * kWaves) void tensor512p_kernel(const c2 *in, c
     s = sn;
   }
 }
-
 // ---------------------------------------------------------------------------
 // N = 4096 = 16 x 256 (16 x 16 x 16), one signal per 512-thread workgroup.
 // n = 256 n1 + m, k = k1 + 16 k2.
