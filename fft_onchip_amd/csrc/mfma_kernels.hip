@@ -203,13 +203,24 @@ __device__ __forceinline__ void load16xm(const c2 *in, int64_t s0, int64_t batch
     a[c] = (s < batch) ? ld_in<INV>(in + s * N + M * (4 * c + g) + n2) : make_double2(0.0, 0.0);
 }
 
+// W_N^k of a unit16xm from the 4096-entry table in HBM, or from a copy of
+// W_N^k, k < N, in LDS (whose reads do not wait for HBM loads in flight).
+template <int N>
+struct TwNGlobal {
+  const c2 *__restrict__ tw;
+  __device__ c2 operator()(int k) const { return tw[k * (kTwN / N)]; }
+};
+struct TwNLds {
+  const c2 *t;
+  __device__ c2 operator()(int k) const { return t[k]; }
+};
+
 // Transforms and stores the unit whose first signal is s0.
-template <int M, bool INV, bool SC>
-__device__ __forceinline__ void unit16xm(const c2 (&a)[4], const Frag &f,
-                                         const c2 *__restrict__ tw, c2 *out, int64_t s0,
-                                         int64_t batch, double scale, int g, int j) {
-  constexpr int N = 16 * M;
+template <int M, bool INV, bool SC, class TW = TwNGlobal<16 * M>>
+__device__ __forceinline__ void unit16xm(const c2 (&a)[4], const Frag &f, const TW &tw, c2 *out,
+                                         int64_t s0, int64_t batch, double scale, int g, int j) {
   constexpr int SPW = 16 / M;
+  constexpr int N = 16 * M;
   v4d yr, yi;
   kgemm(a, f, yr, yi);
   c2 y[4];
@@ -217,7 +228,7 @@ __device__ __forceinline__ void unit16xm(const c2 (&a)[4], const Frag &f,
   for (int r = 0; r < 4; ++r) {
     const int n2 = (4 * r + g) / SPW;
     c2 v = make_double2(yr[r], yi[r]);
-    if (n2 != 0) v = cmul(v, tw[(n2 * j) * (kTwN / N)]);
+    if (n2 != 0) v = cmul(v, tw(n2 * j));
     y[r] = v;
   }
   if constexpr (M == 2) {
@@ -281,7 +292,8 @@ __global__ __launch_bounds__(64 * kWaves) void tensor16xm_kernel(const c2 *in, c
 #pragma unroll
   for (int u = 0; u < G; ++u) load16xm<M, INV>(in, sw + SPW * u, batch, g, j, au[u]);
 #pragma unroll
-  for (int u = 0; u < G; ++u) unit16xm<M, INV, SC>(au[u], f, tw, out, sw + SPW * u, batch, scale, g, j);
+  for (int u = 0; u < G; ++u)
+    unit16xm<M, INV, SC>(au[u], f, TwNGlobal<16 * M>{tw}, out, sw + SPW * u, batch, scale, g, j);
 }
 
 // A unit of a small-N kernel as a type, for the persistent form below:
@@ -297,19 +309,31 @@ struct Unit16xM {
   __device__ static void load(const c2 *in, int64_t s0, int64_t batch, int g, int j, c2 (&a)[4]) {
     load16xm<M, INV>(in, s0, batch, g, j, a);
   }
+  // W_N^k, k < N, copied into LDS once per workgroup (fill_tw) and read
+  // from there by run()
+  static constexpr int kTwElems = 16 * M;
+  __device__ static void fill_tw(c2 *t, const c2 *__restrict__ tw) {
+    for (int k = threadIdx.x; k < kTwElems; k += blockDim.x) t[k] = tw[k * (kTwN / (16 * M))];
+  }
   template <bool INV, bool SC>
-  __device__ static void run(const c2 (&a)[4], const Frag &f, const c2 *__restrict__ tw, c2 *out,
-                             int64_t s0, int64_t batch, double scale, int g, int j) {
-    unit16xm<M, INV, SC>(a, f, tw, out, s0, batch, scale, g, j);
+  __device__ static void run(const c2 (&a)[4], const Frag &f, const c2 *twl, c2 *out, int64_t s0,
+                             int64_t batch, double scale, int g, int j) {
+    unit16xm<M, INV, SC>(a, f, TwNLds{twl}, out, s0, batch, scale, g, j);
   }
 };
 
 // Persistent form: wave w handles groups of G units w, w + (waves in the
 // grid), ...; the next group's loads are issued before the current group is
 // transformed, so they are in flight during its matrix products and stores.
+// The twiddles come from a copy in LDS: read from HBM after those loads,
+// each twiddle's wait (the vector-memory counter retires in order) would
+// also wait for the whole prefetch.
 template <class U, int G, bool INV, bool SC>
 __global__ __launch_bounds__(64 * kWaves) void tensor_persistent_kernel(
     const c2 *in, c2 *out, int64_t batch, const c2 *__restrict__ tw, double scale) {
+  __shared__ c2 twl[U::kTwElems];
+  U::fill_tw(twl, tw);
+  __syncthreads();  // before any wave leaves
   constexpr int SPG = U::kSignals * G;  // signals per group
   const int lane = lane_id(), g = lane >> 4, j = lane & 15;
   const int64_t groups = (batch + SPG - 1) / SPG;
@@ -330,7 +354,7 @@ __global__ __launch_bounds__(64 * kWaves) void tensor_persistent_kernel(
     }
 #pragma unroll
     for (int u = 0; u < G; ++u)
-      U::template run<INV, SC>(cur[u], f, tw, out, q * SPG + U::kSignals * u, batch, scale, g, j);
+      U::template run<INV, SC>(cur[u], f, twl, out, q * SPG + U::kSignals * u, batch, scale, g, j);
     if (qn >= groups) break;
 #pragma unroll
     for (int u = 0; u < G; ++u)
