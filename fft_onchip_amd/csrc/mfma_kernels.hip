@@ -396,22 +396,32 @@ int launch_tensor_persistent(const LaunchArgs &a) {
 // ---------------------------------------------------------------------------
 // 256-point core on one wave: input a[c] = x[64c + l] (n = 16 n1 + n2 with
 // n1 = 4c + g, n2 = j); output z[r] = X[4r + g + 16 j].
-__device__ __forceinline__ void core256(c2 (&a)[4], const Frag &f, const c2 *__restrict__ tw,
-                                        int lane, c2 (&z)[4]) {
+// The inter-stage twiddles of core256 for this lane: w[r] = W_256^{n2 k1},
+// n2 = 4r + g, k1 = j.
+__device__ __forceinline__ void core256_twiddles(const c2 *__restrict__ tw, int lane, c2 (&w)[4]) {
   const int g = lane >> 4, j = lane & 15;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) w[r] = tw[((4 * r + g) * j) << 4];
+}
+
+__device__ __forceinline__ void core256(c2 (&a)[4], const Frag &f, const c2 (&w)[4], c2 (&z)[4]) {
   v4d yr, yi;
   kgemm(a, f, yr, yi);
   // D: reg r = Y[n2 = 4r + g][k1 = j]; twiddle W_256^{n2 k1}
   c2 b[4];
 #pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    const int n2 = 4 * r + g;
-    b[r] = cmul(make_double2(yr[r], yi[r]), tw[(n2 * j) << 4]);
-  }
+  for (int r = 0; r < 4; ++r) b[r] = cmul(make_double2(yr[r], yi[r]), w[r]);
   // stage 2: A chunk r = b[r] (rows = k1 = j, K = n2 = 4r + g)
   kgemm(b, f, yr, yi);
 #pragma unroll
   for (int r = 0; r < 4; ++r) z[r] = make_double2(yr[r], yi[r]);
+}
+
+__device__ __forceinline__ void core256(c2 (&a)[4], const Frag &f, const c2 *__restrict__ tw,
+                                        int lane, c2 (&z)[4]) {
+  c2 w[4];
+  core256_twiddles(tw, lane, w);
+  core256(a, f, w, z);
 }
 
 // LDS element swizzle for the output transposes: XOR the low 4 bits of the
@@ -431,6 +441,13 @@ __global__ __launch_bounds__(64 * kWaves) void tensor256_kernel(const c2 *in, c2
   if (sw >= batch) return;
   c2 *buf = lds + (threadIdx.x >> 6) * 256;
   const Frag f = dft16_frag(tw, lane);
+  // with several units the twiddles (the same for every unit) come before
+  // the data: loaded after it, their wait (the vector-memory counter retires
+  // in order) would hold the first unit until every unit's data had arrived
+  // (G = 2: 115 -> 103 us, G = 4: 113 -> 108 us at 65536 signals; G = 1
+  // keeps them after its data, 104 against 107 us)
+  c2 w[4];
+  if constexpr (G > 1) core256_twiddles(tw, lane, w);
   c2 a[G][4];
 #pragma unroll
   for (int u = 0; u < G; ++u) {
@@ -438,10 +455,11 @@ __global__ __launch_bounds__(64 * kWaves) void tensor256_kernel(const c2 *in, c2
 #pragma unroll
     for (int c = 0; c < 4; ++c) a[u][c] = ld_in<INV>(in + s * 256 + 64 * c + lane);
   }
+  if constexpr (G == 1) core256_twiddles(tw, lane, w);
 #pragma unroll
   for (int u = 0; u < G; ++u) {
     c2 z[4];
-    core256(a[u], f, tw, lane, z);
+    core256(a[u], f, w, z);
     if (u > 0) wave_lds_sync();  // the previous unit's LDS reads are done
 #pragma unroll
     for (int r = 0; r < 4; ++r) buf[swz16(4 * r + g + 16 * j)] = z[r];
@@ -826,9 +844,9 @@ const Variant kT64[] = {{"tensor:16x4/8 per wave", TF, &launch_t64g2, 32},
 const Variant kT128[] = {{"tensor:16x8/4 per wave", TF, &launch_t128g2, 16},
                          {"tensor:16x8/2 per wave", TF, &launch_t128, 8},
                          {"tensor:16x8/8 per wave", TF, &launch_t128g4, 32}};
-const Variant kT256[] = {{"tensor:16x16/4 per wave", TF, &launch_t256g4, 16},
-                         {"tensor:16x16/1 per wave", TF, &launch_t256, 4},
-                         {"tensor:16x16/2 per wave", TF, &launch_t256g2, 8}};
+const Variant kT256[] = {{"tensor:16x16/2 per wave", TF, &launch_t256g2, 8},
+                         {"tensor:16x16/4 per wave", TF, &launch_t256g4, 16},
+                         {"tensor:16x16/1 per wave", TF, &launch_t256, 4}};
 const Variant kT512[] = {{"tensor:16x2x16/1 per wave/persistent", TF, &launch_t512p, 4},
                          {"tensor:16x2x16/1 per wave", TF, &launch_t512, 4},
                          {"tensor:16x2x16/2 per wave", TF, &launch_t512g2, 8}};
