@@ -205,14 +205,29 @@ __device__ __forceinline__ void load16xm(const c2 *in, int64_t s0, int64_t batch
 
 // W_N^k of a unit16xm from the 4096-entry table in HBM, or from a copy of
 // W_N^k, k < N, in LDS (whose reads do not wait for HBM loads in flight).
+// get(r, k): the twiddle W_N^k of result register r.
 template <int N>
 struct TwNGlobal {
   const c2 *__restrict__ tw;
-  __device__ c2 operator()(int k) const { return tw[k * (kTwN / N)]; }
+  __device__ c2 get(int, int k) const { return tw[k * (kTwN / N)]; }
 };
 struct TwNLds {
   const c2 *t;
-  __device__ c2 operator()(int k) const { return t[k]; }
+  __device__ c2 get(int, int k) const { return t[k]; }
+};
+// Loaded into registers ahead (the same for every unit of a lane).
+struct TwNRegs {
+  c2 w[4];
+  template <int M>
+  __device__ void fetch(const c2 *__restrict__ tw, int g, int j) {
+    constexpr int SPW = 16 / M;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int n2 = (4 * r + g) / SPW;
+      w[r] = n2 != 0 ? tw[(n2 * j) * (kTwN / (16 * M))] : make_double2(1.0, 0.0);
+    }
+  }
+  __device__ c2 get(int r, int) const { return w[r]; }
 };
 
 // Transforms and stores the unit whose first signal is s0.
@@ -228,7 +243,7 @@ __device__ __forceinline__ void unit16xm(const c2 (&a)[4], const Frag &f, const 
   for (int r = 0; r < 4; ++r) {
     const int n2 = (4 * r + g) / SPW;
     c2 v = make_double2(yr[r], yi[r]);
-    if (n2 != 0) v = cmul(v, tw(n2 * j));
+    if (n2 != 0) v = cmul(v, tw.get(r, n2 * j));
     y[r] = v;
   }
   if constexpr (M == 2) {
@@ -288,12 +303,23 @@ __global__ __launch_bounds__(64 * kWaves) void tensor16xm_kernel(const c2 *in, c
   const int64_t sw = wave_global() * (SPW * G);
   if (sw >= batch) return;
   const Frag f = dft16_frag(tw, lane);
-  c2 au[G][4];
+  if constexpr (G > 1) {
+    // the twiddles (the same for every unit) before the data: loaded after
+    // it, their wait (the vector-memory counter retires in order) would hold
+    // the first unit until every unit's data had arrived
+    TwNRegs w;
+    w.fetch<M>(tw, g, j);
+    c2 au[G][4];
 #pragma unroll
-  for (int u = 0; u < G; ++u) load16xm<M, INV>(in, sw + SPW * u, batch, g, j, au[u]);
+    for (int u = 0; u < G; ++u) load16xm<M, INV>(in, sw + SPW * u, batch, g, j, au[u]);
 #pragma unroll
-  for (int u = 0; u < G; ++u)
-    unit16xm<M, INV, SC>(au[u], f, TwNGlobal<16 * M>{tw}, out, sw + SPW * u, batch, scale, g, j);
+    for (int u = 0; u < G; ++u)
+      unit16xm<M, INV, SC>(au[u], f, w, out, sw + SPW * u, batch, scale, g, j);
+  } else {
+    c2 a[4];
+    load16xm<M, INV>(in, sw, batch, g, j, a);
+    unit16xm<M, INV, SC>(a, f, TwNGlobal<16 * M>{tw}, out, sw, batch, scale, g, j);
+  }
 }
 
 // A unit of a small-N kernel as a type, for the persistent form below:
@@ -834,8 +860,8 @@ const Variant kT8[] = {{"tensor:diag(F8,F8)/32 per wave", TF, &launch_t8, 128},
 const Variant kT16[] = {{"tensor:16/16 per wave", TF, &launch_t16, 64},
                         {"tensor:16/32 per wave", TF, &launch_t16g2, 128},
                         {"tensor:16/64 per wave", TF, &launch_t16g4, 256}};
-const Variant kT32[] = {{"tensor:16x2/8 per wave", TF, &launch_t32, 32},
-                        {"tensor:16x2/16 per wave", TF, &launch_t32g2, 64},
+const Variant kT32[] = {{"tensor:16x2/16 per wave", TF, &launch_t32g2, 64},
+                        {"tensor:16x2/8 per wave", TF, &launch_t32, 32},
                         {"tensor:16x2/32 per wave", TF, &launch_t32g4, 128}};
 const Variant kT64[] = {{"tensor:16x4/8 per wave", TF, &launch_t64g2, 32},
                         {"tensor:16x4/8 per wave/persistent", TF, &launch_tensor_persistent<Unit16xM<4>, 2>, 32},
