@@ -123,3 +123,24 @@ def test_tile_flag_selects_a_variant_or_is_unsupported():
     v = 0
     other = next(t for t in tiles if t != tiles[v])
     assert lib.fftoc_fft_variant(fake, fake, 4, n, fo.make_flags(tile=other), v, None) == 4
+
+
+def test_batch_routine_table_needs_no_device():
+    """The tile routines plans and servers choose from: every length has at
+    least one, names are unique per length and name their length's split
+    (radix factors multiplying to n), and bad arguments are rejected."""
+    for n in fo.ALL_SIZES:
+        names = fo.batch_routines(n)
+        assert names and len(set(names)) == len(names), (n, names)
+        for name in names:
+            split = name.split("/")[0]
+            prod = 1
+            for f in split.split("x"):
+                prod *= int(f)
+            assert prod == n, (n, name)
+    lib = fo.lib()
+    name = ctypes.c_char_p()
+    assert lib.fftoc_batch_routine_name(100, 0, ctypes.byref(name)) == 1     # not a power of two
+    assert lib.fftoc_batch_routine_name(64, 0, None) == 3                    # null output
+    assert lib.fftoc_batch_routine_name(64, -1, ctypes.byref(name)) == 13    # no such index
+    assert lib.fftoc_plan_set_routine(None, 64, 0, b"8x8/tile/S32") == 13    # no plan
