@@ -398,25 +398,51 @@ def cpu_baseline(x: np.ndarray, gpu_sample: np.ndarray, threads: int, budget_s: 
     err = float(np.max(oracle.rel_l2(gpu_sample, ref, axis=-1)))
     cpu_err = float(np.max(oracle.rel_l2(out[:m], ref, axis=-1)))
     count = batch * reps
+    host = _host_info()
     return {
         "value": count / med, "unit": "FFTs/s", "cores": threads, "kind": "port",
         "sample": f"all {batch} N={n} signals of the workload, {reps}x per timed run; "
                   f"median of 5 runs ({sum(times):.1f} s of CPU work)",
         "GFLOPps": 5.0 * n * math.log2(n) * count / med / 1e9,
-        "cpu_model": _cpu_model(), "cpu_max_rel_l2_vs_oracle": cpu_err,
+        "threads": threads, "threads_rule": host_threads()[1],
+        "nproc": host["nproc"], "affinity_cpus": host["affinity_cpus"],
+        "cpu_model": host["model"], "cpu_id": host["id"],
+        "cpu_max_rel_l2_vs_oracle": cpu_err,
         "gpu_max_rel_l2_vs_oracle": err, "gpu_checked_signals": m,
     }
 
 
-def _cpu_model() -> str:
+def host_threads():
+    """Threads for the CPU baseline: every CPU this process may run on, up to
+    the host-thread share allotted to one GPU (OMP_NUM_THREADS, which the GPU
+    host sets to its per-GPU share), with the rule that chose the count."""
+    affinity = len(os.sched_getaffinity(0))
+    share = os.environ.get("OMP_NUM_THREADS", "")
+    if share.isdigit() and 0 < int(share) < affinity:
+        return int(share), (f"OMP_NUM_THREADS={share}: the host threads allotted to one GPU "
+                            f"({affinity} CPUs in this process's affinity mask)")
+    return affinity, f"all {affinity} CPUs in this process's affinity mask"
+
+
+def _host_info() -> dict:
+    """Host CPU as /proc/cpuinfo reports it: model string plus vendor, family,
+    model and stepping numbers (a virtual machine may report a generic model
+    string), and the CPU counts."""
+    fields = {}
     try:
         with open("/proc/cpuinfo") as f:
             for line in f:
-                if line.startswith("model name"):
-                    return line.split(":", 1)[1].strip()
+                if not line.strip():
+                    break  # first processor's block only
+                k, _, v = line.partition(":")
+                fields[k.strip()] = v.strip()
     except OSError:
         pass
-    return "unknown"
+    ident = {k: fields.get(k) for k in ("vendor_id", "cpu family", "model", "stepping", "cpu MHz",
+                                        "cache size")}
+    ident["avx512f"] = "avx512f" in fields.get("flags", "").split()
+    return {"model": fields.get("model name", "unknown"), "id": ident,
+            "nproc": os.cpu_count(), "affinity_cpus": len(os.sched_getaffinity(0))}
 
 
 TRAFFIC_JSON = os.path.join(ROOT, "profiles", "traffic.json")
@@ -429,18 +455,19 @@ HEADLINE = "fp64_c2c_fwd_n4096_b16384"
 def headline_variant(fo, wl) -> str:
     """Name of the kernel variant fftoc_fft runs for the workload."""
     lib = fo.lib()
-    return lib.fftoc_variant_name(wl.n, lib.fftoc_get_variant(wl.n, wl.batch)).decode()
+    return lib.fftoc_variant_name(wl.n, 0, lib.fftoc_get_variant(wl.n, 0, wl.batch)).decode()
 
 
-def load_traffic(workload: str):
+def load_traffic(workload: str, variant: str):
     """HBM bytes per launch of the headline kernel, from the committed
     rocprofv3 --pmc summary (profiles/traffic.json, written by --traffic from
     counter runs of `bench.py --quick`, which launches only that kernel and
-    the calibration copy)."""
+    the calibration copy).  None unless the summary was taken of this
+    workload run by this kernel variant (a stale summary is not reported)."""
     try:
         with open(TRAFFIC_JSON) as f:
             t = json.load(f)
-        if t.get("workload") != workload or len(t["kernels"]) != 1:
+        if t.get("workload") != workload or t.get("variant") != variant or len(t["kernels"]) != 1:
             return None
         (name, v), = t["kernels"].items()
         return dict(v, kernel=name, fetch_scale=t["fetch_scale"], write_scale=t["write_scale"])
@@ -449,7 +476,7 @@ def load_traffic(workload: str):
 
 
 def traffic_from_pmc(fetch_csv: str, write_csv: str, out_path: str = "",
-                     workload: str = "") -> dict:
+                     workload: str = "", variant: str = "") -> dict:
     """Scales FETCH_SIZE / WRITE_SIZE (KiB per dispatch) by what the same
     counters read for the 1 GiB device-to-device copy this script runs
     (known: 1 GiB read, 1 GiB written) and writes the result as JSON
@@ -474,7 +501,7 @@ def traffic_from_pmc(fetch_csv: str, write_csv: str, out_path: str = "",
     f_scale, w_scale = kib / max(copy_f), kib / max(copy_w)
     out = {"fetch_scale": f_scale, "write_scale": w_scale,
            "calibration": "1 GiB hipMemcpy device-to-device (read 1 GiB + write 1 GiB)",
-           "workload": workload or HEADLINE, "kernels": {}}
+           "workload": workload or HEADLINE, "variant": variant, "kernels": {}}
     for k in fetch:
         if COPY_KERNEL in k or k not in write:
             continue
@@ -504,12 +531,22 @@ def main():
                     help="write profiles/traffic.json from two rocprofv3 --pmc runs and exit")
     ap.add_argument("--traffic-out", default="", metavar="JSON",
                     help="with --traffic: write here instead (counter runs of other workloads)")
+    ap.add_argument("--traffic-variant", default="", metavar="NAME",
+                    help="with --traffic: the kernel variant the counter runs measured "
+                         "(default: the headline workload's)")
     ap.add_argument("--traffic-workload", default="", metavar="NAME",
                     help="with --traffic: the workload the counter runs measured")
     args = ap.parse_args()
     if args.traffic:
+        variant = args.traffic_variant
+        if not variant and not args.traffic_workload:
+            # the headline workload: the variant fftoc_fft picks for it (a
+            # table lookup, no device needed)
+            import fft_onchip_amd as fo
+            variant = headline_variant(fo, Workload(HEADLINE, 4096, 16384, 7, "normal", "best"))
         print(json.dumps(traffic_from_pmc(*args.traffic, out_path=args.traffic_out,
-                                          workload=args.traffic_workload), indent=1))
+                                          workload=args.traffic_workload, variant=variant),
+                         indent=1))
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one GPU; the batched FFT shards by contiguous "
@@ -605,7 +642,7 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
+        threads, _ = host_threads()
         cpu = cpu_baseline(x, sample, threads)
         if prec_samples:
             extra["precision_c4_vs_oracle"] = precision_vs_oracle(prec_samples)
@@ -616,7 +653,7 @@ def main():
                 np.max(oracle.rel_l2(got, ref, axis=-1)))
         print(f"[bench] cpu baseline: {cpu['value']:.4g} FFT/s on {threads} threads", file=sys.stderr, flush=True)
 
-    traffic = load_traffic(head.name)
+    traffic = load_traffic(head.name, headline_variant(fo, head))
     roof = {"bound": "hbm", "achieved": round(res["GBps"], 2), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
             "frac": round(res["hbm_frac"], 4),
             "traffic": round(traffic["bytes"]) if traffic else None,

@@ -126,12 +126,20 @@ def fft(x, batch: int, n: int, out=None, flags: int = 0, stream=None) -> None:
     _call("fftoc_fft", x, out, batch, n, flags, stream)
 
 
-def variant_index(n: int, name: str) -> int:
-    """Index of the kernel variant called `name` for length n."""
-    for v in range(lib().fftoc_variant_count(int(n))):
-        if lib().fftoc_variant_name(int(n), v).decode() == name:
-            return v
-    raise KeyError(f"no variant {name!r} for n={n}")
+def variants(n: int, c64: bool = False) -> list:
+    """Names of the kernel variants of length n for complex128 (or, with
+    c64, complex64) data, in index order."""
+    dt = DATA_C64 if c64 else 0
+    return [lib().fftoc_variant_name(int(n), dt, v).decode()
+            for v in range(lib().fftoc_variant_count(int(n), dt))]
+
+
+def variant_index(n: int, name: str, c64: bool = False) -> int:
+    """Index of the kernel variant called `name` for length n and data type."""
+    try:
+        return variants(n, c64).index(name)
+    except ValueError:
+        raise KeyError(f"no variant {name!r} for n={n}, c64={c64}") from None
 
 
 def fft_variant(x, batch: int, n: int, variant: int, out=None, flags: int = 0,

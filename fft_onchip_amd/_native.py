@@ -30,14 +30,14 @@ SIGNATURES = {
     "fftoc_fft": ([_vp, _vp, _i64, _int, _u32, _vp], _int),
 }
 SIGNATURES.update({
-    "fftoc_variant_count": ([_int], _int),
-    "fftoc_variant_name": ([_int, _int], ctypes.c_char_p),
+    "fftoc_variant_count": ([_int, _u32], _int),
+    "fftoc_variant_name": ([_int, _u32, _int], ctypes.c_char_p),
     "fftoc_fft_variant": ([_vp, _vp, _i64, _int, _u32, _int, _vp], _int),
-    "fftoc_get_variant": ([_int, _i64], _int),
-    "fftoc_set_variant": ([_int, _i64, _int], _int),
-    "fftoc_autotune": ([_int, _i64, _int, _vp], _int),
-    "fftoc_variant_time_us": ([_int, _int], ctypes.c_double),
-    "fftoc_variant_tile": ([_int, _int], _int),
+    "fftoc_get_variant": ([_int, _u32, _i64], _int),
+    "fftoc_set_variant": ([_int, _u32, _i64, _int], _int),
+    "fftoc_autotune": ([_int, _u32, _i64, _int, _vp], _int),
+    "fftoc_variant_time_us": ([_int, _u32, _int], ctypes.c_double),
+    "fftoc_variant_tile": ([_int, _u32, _int], _int),
     "fftoc_tuning_save": ([ctypes.c_char_p], _int),
     "fftoc_tuning_load": ([ctypes.c_char_p], _int),
 })

@@ -17,8 +17,6 @@
 namespace fftoc {
 namespace {
 
-constexpr int kMaxDevices = 64;
-
 struct DeviceState {
   bool ready = false;
   int status = FFTOC_SUCCESS;
@@ -289,18 +287,18 @@ int run(Family fam, const void *in, void *out, int64_t batch, int n, uint32_t fl
   return reg->v[pick].launch(a);
 }
 
-// Times every variant of length n on zero-filled scratch buffers (FFT
-// timing does not depend on the data) and makes the fastest variant of each
-// family the choice of that family's entry point.
-int autotune(int n, int64_t batch, int reps, void *stream_p) {
-  SizeRegistry *reg = registry(n);
+// Times every variant of length n and data type on zero-filled scratch
+// buffers (FFT timing does not depend on the data) and makes the fastest
+// variant of each family the choice of that family's entry point.
+int autotune(int n, bool c64, int64_t batch, int reps, void *stream_p) {
+  SizeRegistry *reg = registry(n, c64);
   if (!reg) return FFTOC_ERROR_INVALID_SIZE;
   if (batch <= 0 || reps <= 0) return FFTOC_ERROR_INVALID_BATCH;
   int dev = 0;
   int rc = init_current_device(&dev);
   if (rc != FFTOC_SUCCESS) return rc;
   hipStream_t stream = static_cast<hipStream_t>(stream_p);
-  const size_t bytes = (size_t)batch * (size_t)n * 16u;
+  const size_t bytes = (size_t)batch * (size_t)n * (c64 ? 8u : 16u);
   void *in = nullptr, *out = nullptr;
   if (hipMalloc(&in, bytes) != hipSuccess) return FFTOC_ERROR_ALLOC;
   if (hipMalloc(&out, bytes) != hipSuccess) {
@@ -314,8 +312,8 @@ int autotune(int n, int64_t batch, int reps, void *stream_p) {
     rc = FFTOC_ERROR_DEVICE;
   std::vector<double> us(reg->v.size(), -1.0);
   for (size_t i = 0; i < reg->v.size() && rc == FFTOC_SUCCESS; ++i) {
-    LaunchArgs a{in, out, batch, g_dev[dev].twiddles, g_dev[dev].twiddles32, stream, false, 1.0, 0,
-                 FFTOC_PREC_FP64, false};
+    LaunchArgs a{in,     out,   batch, c64 ? g_dev[dev].twiddles32 : g_dev[dev].twiddles,
+                 g_dev[dev].twiddles32, stream, false, 1.0, 0, FFTOC_PREC_FP64, c64};
     if (reg->v[i].launch(a) != FFTOC_SUCCESS) continue;  // warm-up
     std::vector<float> t;
     for (int r = 0; r < reps; ++r) {
@@ -430,7 +428,7 @@ const char *fftoc_strerror(int status) {
   }
 }
 
-int fftoc_version(void) { return 100; }
+int fftoc_version(void) { return 200; }
 
 int fftoc_tensor_fft(const void *in, void *out, int64_t batch, int n, uint32_t flags,
                      void *stream) {
@@ -465,33 +463,38 @@ int fftoc_fft(const void *in, void *out, int64_t batch, int n, uint32_t flags, v
   return fftoc::run(Family::Best, in, out, batch, n, flags, stream);
 }
 
-int fftoc_variant_count(int n) {
-  fftoc::SizeRegistry *r = fftoc::registry(n);
+namespace {
+bool dtype_c64(uint32_t flags) { return (flags & FFTOC_DATA_C64) != 0; }
+const char *kDtypeNames[2] = {"c128", "c64"};
+}  // namespace
+
+int fftoc_variant_count(int n, uint32_t flags) {
+  fftoc::SizeRegistry *r = fftoc::registry(n, dtype_c64(flags));
   return r ? (int)r->v.size() : 0;
 }
 
-const char *fftoc_variant_name(int n, int variant) {
-  fftoc::SizeRegistry *r = fftoc::registry(n);
+const char *fftoc_variant_name(int n, uint32_t flags, int variant) {
+  fftoc::SizeRegistry *r = fftoc::registry(n, dtype_c64(flags));
   if (!r || variant < 0 || variant >= (int)r->v.size()) return nullptr;
   return r->v[variant].name;
 }
 
 int fftoc_fft_variant(const void *in, void *out, int64_t batch, int n, uint32_t flags,
                       int variant, void *stream) {
-  fftoc::SizeRegistry *r = fftoc::registry(n, (flags & FFTOC_DATA_C64) != 0);
+  fftoc::SizeRegistry *r = fftoc::registry(n, dtype_c64(flags));
   if (!r) return FFTOC_ERROR_INVALID_SIZE;
   if (variant < 0 || variant >= (int)r->v.size()) return FFTOC_ERROR_INVALID_HANDLE;
   const Family fam = (r->v[variant].families & fftoc::FAM_TENSOR) ? Family::Tensor : Family::Best;
   return fftoc::run(fam, in, out, batch, n, flags, stream, variant);
 }
 
-int fftoc_get_variant(int n, int64_t batch) {
-  fftoc::SizeRegistry *r = fftoc::registry(n);
+int fftoc_get_variant(int n, uint32_t flags, int64_t batch) {
+  fftoc::SizeRegistry *r = fftoc::registry(n, dtype_c64(flags));
   return r ? r->choice[fftoc::family_index(Family::Best)][fftoc::batch_bucket(batch)].load() : -1;
 }
 
-int fftoc_set_variant(int n, int64_t batch, int variant) {
-  fftoc::SizeRegistry *r = fftoc::registry(n);
+int fftoc_set_variant(int n, uint32_t flags, int64_t batch, int variant) {
+  fftoc::SizeRegistry *r = fftoc::registry(n, dtype_c64(flags));
   if (!r) return FFTOC_ERROR_INVALID_SIZE;
   if (variant < 0 || variant >= (int)r->v.size()) return FFTOC_ERROR_INVALID_HANDLE;
   for (int b = 0; b < fftoc::kBuckets; ++b)
@@ -500,37 +503,40 @@ int fftoc_set_variant(int n, int64_t batch, int variant) {
   return FFTOC_SUCCESS;
 }
 
-int fftoc_autotune(int n, int64_t batch, int reps, void *stream) {
-  return fftoc::autotune(n, batch, reps, stream);
+int fftoc_autotune(int n, uint32_t flags, int64_t batch, int reps, void *stream) {
+  return fftoc::autotune(n, dtype_c64(flags), batch, reps, stream);
 }
 
-double fftoc_variant_time_us(int n, int variant) {
-  fftoc::SizeRegistry *r = fftoc::registry(n);
+double fftoc_variant_time_us(int n, uint32_t flags, int variant) {
+  fftoc::SizeRegistry *r = fftoc::registry(n, dtype_c64(flags));
   if (!r || variant < 0 || variant >= (int)r->v.size()) return -1.0;
   std::lock_guard<std::mutex> lock(fftoc::g_tune_mutex);
   return r->tuned_us[variant];
 }
 
-int fftoc_variant_tile(int n, int variant) {
-  fftoc::SizeRegistry *r = fftoc::registry(n);
+int fftoc_variant_tile(int n, uint32_t flags, int variant) {
+  fftoc::SizeRegistry *r = fftoc::registry(n, dtype_c64(flags));
   if (!r || variant < 0 || variant >= (int)r->v.size()) return -1;
   return r->v[variant].tile;
 }
 
-// Text format, one line per (entry point, length, batch bucket):
-//   <family> <n> <floor(log2 batch)> <variant name up to end of line>
+// Text format, one line per (entry point, length, data type, batch bucket):
+//   <family> <n> <c128|c64> <floor(log2 batch)> <variant name up to end of line>
 int fftoc_tuning_save(const char *path) {
   if (!path) return FFTOC_ERROR_NULL_POINTER;
   FILE *f = std::fopen(path, "w");
   if (!f) return FFTOC_ERROR_INVALID_HANDLE;
-  std::fprintf(f, "# fftoc tuning table: family n log2(batch) variant\n");
+  std::fprintf(f, "# fftoc tuning table: family n dtype log2(batch) variant\n");
   for (int k = 3; k <= 12; ++k) {
-    fftoc::SizeRegistry *r = fftoc::registry(1 << k);
-    for (int fi = 0; fi < 4; ++fi) {
-      for (int b = 0; b < fftoc::kBuckets; ++b) {
-        const int c = r->choice[fi][b].load();
-        if (c >= 0)
-          std::fprintf(f, "%s %d %d %s\n", fftoc::kFamilyNames[fi], 1 << k, b, r->v[c].name);
+    for (int c64 = 0; c64 < 2; ++c64) {
+      fftoc::SizeRegistry *r = fftoc::registry(1 << k, c64 != 0);
+      for (int fi = 0; fi < 4; ++fi) {
+        for (int b = 0; b < fftoc::kBuckets; ++b) {
+          const int c = r->choice[fi][b].load();
+          if (c >= 0)
+            std::fprintf(f, "%s %d %s %d %s\n", fftoc::kFamilyNames[fi], 1 << k, kDtypeNames[c64],
+                         b, r->v[c].name);
+        }
       }
     }
   }
@@ -539,6 +545,9 @@ int fftoc_tuning_save(const char *path) {
   return err ? FFTOC_ERROR_INVALID_HANDLE : FFTOC_SUCCESS;
 }
 
+// Lines that do not parse, or name an unknown family, length, data type or
+// variant (or a variant of another family), are skipped and make the call
+// return FFTOC_ERROR_INVALID_HANDLE; every valid line is applied.
 int fftoc_tuning_load(const char *path) {
   if (!path) return FFTOC_ERROR_NULL_POINTER;
   FILE *f = std::fopen(path, "r");
@@ -547,16 +556,19 @@ int fftoc_tuning_load(const char *path) {
   int rc = FFTOC_SUCCESS;
   while (std::fgets(line, sizeof line, f)) {
     if (line[0] == '#' || line[0] == '\n') continue;
-    char fam[32];
+    char fam[32], dt[8];
     int n = 0, bucket = 0, used = 0;
-    if (std::sscanf(line, "%31s %d %d %n", fam, &n, &bucket, &used) != 3 || bucket < 0 ||
+    if (std::sscanf(line, "%31s %d %7s %d %n", fam, &n, dt, &bucket, &used) != 4 || bucket < 0 ||
         bucket >= fftoc::kBuckets) {
       rc = FFTOC_ERROR_INVALID_HANDLE;
       continue;
     }
+    int c64 = -1;
+    for (int i = 0; i < 2; ++i)
+      if (std::strcmp(dt, kDtypeNames[i]) == 0) c64 = i;
     std::string name(line + used);
     while (!name.empty() && (name.back() == '\n' || name.back() == '\r')) name.pop_back();
-    fftoc::SizeRegistry *r = fftoc::registry(n);
+    fftoc::SizeRegistry *r = c64 >= 0 ? fftoc::registry(n, c64 != 0) : nullptr;
     int fi = -1;
     for (int i = 0; i < 4; ++i)
       if (std::strcmp(fam, fftoc::kFamilyNames[i]) == 0) fi = i;

@@ -11,7 +11,7 @@
 // memset node.
 #include "batch_internal.hpp"
 #include "lane_fft.hpp"
-#include "stockham.hpp"
+#include "../../include/fftoc_stockham.hpp"
 
 namespace fftoc {
 namespace {
@@ -398,8 +398,13 @@ void serve_kernel(ServerDev sd) {
         }
       }
       if (state == kIdle) {
+        // A worker holding a ticket leaves only with the dispatcher (dev_stop):
+        // the item of that ticket may be published at any later time while the
+        // dispatcher runs, and a ticket given up here would never be run (the
+        // host relaunches only once the dispatcher has left too).  The own
+        // timer is a backstop for workers without a ticket.
         if (__hip_atomic_load(sd.dev_stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ||
-            wall_clock64() - idle_since > sd.idle_timeout_ticks + 100000000ull)
+            (!has_ticket && wall_clock64() - idle_since > sd.idle_timeout_ticks + 100000000ull))
           state = kExit;
         else if (!has_ticket)
           state = kIdleNoTicket;
@@ -541,19 +546,7 @@ int batch_routine_signals(int id) {
 }
 
 int batch_resident_blocks() {
-  static int cached = 0;
-  if (cached == 0) {
-    int per_cu = 0, dev = 0, cus = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_kernel, kBatchThreads, 0) !=
-            hipSuccess || per_cu <= 0)
-      per_cu = 1;
-    if (hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-        cus <= 0)
-      cus = 256;
-    cached = per_cu * cus;
-  }
-  return cached;
+  return (int)resident_workgroups<batch_kernel, kBatchThreads>();
 }
 
 int launch_batch(const JobDev *d_jobs, int njobs, int64_t total_items, unsigned long long *d_ctr,

@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <atomic>
+
 #include "../../include/fftoc.h"
 
 namespace fftoc {
@@ -23,21 +25,30 @@ struct LaunchArgs {
   bool c64;              // complex64 data (FFTOC_DATA_C64)
 };
 
-// Workgroups of KERNEL (THREADS threads each) resident on the whole device,
-// queried once per kernel: the grid of a persistent launch.
+constexpr int kMaxDevices = 64;
+
+// Workgroups of KERNEL (THREADS threads each) resident on the whole current
+// device: the grid of a persistent launch.  Queried once per (kernel,
+// device); concurrent first calls may both query, and store the same value.
 template <auto KERNEL, int THREADS>
 int64_t resident_workgroups() {
-  static const int64_t n = [] {
-    int per_cu = 0, dev = 0, cus = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, KERNEL, THREADS, 0) != hipSuccess ||
-        per_cu <= 0)
-      per_cu = 1;
-    if (hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-        cus <= 0)
-      cus = 256;
-    return (int64_t)per_cu * cus;
-  }();
+  static std::atomic<int64_t> cache[kMaxDevices];
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) dev = -1;
+  if (dev >= 0 && dev < kMaxDevices) {
+    const int64_t v = cache[dev].load(std::memory_order_relaxed);
+    if (v > 0) return v;
+  }
+  int per_cu = 0, cus = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, KERNEL, THREADS, 0) != hipSuccess ||
+      per_cu <= 0)
+    per_cu = 1;
+  if (dev < 0 ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      cus <= 0)
+    cus = 256;
+  const int64_t n = (int64_t)per_cu * cus;
+  if (dev >= 0 && dev < kMaxDevices) cache[dev].store(n, std::memory_order_relaxed);
   return n;
 }
 

@@ -24,7 +24,7 @@
 // by lane shuffles instead of per-thread registers.
 #pragma once
 
-#include "fftoc_device.hpp"
+#include "../../include/fftoc_device.hpp"
 
 namespace fftoc {
 

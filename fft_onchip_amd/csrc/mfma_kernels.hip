@@ -23,7 +23,7 @@
 //    wave keeps 12 doubles of DFT matrix (Br, Bi, Br+Bi per K-chunk).
 //  * Twiddles and DFT-matrix entries come from the FP64 table W_4096^j built
 //    on the host; only additions are done in-kernel (Br + Bi).
-#include "fftoc_device.hpp"
+#include "../../include/fftoc_device.hpp"
 #include "fftoc_internal.hpp"
 
 namespace fftoc {

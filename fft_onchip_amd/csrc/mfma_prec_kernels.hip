@@ -24,7 +24,7 @@
 // a_p b_q with p + q <= 2, smallest first, accumulated in FP32.
 #include <algorithm>
 
-#include "fftoc_device.hpp"
+#include "../../include/fftoc_device.hpp"
 #include "fftoc_internal.hpp"
 
 namespace fftoc {

@@ -33,7 +33,7 @@
 //    exact-rounded constants.
 #include "fftoc_internal.hpp"
 #include "lane_fft.hpp"
-#include "stockham.hpp"
+#include "../../include/fftoc_stockham.hpp"
 
 namespace fftoc {
 namespace {

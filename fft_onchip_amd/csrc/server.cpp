@@ -9,19 +9,26 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <cmath>
 #include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <map>
 #include <mutex>
 #include <new>
+#include <thread>
 #include <vector>
 
 #include "batch_internal.hpp"
 
 namespace {
 
-// First-fit allocator over [0, size) with coalescing; 256-byte granules.
+// Pool allocations are whole 256-byte granules.
+constexpr size_t granule(size_t bytes) { return (bytes + 255) & ~size_t(255); }
+
+// First-fit allocator over [0, size) with coalescing; 256-byte granules (the
+// size is a multiple of the granule, so anything that fits the capacity can
+// be placed once the pool is empty).
 class PoolAllocator {
  public:
   void reset(size_t size) {
@@ -30,7 +37,7 @@ class PoolAllocator {
     size_ = size;
   }
   bool alloc(size_t bytes, size_t *off) {
-    bytes = (bytes + 255) & ~size_t(255);
+    bytes = granule(bytes);
     for (auto it = free_.begin(); it != free_.end(); ++it) {
       if (it->second >= bytes) {
         *off = it->first;
@@ -45,7 +52,7 @@ class PoolAllocator {
     return false;
   }
   void release(size_t off, size_t bytes) {
-    bytes = (bytes + 255) & ~size_t(255);
+    bytes = granule(bytes);
     used_ -= bytes;
     auto it = free_.emplace(off, bytes).first;
     auto next = std::next(it);
@@ -264,7 +271,7 @@ int fftoc_server_create(size_t hbm_budget_bytes, int max_inflight, fftoc_server 
     s->descs[i].seq = kSeqInvalid;
     s->comps[i].seq = kSeqInvalid;
   }
-  s->pool.reset(hbm_budget_bytes);
+  s->pool.reset(hbm_budget_bytes & ~size_t(255));
   if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
     free_server(s);
     return FFTOC_ERROR_DEVICE;
@@ -315,7 +322,7 @@ int fftoc_server_submit(fftoc_server s, const void *in, int64_t batch, int n, ui
   const int32_t tpi = (int32_t)(deep ? big : std::min(big, spread));
   const int64_t nitems = (ntiles + tpi - 1) / tpi;
   constexpr int64_t kRingItems = (int64_t)kWorkRing / 2;  // keep half the ring as margin
-  if (bytes > s->pool.capacity() || nitems > kRingItems || nitems >= (1 << kItemBits))
+  if (granule(bytes) > s->pool.capacity() || nitems > kRingItems || nitems >= (1 << kItemBits))
     return FFTOC_ERROR_BUDGET;
   if (s->next_seq >= kMaxRequestId) return FFTOC_ERROR_QUEUE_FULL;  // 2^40 per server
   const int slot = (int)(s->next_seq % (uint64_t)s->nslots);
@@ -409,17 +416,26 @@ int fftoc_server_gpu_time(fftoc_server s, uint64_t request_id, uint64_t *ticks) 
 // Replays a request trace against the server: request i arrives at
 // arrival_s[i] seconds after the call starts; requests are admitted in
 // arrival order as slots and budget free up, completions are collected and
-// released.  latency_s[i] = completion seen by the host - arrival.
+// released.  latency_s[i] = completion seen by the host - arrival (NaN for a
+// request that was not served).  Admission that stays refused while none of
+// the trace's requests is in flight (a slot or the budget held by requests
+// submitted outside the trace and not released) ends the replay with
+// FFTOC_ERROR_QUEUE_FULL after kReplayStallSeconds; on any early exit the
+// requests the replay admitted are waited for and released first.
 int fftoc_server_replay(fftoc_server s, const fftoc_job *reqs, int count, const double *arrival_s,
                         double *latency_s, double *span_s) {
   using namespace fftoc;
+  constexpr double kReplayStallSeconds = 1.0;
+  constexpr double kDrainSeconds = 30.0;
   if (!s || !reqs || !arrival_s || !latency_s) return FFTOC_ERROR_NULL_POINTER;
+  if (count < 0) return FFTOC_ERROR_INVALID_BATCH;
   for (int i = 0; i < count; ++i) {
     if (reqs[i].batch <= 0 || !reqs[i].in) return FFTOC_ERROR_INVALID_BATCH;
-    if ((size_t)reqs[i].batch * (size_t)reqs[i].n * ((reqs[i].flags & FFTOC_DATA_C64) ? 8 : 16) >
-        s->pool.capacity())
+    if (granule((size_t)reqs[i].batch * (size_t)reqs[i].n *
+                ((reqs[i].flags & FFTOC_DATA_C64) ? 8 : 16)) > s->pool.capacity())
       return FFTOC_ERROR_BUDGET;
   }
+  for (int i = 0; i < count; ++i) latency_s[i] = std::nan("");
   std::deque<int> waiting;
   struct Live {
     uint64_t id;
@@ -430,23 +446,12 @@ int fftoc_server_replay(fftoc_server s, const fftoc_job *reqs, int count, const 
   int next = 0, completed = 0, rc = FFTOC_SUCCESS;
   double last_check = 0;
   const double t0 = now_s();
-  while (completed < count && rc == FFTOC_SUCCESS) {
-    const double now = now_s() - t0;
-    while (next < count && arrival_s[next] <= now) waiting.push_back(next++);
-    while (!waiting.empty()) {
-      const fftoc_job &r = reqs[waiting.front()];
-      uint64_t id = 0;
-      const int src = fftoc_server_submit(s, r.in, r.batch, r.n, r.flags, &id);
-      if (src == FFTOC_ERROR_QUEUE_FULL) break;
-      if (src != FFTOC_SUCCESS) {
-        rc = src;
-        break;
-      }
-      inflight.push_back({id, (int)(id % (uint64_t)s->nslots), waiting.front()});
-      waiting.pop_front();
-    }
-    // one pass over the completion records, one lock
+  double progress = t0;  // last admission or completion
+  // One pass over the completion records under one lock: releases what has
+  // completed and records its latency.
+  auto collect = [&]() -> int {
     std::lock_guard<std::mutex> lock(s->m);
+    int got = 0;
     for (size_t k = 0; k < inflight.size();) {
       const Live &lv = inflight[k];
       if (__atomic_load_n(&s->comps[lv.slot].seq, __ATOMIC_ACQUIRE) == lv.id) {
@@ -458,19 +463,53 @@ int fftoc_server_replay(fftoc_server s, const fftoc_job *reqs, int count, const 
         s->slot_of.erase(lv.id);
         inflight[k] = inflight.back();
         inflight.pop_back();
-        ++completed;
+        ++got;
       } else {
         ++k;
       }
     }
+    int erc = FFTOC_SUCCESS;
     if (!inflight.empty()) {
       const double t = now_s();
       if (t - last_check > 1e-3) {
         last_check = t;
-        rc = ensure_running(s);
+        erc = ensure_running(s);
       }
     }
+    completed += got;
+    if (got) progress = now_s();
+    return erc;
+  };
+  while (completed < count && rc == FFTOC_SUCCESS) {
+    const double now = now_s() - t0;
+    while (next < count && arrival_s[next] <= now) waiting.push_back(next++);
+    while (!waiting.empty()) {
+      const fftoc_job &r = reqs[waiting.front()];
+      uint64_t id = 0;
+      const int src = fftoc_server_submit(s, r.in, r.batch, r.n, r.flags, &id);
+      if (src == FFTOC_ERROR_QUEUE_FULL) {
+        // nothing of the trace in flight: only a release from outside can
+        // free the slot or the budget
+        if (inflight.empty() && now_s() - progress > kReplayStallSeconds)
+          rc = FFTOC_ERROR_QUEUE_FULL;
+        break;
+      }
+      if (src != FFTOC_SUCCESS) {
+        rc = src;
+        break;
+      }
+      inflight.push_back({id, (int)(id % (uint64_t)s->nslots), waiting.front()});
+      waiting.pop_front();
+      progress = now_s();
+    }
+    const int crc = collect();
+    if (rc == FFTOC_SUCCESS) rc = crc;
   }
+  // an early exit leaves nothing of the trace behind: its admitted requests
+  // finish on the device and their slots and budget are returned
+  const double drain_end = now_s() + kDrainSeconds;
+  while (!inflight.empty() && now_s() < drain_end) (void)collect();
+  if (!inflight.empty() && rc == FFTOC_SUCCESS) rc = FFTOC_ERROR_DEVICE;
   if (span_s) *span_s = now_s() - t0;
   return rc;
 }
@@ -491,7 +530,14 @@ int fftoc_server_set_idle_timeout(fftoc_server s, double seconds) {
   return FFTOC_SUCCESS;
 }
 
+// Stops the loop and frees the server.  The wait for the loop to leave is
+// bounded: the stop flag ends the dispatcher at its next poll and the workers
+// at their next idle check (an item in progress finishes first).  A loop
+// still running after kDestroyWaitSeconds is left alone with the memory it
+// uses (freeing it under a running kernel would fault the device), and
+// FFTOC_ERROR_DEVICE is returned.
 int fftoc_server_destroy(fftoc_server s) {
+  constexpr double kDestroyWaitSeconds = 10.0;
   if (!s) return FFTOC_ERROR_INVALID_HANDLE;
   int cur = 0;
   (void)hipGetDevice(&cur);
@@ -499,8 +545,18 @@ int fftoc_server_destroy(fftoc_server s) {
   int rc = FFTOC_SUCCESS;
   if (s->running) {
     __atomic_store_n(&s->ring->stop, 1u, __ATOMIC_RELEASE);
-    if (hipStreamSynchronize(s->stream) != hipSuccess) rc = FFTOC_ERROR_DEVICE;
+    const double end = fftoc::now_s() + kDestroyWaitSeconds;
+    hipError_t q = hipStreamQuery(s->stream);
+    while (q == hipErrorNotReady && fftoc::now_s() < end) {
+      std::this_thread::sleep_for(std::chrono::microseconds(50));
+      q = hipStreamQuery(s->stream);
+    }
     s->running = false;
+    if (q == hipErrorNotReady) {
+      (void)hipSetDevice(cur);
+      return FFTOC_ERROR_DEVICE;  // the loop's memory stays allocated
+    }
+    if (q != hipSuccess) rc = FFTOC_ERROR_DEVICE;
   }
   fftoc::free_server(s);
   (void)hipSetDevice(cur);

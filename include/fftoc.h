@@ -137,28 +137,38 @@ int fftoc_fft(const void *in, void *out, int64_t batch, int n, uint32_t flags,
 
 /* ---- kernel variants and tuning (no counterpart in the reference, whose
  * UPB/FPU choice was a compile-time sweep, scripts/generate_bench.py:93-107,
- * with results kept in scripts/results/configs.csv) ------------------------ */
-/* Number of kernel variants for length n (0 if n is unsupported). */
-int fftoc_variant_count(int n);
-/* Stable name of variant v of length n ("tensor:..." or "radix:..."). */
-const char *fftoc_variant_name(int n, int variant);
+ * with results kept in scripts/results/configs.csv) ------------------------
+ * Each length has one variant list per data type: `flags` selects it by its
+ * FFTOC_DATA_C64 bit (complex64 data) and is otherwise ignored by the
+ * enumeration calls; a variant index is an index into the list of that data
+ * type, as fftoc_fft_variant and FFTOC_TILE use it. */
+/* Number of kernel variants for length n and data type (0 if n is
+ * unsupported). */
+int fftoc_variant_count(int n, uint32_t flags);
+/* Stable name of variant v of length n and data type ("tensor:..." or
+ * "radix:..."), NULL for an unknown variant. */
+const char *fftoc_variant_name(int n, uint32_t flags, int variant);
 /* Runs one specific variant (same conventions as fftoc_fft). */
 int fftoc_fft_variant(const void *in, void *out, int64_t batch, int n,
                       uint32_t flags, int variant, void *stream);
-/* Variant fftoc_fft uses for length n at this batch size (the choice is kept
- * per floor(log2(batch)) bucket), and its override (batch < 0: every bucket). */
-int fftoc_get_variant(int n, int64_t batch);
-int fftoc_set_variant(int n, int64_t batch, int variant);
-/* Times every variant of length n over `batch` signals (allocates and frees
- * two device scratch buffers, synchronises `stream`) and makes the fastest
- * one of each entry-point family its default for that batch bucket. */
-int fftoc_autotune(int n, int64_t batch, int reps, void *stream);
-/* Median time in microseconds of a variant in the last autotune (<0: none). */
-double fftoc_variant_time_us(int n, int variant);
+/* Variant fftoc_fft uses for length n and data type at this batch size (the
+ * choice is kept per floor(log2(batch)) bucket), and its override (batch < 0:
+ * every bucket). */
+int fftoc_get_variant(int n, uint32_t flags, int64_t batch);
+int fftoc_set_variant(int n, uint32_t flags, int64_t batch, int variant);
+/* Times every variant of length n and data type over `batch` signals
+ * (allocates and frees two device scratch buffers, synchronises `stream`) and
+ * makes the fastest one of each entry-point family its default for that
+ * batch bucket. */
+int fftoc_autotune(int n, uint32_t flags, int64_t batch, int reps, void *stream);
+/* Median time in microseconds of a variant in the last autotune of its
+ * length and data type (<0: none). */
+double fftoc_variant_time_us(int n, uint32_t flags, int variant);
 /* Signals per workgroup of a variant (the FFTOC_TILE value that selects it),
  * or -1 for an unknown variant. */
-int fftoc_variant_tile(int n, int variant);
-/* Persisted tuning table (text: "<family> <n> <log2 batch> <variant>"). */
+int fftoc_variant_tile(int n, uint32_t flags, int variant);
+/* Persisted tuning table, one line per (entry point, length, data type,
+ * batch bucket): "<family> <n> <c128|c64> <log2 batch> <variant name>". */
 int fftoc_tuning_save(const char *path);
 int fftoc_tuning_load(const char *path);
 
@@ -222,7 +232,8 @@ int fftoc_plan_destroy(fftoc_plan plan);
  * wait therefore lasts until the idle timeout after the last request. */
 typedef struct fftoc_server_t *fftoc_server;
 
-/* hbm_budget_bytes: size of the output pool; max_inflight: ring slots. */
+/* hbm_budget_bytes: size of the output pool (rounded down to a multiple of
+ * 256 bytes, the pool's granule); max_inflight: ring slots. */
 int fftoc_server_create(size_t hbm_budget_bytes, int max_inflight, fftoc_server *server);
 /* Admits one request (flags: FFTOC_INVERSE, FFTOC_NORMALIZE, FFTOC_DATA_C64).
  * FFTOC_ERROR_QUEUE_FULL: no slot or budget free now (retry later);
@@ -237,14 +248,20 @@ int fftoc_server_release(fftoc_server server, uint64_t request_id);
 int fftoc_server_gpu_time(fftoc_server server, uint64_t request_id, uint64_t *ticks);
 /* Replays a trace: request i (in, batch, n, flags of reqs[i]; out ignored)
  * arrives arrival_s[i] seconds after the call; admission in arrival order;
- * latency_s[i] = completion seen by the host - arrival; *span_s = wall time. */
+ * latency_s[i] = completion seen by the host - arrival (NaN if not served);
+ * *span_s = wall time.  If admission stays refused for 1 s while none of the
+ * trace's requests is in flight (slots or budget held by requests submitted
+ * outside the trace), returns FFTOC_ERROR_QUEUE_FULL.  On any error the
+ * requests the replay admitted are completed and released before it returns. */
 int fftoc_server_replay(fftoc_server server, const fftoc_job *reqs, int count,
                         const double *arrival_s, double *latency_s, double *span_s);
 int fftoc_server_stats(fftoc_server server, size_t *budget_used, int *inflight);
 /* Idle time after which the device loop leaves (1e-4 .. 60 s); takes effect
  * from the loop's next launch. */
 int fftoc_server_set_idle_timeout(fftoc_server server, double seconds);
-/* Stops the loop (waits for it to drain) and frees everything. */
+/* Stops the loop (waits up to 10 s for it to leave) and frees everything.  A
+ * loop that has not left by then keeps its memory (it is not freed under a
+ * running kernel) and FFTOC_ERROR_DEVICE is returned. */
 int fftoc_server_destroy(fftoc_server server);
 
 #ifdef __cplusplus

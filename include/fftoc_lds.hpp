@@ -22,7 +22,7 @@
 // it synchronises the workgroup.  Element e of signal s is buf[s * N + e].
 #pragma once
 
-#include "../fft_onchip_amd/csrc/stockham.hpp"
+#include "fftoc_stockham.hpp"
 
 namespace fftoc {
 

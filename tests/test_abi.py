@@ -115,8 +115,8 @@ def test_tile_flag_selects_a_variant_or_is_unsupported():
     device work), and a tile contradicting a named variant is rejected."""
     lib = fo.lib()
     n = 256
-    tiles = [lib.fftoc_variant_tile(n, v) for v in range(lib.fftoc_variant_count(n))]
-    assert all(t >= 1 for t in tiles) and lib.fftoc_variant_tile(n, 999) == -1
+    tiles = [lib.fftoc_variant_tile(n, 0, v) for v in range(lib.fftoc_variant_count(n, 0))]
+    assert all(t >= 1 for t in tiles) and lib.fftoc_variant_tile(n, 0, 999) == -1
     fake = 1 << 20  # aligned, never dereferenced: the call fails before any launch
     assert 7 not in tiles
     assert lib.fftoc_fft(fake, fake, 4, n, fo.make_flags(tile=7), None) == 6
@@ -144,3 +144,31 @@ def test_batch_routine_table_needs_no_device():
     assert lib.fftoc_batch_routine_name(64, 0, None) == 3                    # null output
     assert lib.fftoc_batch_routine_name(64, -1, ctypes.byref(name)) == 13    # no such index
     assert lib.fftoc_plan_set_routine(None, 64, 0, b"8x8/tile/S32") == 13    # no plan
+
+
+def test_variant_lists_per_data_type_need_no_device(tmp_path):
+    """Each length has a complex128 and a complex64 variant list; the
+    enumeration calls select one by FFTOC_DATA_C64, names are unique within a
+    list, every tile is positive, and the persisted tuning table keys its lines
+    by data type (written and read back without a device)."""
+    lib = fo.lib()
+    for n in fo.ALL_SIZES:
+        for c64 in (False, True):
+            dt = fo.DATA_C64 if c64 else 0
+            names = fo.variants(n, c64=c64)
+            assert names and len(set(names)) == len(names), (n, c64)
+            assert lib.fftoc_variant_name(n, dt, len(names)) is None
+            assert all(lib.fftoc_variant_tile(n, dt, v) >= 1 for v in range(len(names)))
+            assert 0 <= lib.fftoc_get_variant(n, dt, 1 << 12) < len(names)
+            assert fo.variant_index(n, names[-1], c64=c64) == len(names) - 1
+        # complex64 lists hold no tensor variants (FP64 MFMA only)
+        assert not any(v.startswith("tensor:") for v in fo.variants(n, c64=True))
+    assert lib.fftoc_variant_count(100, 0) == 0
+    path = str(tmp_path / "tune.txt").encode()
+    assert lib.fftoc_tuning_save(path) == 0
+    lines = [ln.split() for ln in open(path) if not ln.startswith("#")]
+    assert {ln[2] for ln in lines} == {"c128", "c64"}
+    assert lib.fftoc_tuning_load(path) == 0
+    bad = tmp_path / "bad.txt"
+    bad.write_text("fft 64 c32 3 radix:8x8/tile/S16\nfft 64 3 radix:8x8/tile/S16\n")
+    assert lib.fftoc_tuning_load(str(bad).encode()) == 13

@@ -43,9 +43,10 @@ def test_complex64_matches_oracle(n, path):
 
 @pytest.mark.parametrize("n", fo.ALL_SIZES)
 def test_every_complex64_variant_matches_oracle(n):
-    """Each complex64 kernel variant (indexes into the complex64 list of the
-    length, until the library reports none), forward and inverse, on a batch
-    that leaves a partial last workgroup."""
+    """Each complex64 kernel variant (the complex64 list of the length, as
+    fftoc_variant_name enumerates it with FFTOC_DATA_C64), forward and
+    inverse, on a batch that leaves a partial last workgroup; one past the
+    last index is rejected."""
     lib = fo.lib()
     batch = max(3, min(515, 65536 // n)) | 1
     x = signals(n, batch, seed=700 + n, c64=True)
@@ -53,20 +54,17 @@ def test_every_complex64_variant_matches_oracle(n):
             True: oracle.reference(x.astype(np.complex128), inverse=True)}
     src = fo.DeviceBuffer.from_array(x)
     dst = fo.DeviceBuffer(x.nbytes)
-    v = 0
-    while True:
-        rc = lib.fftoc_fft_variant(src.ptr, dst.ptr, batch, n, fo.DATA_C64, v, None)
-        if rc != 0:
-            assert rc == 13 and v > 0, (n, v, rc)  # INVALID_HANDLE past the last variant
-            break
+    names = fo.variants(n, c64=True)
+    assert names, n
+    for v, name in enumerate(names):
         for inv in (False, True):
-            print(f"[c64 variant] n={n} v={v} inverse={inv}", flush=True)
+            print(f"[c64 variant] n={n} v={v} {name} inverse={inv}", flush=True)
             flags = fo.DATA_C64 | (fo.INVERSE if inv else 0)
             assert lib.fftoc_fft_variant(src.ptr, dst.ptr, batch, n, flags, v, None) == 0
             fo.synchronize()
             got = dst.download(x.shape, np.complex64)
-            assert np.max(oracle.rel_l2(got, refs[inv], axis=-1)) <= TOL32, (n, v, inv)
-        v += 1
+            assert np.max(oracle.rel_l2(got, refs[inv], axis=-1)) <= TOL32, (n, name, inv)
+    assert lib.fftoc_fft_variant(src.ptr, dst.ptr, batch, n, fo.DATA_C64, len(names), None) == 13
 
 
 def make_jobs(seed, c64_mix=True):
@@ -223,3 +221,28 @@ def test_shutdown_waits_for_plans_and_servers():
     y = np.random.default_rng(1).standard_normal((8, 256)) + 0j
     got = fo.fft_host(y)
     assert np.max(oracle.rel_l2(got, oracle.reference(y), axis=-1)) <= 1e-13
+
+
+def test_plan_jobs_with_more_items_than_the_grid_bit_for_bit():
+    """Jobs of far more work items than the persistent grid has workgroups
+    (about 64 KiB of input per item; 2048 and 1500 items against ~512
+    workgroups) beside small jobs: each equals, bit for bit, the same job in a
+    plan of its own, and its first and last 32 signals match the oracle."""
+    rng = np.random.default_rng(61)
+    specs = [(4096, 2048, False, False), (1024, 6000, True, True), (64, 1000, False, True),
+             (8, 777, True, False), (512, 3001, False, False)]
+    jobs = []
+    for n, batch, c64, inv in specs:
+        ft = np.float32 if c64 else np.float64
+        x = rng.uniform(-1, 1, 2 * n * batch).astype(ft).view(np.complex64 if c64 else np.complex128)
+        flags = (fo.INVERSE if inv else 0) | (fo.DATA_C64 if c64 else 0)
+        jobs.append(dict(n=n, x=x.reshape(batch, n), flags=flags, c64=c64, inv=inv, norm=False))
+    print(f"[plan] {len(jobs)} jobs, two larger than the grid", flush=True)
+    (mixed,), _, _ = run_plan(jobs)
+    for k in (0, 1):
+        (alone,), _, _ = run_plan([jobs[k]])
+        assert np.array_equal(alone[0], mixed[k]), (jobs[k]["n"], jobs[k]["c64"])
+    for j, got in zip(jobs, mixed):
+        b = j["x"].shape[0]
+        for sl in (slice(0, 32), slice(b - 32, b)):
+            check_job(dict(j, x=j["x"][sl]), got[sl])

@@ -109,8 +109,7 @@ def test_pure_tones_land_in_their_bins(path):
 
 
 def _variants(n):
-    lib = fo.lib()
-    return [(v, lib.fftoc_variant_name(n, v).decode()) for v in range(lib.fftoc_variant_count(n))]
+    return list(enumerate(fo.variants(n)))
 
 
 @pytest.mark.parametrize("n", fo.ALL_SIZES)
@@ -173,15 +172,15 @@ def test_tile_flag_runs_a_variant_with_that_tile(n):
     dst = fo.DeviceBuffer(x.nbytes)
     ref = oracle.reference(x)
     for fam, fn in (("radix:", lib.fftoc_fft), ("tensor:", lib.fftoc_tensor_fft)):
-        tiles = sorted({lib.fftoc_variant_tile(n, v) for v, name in _variants(n)
-                        if name.startswith(fam) and lib.fftoc_variant_tile(n, v) <= 255})
+        tiles = sorted({lib.fftoc_variant_tile(n, 0, v) for v, name in _variants(n)
+                        if name.startswith(fam) and lib.fftoc_variant_tile(n, 0, v) <= 255})
         for tile in tiles:
             print(f"[tile] n={n} {fam} tile={tile}", flush=True)
             assert fn(src.ptr, dst.ptr, x.shape[0], n, fo.make_flags(tile=tile), None) == 0
             fo.synchronize()
             assert np.max(oracle.rel_l2(dst.download(x.shape), ref, axis=-1)) <= TOL, (fam, tile)
-    chosen = lib.fftoc_get_variant(n, x.shape[0])
-    tile = lib.fftoc_variant_tile(n, chosen)
+    chosen = lib.fftoc_get_variant(n, 0, x.shape[0])
+    tile = lib.fftoc_variant_tile(n, 0, chosen)
     plain = fo.fft_host(x)
     if tile <= 255:
         assert lib.fftoc_fft(src.ptr, dst.ptr, x.shape[0], n, fo.make_flags(tile=tile), None) == 0
@@ -189,19 +188,59 @@ def test_tile_flag_runs_a_variant_with_that_tile(n):
         assert np.array_equal(dst.download(x.shape), plain)
 
 
-def test_autotune_and_tuning_roundtrip(tmp_path):
+@pytest.mark.parametrize("c64", [False, True])
+def test_autotune_and_tuning_roundtrip(tmp_path, c64):
+    """fftoc_autotune times every variant of the data type's list and makes
+    the fastest the default of its batch bucket; the persisted table restores
+    that choice for that data type only."""
     lib = fo.lib()
-    n = 256
-    assert lib.fftoc_autotune(n, 4096, 3, None) == 0
-    times = [lib.fftoc_variant_time_us(n, v) for v in range(lib.fftoc_variant_count(n))]
+    n = 64 if c64 else 256  # lengths with several variants of the data type
+    dt = fo.DATA_C64 if c64 else 0
+    count = lib.fftoc_variant_count(n, dt)
+    assert count == len(fo.variants(n, c64=c64)) >= 2
+    other_dt = 0 if c64 else fo.DATA_C64
+    before_other = lib.fftoc_get_variant(n, other_dt, 4096)
+    assert lib.fftoc_autotune(n, dt, 4096, 3, None) == 0
+    times = [lib.fftoc_variant_time_us(n, dt, v) for v in range(count)]
     assert all(t > 0 for t in times)
-    best = lib.fftoc_get_variant(n, 4096)
+    best = lib.fftoc_get_variant(n, dt, 4096)
     assert times[best] == min(times)
+    assert lib.fftoc_get_variant(n, other_dt, 4096) == before_other  # other list untouched
     path = str(tmp_path / "tune.txt").encode()
     assert lib.fftoc_tuning_save(path) == 0
-    other = (best + 1) % len(times)
-    assert lib.fftoc_set_variant(n, 4096, other) == 0 and lib.fftoc_get_variant(n, 4096) == other
+    text = open(path).read()
+    assert f"fft {n} {'c64' if c64 else 'c128'} 12 {fo.variants(n, c64=c64)[best]}\n" in text
+    other = (best + 1) % count
+    assert lib.fftoc_set_variant(n, dt, 4096, other) == 0
+    assert lib.fftoc_get_variant(n, dt, 4096) == other
     assert lib.fftoc_tuning_load(path) == 0
-    assert lib.fftoc_get_variant(n, 4096) == best
+    assert lib.fftoc_get_variant(n, dt, 4096) == best
     x = signals(n, 64, seed=9)
-    assert np.max(oracle.rel_l2(fo.fft_host(x), oracle.reference(x), axis=-1)) <= TOL
+    if c64:
+        x = x.astype(np.complex64)
+    tol = 2e-6 if c64 else TOL
+    got = fo.fft_host(x)
+    assert np.max(oracle.rel_l2(got, oracle.reference(x.astype(np.complex128)), axis=-1)) <= tol
+
+
+@pytest.mark.parametrize("n", fo.TENSOR_SIZES)
+def test_per_size_tensor_aliases_on_data(n):
+    """Each per-size entry point fftoc_tensor_fft_N (the drop-in for
+    tensor_fft_N<..., N, ...>) transforms data like fftoc_tensor_fft,
+    forward and normalised inverse, out of place and in place."""
+    x = signals(n, batch_for(n), seed=300 + n)
+    src = fo.DeviceBuffer.from_array(x)
+    dst = fo.DeviceBuffer(x.nbytes)
+    alias = getattr(fo, f"tensor_fft_{n}")
+    print(f"[alias] fftoc_tensor_fft_{n} batch={x.shape[0]}", flush=True)
+    alias(src, x.shape[0], out=dst)
+    fo.synchronize()
+    got = dst.download(x.shape)
+    assert np.max(oracle.rel_l2(got, oracle.reference(x), axis=-1)) <= TOL, n
+    generic = fo.fft_host(x, path="tensor")
+    assert np.array_equal(got, generic), n
+    alias(src, x.shape[0], flags=fo.INVERSE | fo.NORMALIZE)   # in place
+    fo.synchronize()
+    ref = oracle.reference(x, inverse=True)
+    got = src.download(x.shape)
+    assert np.max(oracle.rel_l2(got, oracle.DD(ref.hi / n, ref.lo / n), axis=-1)) <= TOL, n

@@ -2,9 +2,12 @@
 FFTOC_PREC_BF16X3 on fftoc_tensor_fft, n = 64, 256, 4096).
 
 Inputs follow the dynamic-range stress case: re/im N(0,1) with 1% of the
-entries scaled by 1e4.  Tolerance: relative L2 error per signal <= 5e-6
-against the long-double oracle for both forms (FP32 accumulation; observed
-errors are ~1e-7); the FP64 path on the same inputs stays <= 1e-13.
+entries scaled by 1e4.  Tolerance: relative L2 error per signal against the
+long-double oracle, per form: FP32 stages <= 1e-6 (observed maxima 1.7e-7 -
+2.1e-7 over n = 64, 256, 4096), split-BF16 stages <= 1e-6 (observed
+1.2e-7 - 1.3e-7), both above 1e-9 (really computed in reduced precision).
+The two forms must give different bits on the same input, so one cannot fall
+back to the other unseen.  The FP64 path on the same inputs stays <= 1e-13.
 """
 import numpy as np
 import pytest
@@ -13,7 +16,8 @@ import fft_onchip_amd as fo
 import oracle
 
 pytestmark = pytest.mark.gpu
-TOL_SPLIT = 5e-6
+TOL_SPLIT = {"fp32": 1e-6, "bf16x3": 1e-6}
+FLOOR_SPLIT = 1e-9
 
 
 def stress_signals(n, batch, seed):
@@ -40,12 +44,23 @@ def test_split_precision_within_tolerance(n, prec):
     mabs = oracle.max_abs(got, ref, axis=-1)
     print(f"[precision] n={n} {prec}: max rel L2 {err.max():.3e}, median {np.median(err):.3e}, "
           f"max abs {mabs.max():.3e}", flush=True)
-    assert err.max() <= TOL_SPLIT
-    assert err.max() > 1e-12  # really computed in reduced precision
+    assert err.max() <= TOL_SPLIT[prec]
+    assert err.max() > FLOOR_SPLIT  # really computed in reduced precision
     # inverse with FP64 normalisation
     refi = oracle.reference(x, inverse=True)
     goti = fo.fft_host(x, path="tensor", precision=prec, inverse=True, normalize=True)
-    assert np.max(oracle.rel_l2(goti, oracle.DD(refi.hi / n, refi.lo / n), axis=-1)) <= TOL_SPLIT
+    assert np.max(oracle.rel_l2(goti, oracle.DD(refi.hi / n, refi.lo / n), axis=-1)) <= TOL_SPLIT[prec]
+
+
+@pytest.mark.parametrize("n", [64, 256, 4096])
+def test_split_forms_are_distinct(n):
+    """FP32 stages and split-BF16 stages round differently: their outputs
+    differ from each other and from the FP64 path on the same input."""
+    x = stress_signals(n, 17, seed=13)
+    outs = {p: fo.fft_host(x, path="tensor", precision=p) for p in ("fp64", "fp32", "bf16x3")}
+    assert not np.array_equal(outs["fp32"], outs["bf16x3"]), n
+    assert not np.array_equal(outs["fp32"], outs["fp64"]), n
+    assert not np.array_equal(outs["bf16x3"], outs["fp64"]), n
 
 
 @pytest.mark.parametrize("n", [64, 4096])

@@ -183,3 +183,149 @@ def test_server_replay_poisson_trace():
     assert np.all(np.isfinite(lat)) and np.all(lat > 0)
     print(f"[server] replay {count} requests in {span:.3f} s: p50 {np.percentile(lat, 50)*1e6:.0f} us, "
           f"p99 {np.percentile(lat, 99)*1e6:.0f} us", flush=True)
+
+
+def test_server_slow_stream_every_request_completes():
+    """About 3 s of 1-signal requests at ~100 per second with the default idle
+    timeout.  Each CU's primary worker holds a ticket far ahead of the
+    published items (request k's item is ticket k), so holders wait seconds
+    for their item while the dispatcher, reset by every arrival, never times
+    out.  Every request must complete within a host-side deadline: a worker
+    that gave its ticket up on its own timer would leave that request's item
+    unrun for good."""
+    x = signals(8, 1, seed=31)
+    buf = fo.DeviceBuffer.from_array(x)
+    ref = oracle.reference(x)
+    with fo.Server(1 << 20, 8) as srv:
+        t0 = time.perf_counter()
+        # no HIP call inside the stream (a synchronous copy can share the
+        # loop's hardware queue and wait for the loop to time out, which would
+        # restart the loop and its tickets every request): polls read host
+        # memory only
+        for k in range(300):
+            rid = submit_retry(srv, buf, 1, 8, 0, timeout=5.0)
+            srv.wait(rid, timeout_s=2.0)
+            srv.release(rid)
+            if k % 100 == 0:
+                print(f"[server] slow stream: request {k} at {time.perf_counter() - t0:.2f} s",
+                      flush=True)
+            # pace at ~100 requests per second
+            delay = t0 + 0.01 * (k + 1) - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+        assert time.perf_counter() - t0 > 2.5
+        rid = submit_retry(srv, buf, 1, 8, 0)
+        got = hip.download_ptr(srv.wait(rid), x.shape)
+        srv.release(rid)
+    assert np.max(oracle.rel_l2(got, ref, axis=-1)) <= TOL64
+
+
+# Requests of the load test, in submission order: (n, batch, complex64,
+# inverse, normalise).  With nothing released, the output pool fills as they
+# are submitted: the first two are sized with the light-load rule, every later
+# one finds more than 256 MiB of outputs in flight and gets the deep-backlog
+# item size (256 KiB - 4 MiB, growing with the backlog).  A, C and I have
+# more work items (1024, 1024, ~560) than the loop has workers (511), so
+# their items are counted in pairs.
+DEEP_REQUESTS = [
+    (4096, 4096, False, False, False),   # A 256 MiB
+    (512, 32768, True, False, False),    # B 128 MiB
+    (2048, 8192, False, True, False),    # C 256 MiB
+    (8, 1 << 20, False, False, False),   # D 128 MiB
+    (64, 200003, True, True, True),      # E
+    (256, 50001, False, False, True),    # F
+    (4096, 2001, True, False, False),    # G
+    (16, 12345, False, True, False),     # H
+    (1024, 20000, False, False, False),  # I 312 MiB
+    (128, 70001, True, False, False),    # J
+    (32, 99999, False, False, False),    # K
+]
+
+
+def _load_inputs(seed):
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for n, batch, c64, inv, norm in DEEP_REQUESTS:
+        ft = np.float32 if c64 else np.float64
+        x = rng.uniform(-1, 1, 2 * n * batch).astype(ft).view(np.complex64 if c64 else np.complex128)
+        x = x.reshape(batch, n)
+        flags = (fo.INVERSE if inv else 0) | (fo.NORMALIZE if norm else 0) | (fo.DATA_C64 if c64 else 0)
+        reqs.append(dict(n=n, x=x, flags=flags, c64=c64, inv=inv, norm=norm,
+                         buf=fo.DeviceBuffer.from_array(x)))
+    return reqs
+
+
+def _check_edges(r, got, edge=32):
+    """First and last `edge` signals against the oracle."""
+    b = r["x"].shape[0]
+    for sl in (slice(0, min(edge, b)), slice(max(0, b - edge), b)):
+        ref = oracle.reference(r["x"][sl].astype(np.complex128), inverse=r["inv"])
+        if r["norm"]:
+            ref = oracle.DD(ref.hi / r["n"], ref.lo / r["n"])
+        err = np.max(oracle.rel_l2(got[sl], ref, axis=-1))
+        assert err <= (TOL32 if r["c64"] else TOL64), (r["n"], r["x"].shape, r["c64"], err)
+
+
+def test_server_deep_backlog_matches_alone_bit_for_bit():
+    """Past the deep-backlog threshold (backlog-sized items) and with requests
+    of more items than workers (paired completion counts), every request's
+    output equals, bit for bit, the same request served alone on an idle
+    server (light-load item sizes), and its first and last 32 signals match
+    the oracle."""
+    reqs = _load_inputs(41)
+    total = sum(r["x"].nbytes for r in reqs)
+    with fo.Server(4 << 30, 64) as srv:
+        print(f"[server] deep backlog: {len(reqs)} requests, {total / 2**20:.0f} MiB", flush=True)
+        ids = [submit_retry(srv, r["buf"], r["x"].shape[0], r["n"], r["flags"]) for r in reqs]
+        used, inflight = srv.stats()
+        assert used >= total and inflight == len(reqs)
+        mixed = []
+        for r, rid in zip(reqs, ids):
+            mixed.append(hip.download_ptr(srv.wait(rid, timeout_s=30.0), r["x"].shape, r["x"].dtype))
+        for rid in ids:
+            srv.release(rid)
+        assert srv.stats() == (0, 0)
+        for k, r in enumerate(reqs):
+            print(f"[server] alone: n={r['n']} batch={r['x'].shape[0]} c64={r['c64']}", flush=True)
+            rid = submit_retry(srv, r["buf"], r["x"].shape[0], r["n"], r["flags"])
+            alone = hip.download_ptr(srv.wait(rid, timeout_s=30.0), r["x"].shape, r["x"].dtype)
+            srv.release(rid)
+            assert np.array_equal(alone, mixed[k]), (r["n"], r["x"].shape, r["c64"])
+    for r, got in zip(reqs, mixed):
+        _check_edges(r, got)
+
+
+def test_server_replay_refuses_a_stall_and_drains():
+    """A slot held outside the trace makes admission impossible: the replay
+    returns QUEUE_FULL after its stall bound (it used to spin forever), and the
+    server still serves afterwards.  A budget that is not a multiple of the
+    pool's 256-byte granule is rounded down, so a request in the gap is
+    refused as over budget instead of waiting forever."""
+    x = signals(64, 4, seed=51)
+    buf = fo.DeviceBuffer.from_array(x)
+    with fo.Server(1 << 20, 1) as srv:
+        held = srv.submit(buf, 4, 64, 0)
+        assert held is not None
+        srv.wait(held)
+        t0 = time.perf_counter()
+        with pytest.raises(fo.FftocError) as e:
+            srv.replay([(buf, 4, 64, 0)] * 3, [0.0, 0.0, 0.0])
+        assert e.value.name == "QUEUE_FULL"
+        assert time.perf_counter() - t0 < 10.0
+        srv.release(held)
+        lat, _ = srv.replay([(buf, 4, 64, 0)] * 3, [0.0, 0.0, 0.0])
+        assert np.all(np.isfinite(lat))
+        assert srv.stats() == (0, 0)
+    with fo.Server(4096 + 100, 4) as srv:
+        y = signals(8, 33, seed=52)                  # 4224 bytes: in the 100-byte gap
+        ybuf = fo.DeviceBuffer.from_array(y)
+        with pytest.raises(fo.FftocError) as e:
+            srv.submit(ybuf, 33, 8, 0)
+        assert e.value.name == "BUDGET"
+        with pytest.raises(fo.FftocError) as e:
+            srv.replay([(ybuf, 33, 8, 0)], [0.0])
+        assert e.value.name == "BUDGET"
+        rid = srv.submit(ybuf, 32, 8, 0)             # 4096 bytes fit
+        got = hip.download_ptr(srv.wait(rid), (32, 8))
+        srv.release(rid)
+        assert np.max(oracle.rel_l2(got, oracle.reference(y[:32]), axis=-1)) <= TOL64
