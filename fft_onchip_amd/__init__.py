@@ -232,6 +232,23 @@ class Plan:
             pass
 
 
+NOALLOC_MAX_JOBS = 32
+
+
+def batch_fft_noalloc(jobs, stream=None) -> None:
+    """One launch over up to NOALLOC_MAX_JOBS jobs (in_buf, out_buf, batch, n,
+    flags) that allocates nothing (fftoc_batch_fft_noalloc): same results as
+    a Plan of the same jobs.  Calls overlapping on one device race."""
+    jobs = list(jobs)
+    arr = (_native.Job * max(1, len(jobs)))()
+    for i, (src, dst, batch, n, flags) in enumerate(jobs):
+        arr[i] = _native.Job(_ptr(src), _ptr(dst) if dst is not None else _ptr(src),
+                             int(batch), int(n), int(flags))
+    rc = lib().fftoc_batch_fft_noalloc(arr, len(jobs), _stream(stream))
+    if rc != 0:
+        raise FftocError(rc, "fftoc_batch_fft_noalloc")
+
+
 def batch_routines(n: int) -> list:
     """Names of the tile routines plans and servers can use for length n."""
     names, i = [], 0

@@ -57,6 +57,7 @@ SIGNATURES.update({
     "fftoc_plan_set_routine": ([_vp, _int, _u32, ctypes.c_char_p], _int),
     "fftoc_batch_routine_name": ([_int, _int, ctypes.POINTER(ctypes.c_char_p)], _int),
     "fftoc_plan_destroy": ([_vp], _int),
+    "fftoc_batch_fft_noalloc": ([ctypes.POINTER(Job), _int, _vp], _int),
     "fftoc_server_create": ([ctypes.c_size_t, _int, ctypes.POINTER(_vp)], _int),
     "fftoc_server_submit": ([_vp, _vp, _i64, _int, _u32, ctypes.POINTER(ctypes.c_uint64)], _int),
     "fftoc_server_poll": ([_vp, ctypes.c_uint64, ctypes.POINTER(_int), ctypes.POINTER(_vp)], _int),

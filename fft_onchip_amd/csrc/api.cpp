@@ -20,20 +20,28 @@ namespace {
 struct DeviceState {
   bool ready = false;
   int status = FFTOC_SUCCESS;
-  void *twiddles = nullptr;    // kTwN x double2
+  void *twiddles = nullptr;    // kTwN x double2 (module global of device_tables.hip)
   void *twiddles32 = nullptr;  // kTwN x float2, rounded from the FP64 table
 };
 
 std::mutex g_mutex;
 DeviceState g_dev[kMaxDevices];
 
+// Host copies of the tables, built once (static storage: initialising a
+// device allocates nothing, on the host or on the device).
+constexpr int kTw = 4096;
+double g_tw_host[2 * kTw];
+float g_tw32_host[2 * kTw];
+bool g_tw_built = false;
+
 // W_4096^j = exp(-2 pi i j / 4096) to FP64.  The first octant (j <= 512) is
 // evaluated with long-double cosl/sinl; every other entry is an exact
 // symmetry of it (swap / negate), so all 4096 values are the correctly
-// rounded FP64 values of long-double results.
-std::vector<double> build_twiddles() {
-  const int n = 4096;
-  std::vector<long double> c(n / 8 + 1), s(n / 8 + 1);
+// rounded FP64 values of long-double results.  Called with g_mutex held.
+void build_twiddles() {
+  if (g_tw_built) return;
+  const int n = kTw;
+  long double c[kTw / 8 + 1], s[kTw / 8 + 1];
   const long double two_pi = 6.283185307179586476925286766559005768L;
   for (int j = 0; j <= n / 8; ++j) {
     const long double ang = two_pi * (long double)j / (long double)n;
@@ -41,7 +49,6 @@ std::vector<double> build_twiddles() {
     s[j] = sinl(ang);
   }
   c[n / 8] = s[n / 8] = 0.707106781186547524400844362104849039L;
-  std::vector<double> t(2 * n);
   for (int j = 0; j < n; ++j) {
     const int q = j / (n / 4);        // quadrant
     const int r = j - q * (n / 4);    // 0 .. n/4-1
@@ -60,12 +67,16 @@ std::vector<double> build_twiddles() {
       case 2: cv = -cr; sv = -sr; break;
       default: cv = sr; sv = -cr; break;
     }
-    t[2 * j] = (double)cv;
-    t[2 * j + 1] = (double)(-sv);  // forward root: exp(-i angle)
+    g_tw_host[2 * j] = (double)cv;
+    g_tw_host[2 * j + 1] = (double)(-sv);  // forward root: exp(-i angle)
   }
-  return t;
+  for (int i = 0; i < 2 * n; ++i) g_tw32_host[i] = (float)g_tw_host[i];
+  g_tw_built = true;
 }
 
+// Sets up the current device once: checks it is a gfx950, copies the tables
+// into its module globals and locates the allocation-free batch entry point's
+// scratch array on it.  Allocates nothing.
 int init_current_device(int *dev_out) {
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess) return FFTOC_ERROR_DEVICE;
@@ -80,21 +91,14 @@ int init_current_device(int *dev_out) {
     // kernels are built for gfx950 only
     return FFTOC_ERROR_DEVICE;
   }
-  std::vector<double> tw = build_twiddles();
-  std::vector<float> tw32(tw.begin(), tw.end());
+  build_twiddles();
   void *d = nullptr, *d32 = nullptr;
-  if (hipMalloc(&d, tw.size() * sizeof(double)) != hipSuccess) return FFTOC_ERROR_ALLOC;
-  if (hipMalloc(&d32, tw32.size() * sizeof(float)) != hipSuccess) {
-    (void)hipFree(d);
-    return FFTOC_ERROR_ALLOC;
-  }
-  if (hipMemcpy(d, tw.data(), tw.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(d32, tw32.data(), tw32.size() * sizeof(float), hipMemcpyHostToDevice) !=
-          hipSuccess) {
-    (void)hipFree(d);
-    (void)hipFree(d32);
+  if (device_table_addresses(&d, &d32) != FFTOC_SUCCESS) return FFTOC_ERROR_DEVICE;
+  if (hipMemcpy(d, g_tw_host, sizeof g_tw_host, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(d32, g_tw32_host, sizeof g_tw32_host, hipMemcpyHostToDevice) != hipSuccess)
     return FFTOC_ERROR_DEVICE;
-  }
+  const int rc = batch_noalloc_setup(dev);
+  if (rc != FFTOC_SUCCESS) return rc;
   st.twiddles = d;
   st.twiddles32 = d32;
   st.status = FFTOC_SUCCESS;
@@ -393,18 +397,9 @@ int fftoc_shutdown(void) {
   std::lock_guard<std::mutex> lock(fftoc::g_mutex);
   // a plan's or server's kernels read the tables
   if (fftoc::g_table_holders.load() > 0) return FFTOC_ERROR_UNSUPPORTED;
-  int cur = 0;
-  (void)hipGetDevice(&cur);
-  for (int d = 0; d < fftoc::kMaxDevices; ++d) {
-    fftoc::DeviceState &st = fftoc::g_dev[d];
-    if (st.twiddles || st.twiddles32) {
-      (void)hipSetDevice(d);
-      if (st.twiddles) (void)hipFree(st.twiddles);
-      if (st.twiddles32) (void)hipFree(st.twiddles32);
-    }
-    st = fftoc::DeviceState{};
-  }
-  (void)hipSetDevice(cur);
+  // the tables are module globals that live as long as the library is
+  // loaded; a device is set up again (tables copied in) on its next use
+  for (int d = 0; d < fftoc::kMaxDevices; ++d) fftoc::g_dev[d] = fftoc::DeviceState{};
   return FFTOC_SUCCESS;
 }
 

@@ -80,6 +80,16 @@ constexpr int kCuKeys = 2048;                // (XCC, SE, SH, CU) of a workgroup
 int launch_server(const ServerDev &sd, int grid, hipStream_t stream);
 int server_grid(int max_per_cu);
 
+// ---- allocation-free batches (fftoc_batch_fft_noalloc) ----------------------
+// The descriptors travel as a kernel argument into a fixed per-device scratch
+// array of the library, which also holds the work counters.
+constexpr int kNoAllocMaxJobs = 32;
+struct NoAllocJobs {
+  JobDev j[kNoAllocMaxJobs];
+};
+int launch_batch_noalloc(const NoAllocJobs &jobs, int njobs, int64_t total_items,
+                         const void *tw64, const void *tw32, int grid, hipStream_t stream);
+
 int batch_routine_count();
 int batch_routine_default(int n, bool c64);  // untuned routine of a length / data type
 int batch_routine_n(int id);

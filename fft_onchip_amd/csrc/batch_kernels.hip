@@ -549,6 +549,61 @@ int batch_resident_blocks() {
   return (int)resident_workgroups<batch_kernel, kBatchThreads>();
 }
 
+// ---- allocation-free batches --------------------------------------------------
+// Scratch of fftoc_batch_fft_noalloc: the job descriptors (JobDev[32]) and the
+// two work counters, 4 KiB per device as a module global.  Host code never
+// passes `scratch` itself to a kernel (on the host it names a stand-in, not
+// device memory): it uses the address hipGetSymbolAddress returned for the
+// current device at set-up.  Every call on a device shares its array, so calls
+// that overlap on one device race.
+constexpr size_t kNoAllocScratchBytes = 4096;
+constexpr size_t kNoAllocCtrOffset = kNoAllocMaxJobs * sizeof(JobDev);
+static_assert(kNoAllocCtrOffset % 8 == 0 &&
+                  kNoAllocCtrOffset + 2 * sizeof(unsigned long long) <= kNoAllocScratchBytes,
+              "scratch layout");
+static_assert(sizeof(JobDev) % 8 == 0, "descriptors are copied as 8-byte words");
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+void *g_scratch_ptr[kMaxDevices];  // device address of `scratch`, per device
+
+int batch_noalloc_setup(int dev) {
+  if (dev < 0 || dev >= kMaxDevices) return FFTOC_ERROR_DEVICE;
+  void *ptr = nullptr;
+  if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr)
+    return FFTOC_ERROR_DEVICE;
+  g_scratch_ptr[dev] = ptr;
+  return FFTOC_SUCCESS;
+}
+
+namespace {
+// Copies the descriptors from the kernel argument into the scratch array and
+// clears the work counters (a kernel, not a memset: the launch sequence stays
+// capturable into a graph).
+__global__ __launch_bounds__(64) void batch_noalloc_setup_kernel(NoAllocJobs jobs, int njobs,
+                                                                 unsigned char *sp) {
+  const uint64_t *src = reinterpret_cast<const uint64_t *>(&jobs.j[0]);
+  uint64_t *dst = reinterpret_cast<uint64_t *>(sp);
+  const int words = njobs * (int)(sizeof(JobDev) / 8);
+  for (int w = threadIdx.x; w < words; w += 64) dst[w] = src[w];
+  if (threadIdx.x < 2)
+    reinterpret_cast<unsigned long long *>(sp + kNoAllocCtrOffset)[threadIdx.x] = 0ull;
+}
+}  // namespace
+
+int launch_batch_noalloc(const NoAllocJobs &jobs, int njobs, int64_t total_items,
+                         const void *tw64, const void *tw32, int grid, hipStream_t stream) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return FFTOC_ERROR_DEVICE;
+  unsigned char *sp = static_cast<unsigned char *>(g_scratch_ptr[dev]);
+  if (!sp) return FFTOC_ERROR_DEVICE;  // the device was not set up
+  if (njobs < 1 || njobs > kNoAllocMaxJobs || grid <= 0) return FFTOC_ERROR_INVALID_BATCH;
+  hipLaunchKernelGGL(batch_noalloc_setup_kernel, dim3(1), dim3(64), 0, stream, jobs, njobs, sp);
+  hipLaunchKernelGGL(batch_kernel, dim3(grid), dim3(kBatchThreads), 0, stream,
+                     reinterpret_cast<const JobDev *>(sp), njobs, total_items,
+                     reinterpret_cast<unsigned long long *>(sp + kNoAllocCtrOffset),
+                     static_cast<const c2 *>(tw64), static_cast<const f2 *>(tw32));
+  return hipGetLastError() == hipSuccess ? FFTOC_SUCCESS : FFTOC_ERROR_LAUNCH;
+}
+
 int launch_batch(const JobDev *d_jobs, int njobs, int64_t total_items, unsigned long long *d_ctr,
                  const void *tw64, const void *tw32, int grid, hipStream_t stream) {
   if (grid <= 0) return FFTOC_ERROR_INVALID_BATCH;

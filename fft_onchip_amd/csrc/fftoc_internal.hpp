@@ -57,6 +57,12 @@ int launch_tensor_precision(int n, const LaunchArgs &a);
 
 // Per-device tables shared with the plan / serving code (api.cpp).
 int device_tables(const void **tw64, const void **tw32);
+// device_tables.hip: the current device's copies of the module-global tables.
+int device_table_addresses(void **tw64, void **tw32);
+// batch_kernels.hip: locates the allocation-free batch entry point's scratch
+// array on device `dev` (which must be current); called once per device by
+// the device set-up.
+int batch_noalloc_setup(int dev);
 // Plans and servers hold the tables while they exist: fftoc_shutdown refuses
 // to free them until every holder is destroyed.
 void hold_tables();

@@ -86,6 +86,35 @@ int check_cross_overlap(const fftoc_job *jobs, int njobs) {
 }
 
 
+// Descriptor of one job whose work items start at item_begin: about 64 KiB
+// of input per item.
+JobDev make_jobdev(const fftoc_job &j, int routine, int64_t item_begin) {
+  const bool c64 = (j.flags & FFTOC_DATA_C64) != 0;
+  JobDev d{};
+  d.in = j.in;
+  d.out = j.out;
+  d.batch = j.batch;
+  d.routine = routine;
+  const int S = batch_routine_signals(d.routine);
+  d.ntiles = (j.batch + S - 1) / S;
+  const int64_t tile_bytes = (int64_t)S * j.n * (c64 ? 8 : 16);
+  d.tiles_per_item = (int32_t)std::max<int64_t>(1, 65536 / tile_bytes);
+  d.item_begin = item_begin;
+  d.inverse = (j.flags & FFTOC_INVERSE) ? 1 : 0;
+  d.c64 = c64 ? 1 : 0;
+  d.scale = (j.flags & FFTOC_NORMALIZE) ? 1.0 / (double)j.n : 1.0;
+  return d;
+}
+
+int64_t job_items(const JobDev &d) { return (d.ntiles + d.tiles_per_item - 1) / d.tiles_per_item; }
+
+// Device order of the jobs: heaviest items first (longer transforms, then
+// bigger batches); stable.
+bool heavier(const fftoc_job &a, const fftoc_job &b) {
+  if (a.n != b.n) return a.n > b.n;
+  return a.batch > b.batch;
+}
+
 // Lays the jobs out as work items and uploads the descriptors.
 int layout(fftoc_plan_t *p) {
   p->host.clear();
@@ -93,22 +122,9 @@ int layout(fftoc_plan_t *p) {
   for (int idx : p->order) {
     const fftoc_job &j = p->jobs[idx];
     if (j.batch == 0) continue;
-    const bool c64 = (j.flags & FFTOC_DATA_C64) != 0;
-    JobDev d{};
-    d.in = j.in;
-    d.out = j.out;
-    d.batch = j.batch;
-    d.routine = p->routine[lg(j.n)][c64 ? 1 : 0];
-    const int S = batch_routine_signals(d.routine);
-    d.ntiles = (j.batch + S - 1) / S;
-    // about 64 KiB of input per claimed item
-    const int64_t tile_bytes = (int64_t)S * j.n * (c64 ? 8 : 16);
-    d.tiles_per_item = (int32_t)std::max<int64_t>(1, 65536 / tile_bytes);
-    d.item_begin = p->total_items;
-    d.inverse = (j.flags & FFTOC_INVERSE) ? 1 : 0;
-    d.c64 = c64 ? 1 : 0;
-    d.scale = (j.flags & FFTOC_NORMALIZE) ? 1.0 / (double)j.n : 1.0;
-    p->total_items += (d.ntiles + d.tiles_per_item - 1) / d.tiles_per_item;
+    const JobDev d =
+        make_jobdev(j, p->routine[lg(j.n)][(j.flags & FFTOC_DATA_C64) ? 1 : 0], p->total_items);
+    p->total_items += job_items(d);
     p->host.push_back(d);
   }
   if (p->d_jobs) {
@@ -174,10 +190,8 @@ int fftoc_plan_create(const fftoc_job *jobs, int njobs, fftoc_plan *out) {
   p->order.resize(njobs);
   for (int i = 0; i < njobs; ++i) p->order[i] = i;
   // heaviest items first: longer transforms, then bigger batches
-  std::stable_sort(p->order.begin(), p->order.end(), [&](int a, int b) {
-    if (p->jobs[a].n != p->jobs[b].n) return p->jobs[a].n > p->jobs[b].n;
-    return p->jobs[a].batch > p->jobs[b].batch;
-  });
+  std::stable_sort(p->order.begin(), p->order.end(),
+                   [&](int a, int b) { return heavier(p->jobs[a], p->jobs[b]); });
   for (int k = 0; k < 13; ++k) {
     p->routine[k][0] = batch_routine_default(1 << k, false);
     p->routine[k][1] = batch_routine_default(1 << k, true);
@@ -347,6 +361,67 @@ int fftoc_plan_destroy(fftoc_plan p) {
   if (p->holds_tables) fftoc::drop_tables();
   delete p;
   return FFTOC_SUCCESS;
+}
+
+// One launch over up to kNoAllocMaxJobs jobs with no allocation: the same
+// validation, job order, tile routines (the untuned defaults) and work items
+// as a plan of these jobs, with the descriptors and the work counter in the
+// library's fixed per-device scratch array instead of plan memory.  Uses no
+// heap either: the descriptors are built on the stack and reach the device as
+// a kernel argument.
+int fftoc_batch_fft_noalloc(const fftoc_job *jobs, int njobs, void *stream) {
+  using namespace fftoc;
+  if (njobs < 0 || (njobs > 0 && !jobs)) return FFTOC_ERROR_NULL_POINTER;
+  if (njobs > kNoAllocMaxJobs) return FFTOC_ERROR_BUDGET;
+  for (int i = 0; i < njobs; ++i) {
+    const int rc = validate_job(jobs[i]);
+    if (rc != FFTOC_SUCCESS) return rc;
+  }
+  // every job's output disjoint from every other job's input and output
+  auto range = [](const void *p, const fftoc_job &j, uintptr_t *lo, uintptr_t *hi) {
+    *lo = reinterpret_cast<uintptr_t>(p);
+    *hi = *lo + (uintptr_t)j.batch * (uintptr_t)j.n * ((j.flags & FFTOC_DATA_C64) ? 8u : 16u);
+  };
+  auto meets = [](uintptr_t a0, uintptr_t a1, uintptr_t b0, uintptr_t b1) {
+    return a0 < b1 && b0 < a1;
+  };
+  for (int a = 0; a < njobs; ++a) {
+    if (jobs[a].batch == 0) continue;
+    uintptr_t ao0, ao1;
+    range(jobs[a].out, jobs[a], &ao0, &ao1);
+    for (int b = 0; b < njobs; ++b) {
+      if (b == a || jobs[b].batch == 0) continue;
+      uintptr_t bi0, bi1, bo0, bo1;
+      range(jobs[b].in, jobs[b], &bi0, &bi1);
+      range(jobs[b].out, jobs[b], &bo0, &bo1);
+      if (meets(ao0, ao1, bi0, bi1) || meets(ao0, ao1, bo0, bo1)) return FFTOC_ERROR_OVERLAP;
+    }
+  }
+  int order[kNoAllocMaxJobs];
+  int live = 0;
+  for (int i = 0; i < njobs; ++i) {
+    if (jobs[i].batch == 0) continue;
+    int k = live++;
+    while (k > 0 && heavier(jobs[i], jobs[order[k - 1]])) {  // stable insertion
+      order[k] = order[k - 1];
+      --k;
+    }
+    order[k] = i;
+  }
+  if (live == 0) return FFTOC_SUCCESS;
+  const void *tw64 = nullptr, *tw32 = nullptr;
+  const int rc = device_tables(&tw64, &tw32);  // sets the device up on first use
+  if (rc != FFTOC_SUCCESS) return rc;
+  NoAllocJobs desc;
+  int64_t total = 0;
+  for (int k = 0; k < live; ++k) {
+    const fftoc_job &j = jobs[order[k]];
+    desc.j[k] = make_jobdev(j, batch_routine_default(j.n, (j.flags & FFTOC_DATA_C64) != 0), total);
+    total += job_items(desc.j[k]);
+  }
+  const int grid = (int)std::min<int64_t>(batch_resident_blocks(), total);
+  return launch_batch_noalloc(desc, live, total, tw64, tw32, grid,
+                              static_cast<hipStream_t>(stream));
 }
 
 }  // extern "C"

@@ -212,6 +212,23 @@ int fftoc_plan_set_routine(fftoc_plan plan, int n, uint32_t flags, const char *n
 int fftoc_batch_routine_name(int n, int index, const char **name);
 int fftoc_plan_destroy(fftoc_plan plan);
 
+/* ---- allocation-free batch ------------------------------------------------
+ * No counterpart in the reference (nor in the plan calls above, which allocate
+ * at create).  For callers that cannot allocate device memory: code under
+ * hipGraph stream capture, tight serving loops, harnesses that forbid
+ * hipMalloc.  One launch over `njobs` jobs, with the same validation, results
+ * and ordering as fftoc_plan_create + fftoc_plan_execute of the same jobs
+ * (untuned tile routines), asynchronous on `stream`.  It calls no allocating
+ * routine: the job descriptors and the work counter live in a fixed 4 KiB
+ * scratch array per device that the library holds until the process ends
+ * (the device's first use of the library locates it; twiddle tables are
+ * module globals too).  At most FFTOC_NOALLOC_MAX_JOBS jobs per call
+ * (FFTOC_ERROR_BUDGET beyond).  Every call on a device shares that device's
+ * array: calls that overlap in time on one device race and give wrong results
+ * -- order them (one stream, or events). */
+#define FFTOC_NOALLOC_MAX_JOBS 32
+int fftoc_batch_fft_noalloc(const fftoc_job *jobs, int njobs, void *stream);
+
 /* ---- serving: device-resident request loop with HBM-budget admission -----
  * No counterpart in the reference.  fftoc_server_create launches a
  * persistent kernel that serves requests posted through a ring in pinned

@@ -165,6 +165,19 @@ void plans_and_servers() {
   jobs[123].out = static_cast<char *>(jobs[200].out) + 256;  // in place inside job 200
   jobs[123].in = jobs[123].out;
   EXPECT(fftoc_plan_create(jobs.data(), (int)jobs.size(), &p), 5);
+  // the allocation-free batch: the same checks, before any device work
+  EXPECT(fftoc_batch_fft_noalloc(nullptr, 1, nullptr), 3);
+  EXPECT(fftoc_batch_fft_noalloc(&j, -1, nullptr), 3);
+  EXPECT(fftoc_batch_fft_noalloc(nullptr, 0, nullptr), 0);
+  EXPECT(fftoc_batch_fft_noalloc(jobs.data(), FFTOC_NOALLOC_MAX_JOBS + 1, nullptr), 11);
+  EXPECT(fftoc_batch_fft_noalloc(&bad, 1, nullptr), 5);
+  std::vector<fftoc_job> few(jobs.begin() + 120, jobs.begin() + 120 + FFTOC_NOALLOC_MAX_JOBS);
+  few[3].out = static_cast<char *>(few[20].out) + 256;  // writes into job 20's range
+  few[3].in = few[3].out;
+  EXPECT(fftoc_batch_fft_noalloc(few.data(), (int)few.size(), nullptr), 5);
+  few[3].batch = 0;  // empty jobs are not checked for overlap
+  few[7].n = 48;
+  EXPECT(fftoc_batch_fft_noalloc(few.data(), (int)few.size(), nullptr), 1);
   EXPECT(fftoc_plan_execute(nullptr, nullptr), 13);
   EXPECT(fftoc_plan_execute_graph(nullptr, nullptr), 13);
   EXPECT(fftoc_plan_autotune(nullptr, 2, nullptr), 13);

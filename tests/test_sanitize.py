@@ -12,7 +12,8 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 SAN = os.path.join(ROOT, "tests", "sanitize")
 OUT = os.path.join(ROOT, "fft_onchip_amd", "_build", "sanitize")
 KERNEL_OBJS = [os.path.join(ROOT, "fft_onchip_amd", "_build", f"{k}.o")
-               for k in ("radix_kernels", "mfma_kernels", "mfma_prec_kernels", "batch_kernels")]
+               for k in ("radix_kernels", "mfma_kernels", "mfma_prec_kernels", "batch_kernels",
+                             "device_tables")]
 
 pytestmark = pytest.mark.skipif(not os.path.isdir(SAN), reason="sanitizer sources not present")
 
