@@ -25,6 +25,7 @@
 //    on the host; only additions are done in-kernel (Br + Bi).
 #include "../../include/fftoc_device.hpp"
 #include "fftoc_internal.hpp"
+#include "lane_fft.hpp"
 
 namespace fftoc {
 namespace {
@@ -91,6 +92,127 @@ __device__ __forceinline__ void kgemm(const c2 (&a)[4], const Frag &f, v4d &yr, 
   yr = p1 - p2;
   yi = p3 - p1 - p2;
 }
+
+// ---------------------------------------------------------------------------
+// Two-product DFT16 contraction.  The forward DFT16 matrix is C - iS with
+// C[k][m] = cos(2 pi k m / 16) even and S[k][m] = sin(2 pi k m / 16) odd in m,
+// so its 16 columns hold only 9 distinct cosine columns (m = 0..8) and 7
+// distinct sine columns (m = 1..7).  Packed side by side they make one real
+// 16 x 16 matrix P (columns 0..8 = C[.][0..8], columns 9..15 = S[.][1..7]),
+// and a complex product X * F needs two real MFMA products, Xr P and Xi P,
+// instead of Karatsuba's three:
+//   X[m]      = (Cr + Si) + i (Ci - Sr)      m = 1..7
+//   X[16 - m] = (Cr - Si) + i (Ci + Sr)      m = 1..7
+//   X[0], X[8] = the cosine sums alone,
+// with Cr, Ci (Sr, Si) the cosine (sine) sums of the real and imaginary parts.
+// P is symmetric in role: lane l holds P[4c + g][j] in K-chunk c both as the
+// B operand (data in A: packed frequency = column = lane j) and as the A
+// operand (data in B, P^T: packed frequency = row 4r + g = register r).
+struct HFrag {
+  double p[4];
+};
+
+__device__ __forceinline__ HFrag hdft16_frag(const c2 *__restrict__ tw, int lane) {
+  const int g = lane >> 4, j = lane & 15;
+  const int m = j <= 8 ? j : j - 8;
+  HFrag f;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const c2 w = tw[(((4 * c + g) * m) & 15) << 8];  // W_16^{km} = cos - i sin
+    f.p[c] = j <= 8 ? w.x : -w.y;
+  }
+  return f;
+}
+
+// Frequency of packed column j after the combine: j for the cosine columns,
+// 16 - (j - 8) for the sine columns.
+__device__ __forceinline__ int hphi(int j) { return j <= 8 ? j : 24 - j; }
+
+// Per-lane constants of the data-in-A combine: the lane's result is
+//   re = a u + b u' + c v + d v',  im = a v + b v' - c u - d u'
+// with (u, v) its own sums and (u', v') those of lane l ^ 8 (the other half of
+// its frequency).  Every coefficient is 0 or +-1, so the sums are exact.
+struct HCoef {
+  double a, b, c, d;
+};
+__device__ __forceinline__ HCoef hcoef(int j) {
+  HCoef k;
+  const bool cosine = j <= 8, edge = (j & 7) == 0;  // j = 0, 8: no sine partner
+  k.a = cosine ? 1.0 : 0.0;
+  k.b = cosine ? 0.0 : 1.0;
+  k.c = cosine ? 0.0 : -1.0;
+  k.d = (cosine && !edge) ? 1.0 : 0.0;
+  return k;
+}
+
+// DFT16 over K with the data as the A operand (rows = independent
+// transforms, K chunk c = k = 4c + g): y[r] = X[row 4r + g][hphi(j)].
+__device__ __forceinline__ void hgemm_a(const c2 (&a)[4], const HFrag &f, const HCoef &k,
+                                        c2 (&y)[4]) {
+  v4d u = vzero(), v = vzero();
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    u = mfma16(a[c].x, f.p[c], u);
+    v = mfma16(a[c].y, f.p[c], v);
+  }
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const double up = dpp<kDppRowRor8>(u[r]), vp = dpp<kDppRowRor8>(v[r]);
+    y[r].x = fma(k.d, vp, fma(k.c, v[r], fma(k.b, up, k.a * u[r])));
+    y[r].y = fma(-k.d, up, fma(-k.c, u[r], fma(k.b, vp, k.a * v[r])));
+  }
+}
+
+// DFT16 over K with the data as the B operand (columns = independent
+// transforms, K chunk c = k = 4c + g): o[q] = X[hk2(g, q)][column j].
+__device__ __forceinline__ void hgemm_b(const HFrag &f, const c2 (&b)[4], int g, c2 (&o)[4]) {
+  v4d u = vzero(), v = vzero();
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    u = mfma16(f.p[c], b[c].x, u);
+    v = mfma16(f.p[c], b[c].y, v);
+  }
+  // rows 4r + g: r = 0 cosine g, r = 1 cosine 4 + g, r = 2 sine g (cosine 8
+  // when g = 0), r = 3 sine 4 + g
+  const double e = g ? 1.0 : 0.0;
+  o[0] = make_double2(fma(e, v[2], u[0]), fma(-e, u[2], v[0]));
+  const c2 t = make_double2(u[0] - v[2], v[0] + u[2]);
+  o[1] = g ? t : make_double2(u[2], v[2]);
+  o[2] = make_double2(u[1] + v[3], v[1] - u[3]);
+  o[3] = make_double2(u[1] - v[3], v[1] + u[3]);
+}
+// Frequency of hgemm_b's output o[q] in a lane of group g.
+__device__ __forceinline__ int hk2(int g, int q) {
+  return q == 0 ? g : q == 1 ? (g ? 16 - g : 8) : q == 2 ? 4 + g : 12 - g;
+}
+
+// A DFT16 contraction with the data as the A operand, as a type: load() its
+// per-lane table values, run(a, y) -> y[r] = X[row 4r + g][frequency col].
+struct KaratsubaDft16 {
+  Frag f;
+  int col;
+  __device__ __forceinline__ void load(const c2 *__restrict__ tw, int lane) {
+    f = dft16_frag(tw, lane);
+    col = lane & 15;
+  }
+  __device__ __forceinline__ void run(const c2 (&a)[4], c2 (&y)[4]) const {
+    v4d yr, yi;
+    kgemm(a, f, yr, yi);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) y[r] = make_double2(yr[r], yi[r]);
+  }
+};
+struct TwoProductDft16 {
+  HFrag f;
+  HCoef k;
+  int col;
+  __device__ __forceinline__ void load(const c2 *__restrict__ tw, int lane) {
+    f = hdft16_frag(tw, lane);
+    k = hcoef(lane & 15);
+    col = hphi(lane & 15);
+  }
+  __device__ __forceinline__ void run(const c2 (&a)[4], c2 (&y)[4]) const { hgemm_a(a, f, k, y); }
+};
 
 template <bool INV>
 __device__ __forceinline__ c2 ld_in(const c2 *p) {
@@ -219,32 +341,32 @@ struct TwNLds {
 struct TwNRegs {
   c2 w[4];
   template <int M>
-  __device__ void fetch(const c2 *__restrict__ tw, int g, int j) {
+  __device__ void fetch(const c2 *__restrict__ tw, int g, int k1) {
     constexpr int SPW = 16 / M;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int n2 = (4 * r + g) / SPW;
-      w[r] = n2 != 0 ? tw[(n2 * j) * (kTwN / (16 * M))] : make_double2(1.0, 0.0);
+      w[r] = n2 != 0 ? tw[(n2 * k1) * (kTwN / (16 * M))] : make_double2(1.0, 0.0);
     }
   }
   __device__ c2 get(int r, int) const { return w[r]; }
 };
 
-// Transforms and stores the unit whose first signal is s0.
-template <int M, bool INV, bool SC, class TW = TwNGlobal<16 * M>>
-__device__ __forceinline__ void unit16xm(const c2 (&a)[4], const Frag &f, const TW &tw, c2 *out,
-                                         int64_t s0, int64_t batch, double scale, int g, int j) {
+// Transforms and stores the unit whose first signal is s0: the DFT16 over n1
+// on the matrix cores (contraction D), whose lane column holds frequency
+// k1 = d.col, then the twiddle W_N^{n2 k1} and the DFT_M over n2.
+template <int M, bool INV, bool SC, class TW = TwNGlobal<16 * M>, class D = KaratsubaDft16>
+__device__ __forceinline__ void unit16xm(const c2 (&a)[4], const D &d, const TW &tw, c2 *out,
+                                         int64_t s0, int64_t batch, double scale, int g) {
   constexpr int SPW = 16 / M;
   constexpr int N = 16 * M;
-  v4d yr, yi;
-  kgemm(a, f, yr, yi);
+  const int j = d.col;  // the frequency k1 of this lane's results
   c2 y[4];
+  d.run(a, y);
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
     const int n2 = (4 * r + g) / SPW;
-    c2 v = make_double2(yr[r], yi[r]);
-    if (n2 != 0) v = cmul(v, tw.get(r, n2 * j));
-    y[r] = v;
+    if (n2 != 0) y[r] = cmul(y[r], tw.get(r, n2 * j));
   }
   if constexpr (M == 2) {
     // registers (0, 2) -> signal g, (1, 3) -> signal 4 + g; DFT2 over n2
@@ -293,7 +415,7 @@ __device__ __forceinline__ void unit16xm(const c2 (&a)[4], const Frag &f, const 
   }
 }
 
-template <int M, int G, bool INV, bool SC>
+template <int M, int G, class D, bool INV, bool SC>
 __global__ __launch_bounds__(64 * kWaves) void tensor16xm_kernel(const c2 *in, c2 *out,
                                                                  int64_t batch,
                                                                  const c2 *__restrict__ tw,
@@ -302,23 +424,26 @@ __global__ __launch_bounds__(64 * kWaves) void tensor16xm_kernel(const c2 *in, c
   const int lane = lane_id(), g = lane >> 4, j = lane & 15;
   const int64_t sw = wave_global() * (SPW * G);
   if (sw >= batch) return;
-  const Frag f = dft16_frag(tw, lane);
+  D d;
+  d.load(tw, lane);
   if constexpr (G > 1) {
     // the twiddles (the same for every unit) before the data: loaded after
     // it, their wait (the vector-memory counter retires in order) would hold
     // the first unit until every unit's data had arrived
     TwNRegs w;
-    w.fetch<M>(tw, g, j);
+    w.fetch<M>(tw, g, d.col);
     c2 au[G][4];
+    __builtin_amdgcn_s_setprio(3);  // loads at raised priority (see tensor4096h_kernel)
 #pragma unroll
     for (int u = 0; u < G; ++u) load16xm<M, INV>(in, sw + SPW * u, batch, g, j, au[u]);
+    __builtin_amdgcn_s_setprio(0);
 #pragma unroll
     for (int u = 0; u < G; ++u)
-      unit16xm<M, INV, SC>(au[u], f, w, out, sw + SPW * u, batch, scale, g, j);
+      unit16xm<M, INV, SC>(au[u], d, w, out, sw + SPW * u, batch, scale, g);
   } else {
     c2 a[4];
     load16xm<M, INV>(in, sw, batch, g, j, a);
-    unit16xm<M, INV, SC>(a, f, TwNGlobal<16 * M>{tw}, out, sw, batch, scale, g, j);
+    unit16xm<M, INV, SC>(a, d, TwNGlobal<16 * M>{tw}, out, sw, batch, scale, g);
   }
 }
 
@@ -327,10 +452,14 @@ __global__ __launch_bounds__(64 * kWaves) void tensor16xm_kernel(const c2 *in, c
 // the unit starting at signal s0, run() its transform and stores.  (Measured
 // persistent forms of N = 8, 16, 128 and 256 were no faster than their
 // one-pass kernels; N = 64 below 2^17 signals gains 6 %.)
-template <int M>
+template <int M, class D = KaratsubaDft16>
 struct Unit16xM {
   static constexpr int kSignals = 16 / M;
-  __device__ static Frag frag(const c2 *__restrict__ tw, int lane) { return dft16_frag(tw, lane); }
+  __device__ static D frag(const c2 *__restrict__ tw, int lane) {
+    D d;
+    d.load(tw, lane);
+    return d;
+  }
   template <bool INV>
   __device__ static void load(const c2 *in, int64_t s0, int64_t batch, int g, int j, c2 (&a)[4]) {
     load16xm<M, INV>(in, s0, batch, g, j, a);
@@ -342,9 +471,9 @@ struct Unit16xM {
     for (int k = threadIdx.x; k < kTwElems; k += blockDim.x) t[k] = tw[k * (kTwN / (16 * M))];
   }
   template <bool INV, bool SC>
-  __device__ static void run(const c2 (&a)[4], const Frag &f, const c2 *twl, c2 *out, int64_t s0,
+  __device__ static void run(const c2 (&a)[4], const D &f, const c2 *twl, c2 *out, int64_t s0,
                              int64_t batch, double scale, int g, int j) {
-    unit16xm<M, INV, SC>(a, f, TwNLds{twl}, out, s0, batch, scale, g, j);
+    unit16xm<M, INV, SC>(a, f, TwNLds{twl}, out, s0, batch, scale, g);
   }
 };
 
@@ -358,25 +487,37 @@ template <class U, int G, bool INV, bool SC>
 __global__ __launch_bounds__(64 * kWaves) void tensor_persistent_kernel(
     const c2 *in, c2 *out, int64_t batch, const c2 *__restrict__ tw, double scale) {
   __shared__ c2 twl[U::kTwElems];
-  U::fill_tw(twl, tw);
-  __syncthreads();  // before any wave leaves
   constexpr int SPG = U::kSignals * G;  // signals per group
   const int lane = lane_id(), g = lane >> 4, j = lane & 15;
   const int64_t groups = (batch + SPG - 1) / SPG;
   const int64_t stride = (int64_t)gridDim.x * kWaves;
   int64_t q = wave_global();
-  if (q >= groups) return;
-  const Frag f = U::frag(tw, lane);
+  const bool active = q < groups;  // uniform over the wave
+  // The first group's loads go out before the twiddle table is copied into
+  // LDS: copied first, the table's round trip (from HBM at the kernel's
+  // start) and the barrier came before any data was requested.
+  // Loads are issued at raised wave priority (see tensor4096h_kernel).
   c2 cur[G][4];
+  if (active) {
+    __builtin_amdgcn_s_setprio(3);
 #pragma unroll
-  for (int u = 0; u < G; ++u) U::template load<INV>(in, q * SPG + U::kSignals * u, batch, g, j, cur[u]);
+    for (int u = 0; u < G; ++u)
+      U::template load<INV>(in, q * SPG + U::kSignals * u, batch, g, j, cur[u]);
+    __builtin_amdgcn_s_setprio(0);
+  }
+  const auto f = U::frag(tw, lane);
+  U::fill_tw(twl, tw);
+  __syncthreads();  // before any wave leaves
+  if (!active) return;
   for (;;) {
     const int64_t qn = q + stride;
     c2 nxt[G][4];
     if (qn < groups) {
+      __builtin_amdgcn_s_setprio(3);
 #pragma unroll
       for (int u = 0; u < G; ++u)
         U::template load<INV>(in, qn * SPG + U::kSignals * u, batch, g, j, nxt[u]);
+      __builtin_amdgcn_s_setprio(0);
     }
 #pragma unroll
     for (int u = 0; u < G; ++u)
@@ -450,13 +591,66 @@ __device__ __forceinline__ void core256(c2 (&a)[4], const Frag &f, const c2 *__r
   core256(a, f, w, z);
 }
 
+// 256-point core with the two-product contraction: input a[c] = x[64c + l]
+// (n = 16 n1 + n2, n1 = 4c + g, n2 = j); stage 1 with the data in A gives
+// Y[n2 = 4r + g][k1 = hphi(j)] in register r, which after the twiddle
+// W_256^{n2 k1} is already the B operand of stage 2 (K = n2 = 4r + g, column
+// j); output z[q] = X[hphi(j) + 16 hk2(g, q)].  16 MFMAs against 24.
+__device__ __forceinline__ void core256h_twiddles(const c2 *__restrict__ tw, int lane,
+                                                  c2 (&w)[4]) {
+  const int g = lane >> 4, k1 = hphi(lane & 15);
+#pragma unroll
+  for (int r = 0; r < 4; ++r) w[r] = tw[((4 * r + g) * k1) << 4];
+}
+
+__device__ __forceinline__ void core256h(const c2 (&a)[4], const HFrag &f, const HCoef &k,
+                                         const c2 (&w)[4], int g, c2 (&z)[4]) {
+  c2 y[4];
+  hgemm_a(a, f, k, y);
+#pragma unroll
+  for (int r = 0; r < 4; ++r) y[r] = cmul(y[r], w[r]);
+  hgemm_b(f, y, g, z);
+}
+
+// A 256-point core as a type: its per-lane table values (load, once per
+// kernel), the transform of one row (run: a[c] = x[64c + l] in, z[q] out) and
+// the row position k1 + 16 k2 of z[q] (out_index).
+struct KaratsubaCore {  // three real products per DFT16 (24 MFMAs per core)
+  Frag f;
+  c2 w[4];
+  int g, j;
+  __device__ __forceinline__ void load(const c2 *__restrict__ tw, int lane) {
+    f = dft16_frag(tw, lane);
+    core256_twiddles(tw, lane, w);
+    g = lane >> 4;
+    j = lane & 15;
+  }
+  __device__ __forceinline__ void run(c2 (&a)[4], c2 (&z)[4]) const { core256(a, f, w, z); }
+  __device__ __forceinline__ int out_index(int q) const { return 4 * q + g + 16 * j; }
+};
+struct TwoProductCore {  // two real products per DFT16 (16 MFMAs per core)
+  HFrag f;
+  HCoef k;
+  c2 w[4];
+  int g, j;
+  __device__ __forceinline__ void load(const c2 *__restrict__ tw, int lane) {
+    f = hdft16_frag(tw, lane);
+    g = lane >> 4;
+    j = lane & 15;
+    k = hcoef(j);
+    core256h_twiddles(tw, lane, w);
+  }
+  __device__ __forceinline__ void run(c2 (&a)[4], c2 (&z)[4]) const { core256h(a, f, k, w, g, z); }
+  __device__ __forceinline__ int out_index(int q) const { return hphi(j) + 16 * hk2(g, q); }
+};
+
 // LDS element swizzle for the output transposes: XOR the low 4 bits of the
 // element index with bits 4..7 (16 x 16-byte slots per 256 B bank window).
 __device__ __forceinline__ int swz16(int e) { return e ^ ((e >> 4) & 15); }
 
 // N = 256: one signal per unit, two MFMA stages, output re-ordered through
 // 4 KiB of LDS per wave so every store instruction writes 1 KiB contiguously.
-template <int G, bool INV, bool SC>
+template <int G, class CORE, bool INV, bool SC>
 __global__ __launch_bounds__(64 * kWaves) void tensor256_kernel(const c2 *in, c2 *out,
                                                                 int64_t batch,
                                                                 const c2 *__restrict__ tw,
@@ -466,14 +660,13 @@ __global__ __launch_bounds__(64 * kWaves) void tensor256_kernel(const c2 *in, c2
   const int64_t sw = wave_global() * G;
   if (sw >= batch) return;
   c2 *buf = lds + (threadIdx.x >> 6) * 256;
-  const Frag f = dft16_frag(tw, lane);
   // with several units the twiddles (the same for every unit) come before
   // the data: loaded after it, their wait (the vector-memory counter retires
   // in order) would hold the first unit until every unit's data had arrived
   // (G = 2: 115 -> 103 us, G = 4: 113 -> 108 us at 65536 signals; G = 1
   // keeps them after its data, 104 against 107 us)
-  c2 w[4];
-  if constexpr (G > 1) core256_twiddles(tw, lane, w);
+  CORE core;
+  if constexpr (G > 1) core.load(tw, lane);
   c2 a[G][4];
 #pragma unroll
   for (int u = 0; u < G; ++u) {
@@ -481,19 +674,23 @@ __global__ __launch_bounds__(64 * kWaves) void tensor256_kernel(const c2 *in, c2
 #pragma unroll
     for (int c = 0; c < 4; ++c) a[u][c] = ld_in<INV>(in + s * 256 + 64 * c + lane);
   }
-  if constexpr (G == 1) core256_twiddles(tw, lane, w);
+  if constexpr (G == 1) core.load(tw, lane);
 #pragma unroll
   for (int u = 0; u < G; ++u) {
     c2 z[4];
-    core256(a[u], f, w, z);
+    core.run(a[u], z);
     if (u > 0) wave_lds_sync();  // the previous unit's LDS reads are done
 #pragma unroll
-    for (int r = 0; r < 4; ++r) buf[swz16(4 * r + g + 16 * j)] = z[r];
+    for (int r = 0; r < 4; ++r) buf[swz16(core.out_index(r))] = z[r];
     wave_lds_sync();
     if (sw + u < batch) {
       c2 *y = out + (sw + u) * 256;
+      c2 o[4];  // every read before the first store (see tensor4096h_kernel)
 #pragma unroll
-      for (int c = 0; c < 4; ++c) st_out<INV, SC>(y + 64 * c + lane, buf[swz16(64 * c + lane)], scale);
+      for (int c = 0; c < 4; ++c) o[c] = buf[swz16(64 * c + lane)];
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int c = 0; c < 4; ++c) st_out<INV, SC>(y + 64 * c + lane, o[c], scale);
     }
   }
 }
@@ -575,11 +772,15 @@ __device__ __forceinline__ void unit512(const c2 (&a)[2][4], const Frag &f, cons
   wave_lds_sync();
   if (s < batch) {
     c2 *y = out + s * 512;
+    c2 o[8];  // every read before the first store (see tensor4096h_kernel)
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
       const int e = 64 * c + lane;
-      st_out<INV, SC>(y + e, buf[e ^ ((e >> 5) & 15)], scale);
+      o[c] = buf[e ^ ((e >> 5) & 15)];
     }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int c = 0; c < 8; ++c) st_out<INV, SC>(y + 64 * c + lane, o[c], scale);
   }
 }
 
@@ -615,17 +816,19 @@ __global__ __launch_bounds__(64 * kWaves) void tensor512p_kernel(const c2 *in, c
                                                                  double scale) {
   __shared__ c2 lds[kWaves * 512];
   __shared__ c2 twl[kTw512LdsElems];
-  fill_tw512_lds(twl, tw);
-  __syncthreads();  // before any wave leaves
   const int lane = lane_id(), g = lane >> 4, j = lane & 15;
   const int64_t stride = (int64_t)gridDim.x * kWaves;
   int64_t s = wave_global();
-  if (s >= batch) return;
-  c2 *buf = lds + (threadIdx.x >> 6) * 512;
-  const Frag f = dft16_frag(tw, lane);
-  const Tw512Lds twt{twl};
+  const bool active = s < batch;  // uniform over the wave
+  // first signal's loads before the twiddle copy (see tensor_persistent_kernel)
   c2 cur[2][4];
-  load512<INV>(in, s, g, j, cur);
+  if (active) load512<INV>(in, s, g, j, cur);
+  const Frag f = dft16_frag(tw, lane);
+  fill_tw512_lds(twl, tw);
+  __syncthreads();  // before any wave leaves
+  if (!active) return;
+  c2 *buf = lds + (threadIdx.x >> 6) * 512;
+  const Tw512Lds twt{twl};
   for (bool reuse = false;; reuse = true) {
     const int64_t sn = s + stride;
     c2 nxt[2][4];
@@ -706,11 +909,72 @@ __global__ __launch_bounds__(64 * k4096Waves) void tensor4096_kernel(const c2 *i
   }
   __syncthreads();
   c2 *y = out + s * 4096;
+  c2 o[8];  // every read before the first store (see tensor4096h_kernel)
 #pragma unroll
-  for (int c = 0; c < 8; ++c) {
-    const int e = 512 * c + threadIdx.x;
-    st_out<INV, SC>(y + e, lds[swz_x(e)], scale);
+  for (int c = 0; c < 8; ++c) o[c] = lds[swz_x(512 * c + threadIdx.x)];
+  __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+  for (int c = 0; c < 8; ++c) st_out<INV, SC>(y + 512 * c + threadIdx.x, o[c], scale);
+}
+
+// N = 4096 = 16 x 16 x 16 with all three DFT16 stages on the matrix cores and
+// the two-product contraction: 384 MFMAs per signal (Karatsuba's three stages
+// take 576).  Stage A (each wave: 2 groups of 16 columns m) has the data as
+// the B operand (K = n1, column = m), so its combine stays in the lane: o[q]
+// = Z'[k1 = hk2(g, q)][m = m0 + j], twiddled by W_4096^{m k1} into LDS.
+// Stage B is the two-product 256-point core of each row k1.  The same LDS
+// layouts and output order as tensor4096_kernel.
+template <bool INV, bool SC>
+__global__ __launch_bounds__(64 * k4096Waves) void tensor4096t_kernel(const c2 *in, c2 *out,
+                                                                      int64_t batch,
+                                                                      const c2 *__restrict__ tw,
+                                                                      double scale) {
+  __shared__ c2 lds[4096];
+  const int lane = lane_id(), g = lane >> 4, j = lane & 15;
+  const int w = threadIdx.x >> 6;
+  const int64_t s = blockIdx.x;
+  if (s >= batch) return;  // uniform over the workgroup
+  const c2 *x = in + s * 4096;
+  TwoProductCore core;
+  core.load(tw, lane);
+  // ---- stage A
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    const int m = 16 * (2 * w + q) + j;
+    c2 wa[4], a[4], o[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) wa[r] = tw[m * hk2(g, r)];
+    // K chunk c, slot g: n1 = 4c + g; column j: m
+#pragma unroll
+    for (int c = 0; c < 4; ++c) a[c] = ld_in<INV>(x + 256 * (4 * c + g) + m);
+    hgemm_b(core.f, a, g, o);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) lds[swz_z(hk2(g, r), m)] = cmul(o[r], wa[r]);
   }
+  __syncthreads();
+  // ---- stage B: rows k1 = 2w + q
+  c2 z[2][4];
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    const int k1 = 2 * w + q;
+    c2 b[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) b[c] = lds[swz_z(k1, 64 * c + lane)];
+    core.run(b, z[q]);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < 2; ++q)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) lds[swz_x(2 * w + q + 16 * core.out_index(r))] = z[q][r];
+  __syncthreads();
+  c2 *y = out + s * 4096;
+  c2 ov[8];  // every read before the first store (see tensor4096h_kernel)
+#pragma unroll
+  for (int c = 0; c < 8; ++c) ov[c] = lds[swz_x(512 * c + threadIdx.x)];
+  __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+  for (int c = 0; c < 8; ++c) st_out<INV, SC>(y + 512 * c + threadIdx.x, ov[c], scale);
 }
 
 // N = 4096 with stage A on the vector ALUs (threads 0..255; thread t owns
@@ -725,7 +989,7 @@ __device__ __forceinline__ int swz_row(int k1, int k2) {
   return k1 * 256 + (k2 ^ (((k2 >> 4) ^ k1) & 15));
 }
 
-template <int WAVES, bool INV, bool SC>
+template <int WAVES, class CORE, bool INV, bool SC>
 __global__ __launch_bounds__(64 * WAVES) void tensor4096h_kernel(const c2 *in, c2 *out,
                                                                  int64_t batch,
                                                                  const c2 *__restrict__ tw,
@@ -738,36 +1002,61 @@ __global__ __launch_bounds__(64 * WAVES) void tensor4096h_kernel(const c2 *in, c
   const int64_t s = blockIdx.x;
   if (s >= batch) return;  // uniform over the workgroup
   const c2 *x = in + s * 4096;
+  // The waves of a starting workgroup issue their loads at raised priority:
+  // the other resident workgroup is usually in its matrix stage, and its
+  // instructions would otherwise delay these loads, whose HBM round trip is
+  // half of a workgroup's time (+2-3 % at N = 4096 x 16384).
+  __builtin_amdgcn_s_setprio(3);
+  // Every table value the thread needs is loaded before the signal: the DFT16
+  // fragment and the core's twiddles (the same for every row of stage B) and
+  // the stage-A twiddle bases of its column.  Loaded after the signal, each
+  // wait for them would also wait for it (the vector-memory counter retires in
+  // order); loaded after the barrier (as before), they held stage B's first
+  // matrix product for a cache round trip in every wave.
+  CORE core;
+  core.load(tw, lane);
+  TwBase<16, c2> ta;
+  if (t < 256) ta.fetch(tw, t);
   if (t < 256) {
     // ---- stage A (VALU): column m = t
     c2 v[16];
 #pragma unroll
     for (int n1 = 0; n1 < 16; ++n1) v[n1] = ld_in<INV>(x + 256 * n1 + t);
+    __builtin_amdgcn_s_setprio(0);
     dft<16>(v);
-    apply_twiddles<16>(v, tw, t);  // v[k1] *= W_4096^{t k1}
+    apply_twiddle_base<16>(v, ta);  // v[k1] *= W_4096^{t k1}
 #pragma unroll
     for (int k1 = 0; k1 < 16; ++k1) lds[swz_z(k1, t)] = v[k1];
   }
+  __builtin_amdgcn_s_setprio(0);  // waves without a column (WAVES > 4)
   __syncthreads();
   // ---- stage B (MFMA): rows k1 = kRowsPerWave w + q, results back into the same row
-  const Frag f = dft16_frag(tw, lane);
 #pragma unroll
   for (int q = 0; q < kRowsPerWave; ++q) {
     const int k1 = kRowsPerWave * w + q;
     c2 a[4], z[4];
 #pragma unroll
     for (int c = 0; c < 4; ++c) a[c] = lds[swz_z(k1, 64 * c + lane)];
-    core256(a, f, tw, lane, z);
+    core.run(a, z);
 #pragma unroll
-    for (int r = 0; r < 4; ++r) lds[swz_row(k1, 4 * r + g + 16 * j)] = z[r];
+    for (int r = 0; r < 4; ++r) lds[swz_row(k1, core.out_index(r))] = z[r];
   }
   __syncthreads();
+  // All of the thread's outputs are read from LDS before the first store: the
+  // compiler otherwise pairs each read with its store and waits for that read
+  // alone, and the sixteen LDS round trips run one after another (about a
+  // microsecond per signal).
+  constexpr int kOut = 4096 / (64 * WAVES);
+  c2 o[kOut];
+#pragma unroll
+  for (int c = 0; c < kOut; ++c) {
+    const int e = 64 * WAVES * c + t;
+    o[c] = lds[swz_row(e & 15, e >> 4)];
+  }
+  __builtin_amdgcn_sched_barrier(0);
   c2 *y = out + s * 4096;
 #pragma unroll
-  for (int c = 0; c < 4096 / (64 * WAVES); ++c) {
-    const int e = 64 * WAVES * c + t;
-    st_out<INV, SC>(y + e, lds[swz_row(e & 15, e >> 4)], scale);
-  }
+  for (int c = 0; c < kOut; ++c) st_out<INV, SC>(y + 64 * WAVES * c + t, o[c], scale);
 }
 
 int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
@@ -775,12 +1064,12 @@ int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 // Persistent launch of a one-signal-per-wave kernel: one wave per signal up
 // to the resident count.
 
-#define FFTOC_PERSISTENT_LAUNCHER(FN, KERNEL)                                                  \
+#define FFTOC_PERSISTENT_LAUNCHER(FN, KERNEL, UNITS_PER_WG)                                   \
   int FN(const LaunchArgs &a) {                                                               \
     if (a.precision != FFTOC_PREC_FP64) return FFTOC_ERROR_UNSUPPORTED;                       \
     if (a.batch <= 0) return FFTOC_SUCCESS;                                                   \
     const int64_t resident = resident_workgroups<KERNEL<false, false>, 64 * kWaves>();        \
-    const int64_t need = ceil_div(a.batch, kWaves);                                           \
+    const int64_t need = ceil_div(a.batch, UNITS_PER_WG);                                     \
     const dim3 gd((unsigned)(need < resident ? need : resident)), bd(64 * kWaves);            \
     const c2 *in = static_cast<const c2 *>(a.in);                                             \
     c2 *out = static_cast<c2 *>(a.out);                                                       \
@@ -800,7 +1089,7 @@ int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
                          a.scale);                                                            \
     return hipGetLastError() == hipSuccess ? FFTOC_SUCCESS : FFTOC_ERROR_LAUNCH;              \
   }
-FFTOC_PERSISTENT_LAUNCHER(launch_t512p, tensor512p_kernel)
+FFTOC_PERSISTENT_LAUNCHER(launch_t512p, tensor512p_kernel, kWaves)
 #undef FFTOC_PERSISTENT_LAUNCHER
 
 #define FFTOC_TENSOR_LAUNCHER(FN, KERNEL, UNITS_PER_WG, BLOCK, ...)                              \
@@ -834,22 +1123,27 @@ FFTOC_TENSOR_LAUNCHER(launch_t8g4, tensor8_kernel, 128 * kWaves, 64 * kWaves, 4)
 FFTOC_TENSOR_LAUNCHER(launch_t16, tensor16_kernel, 16 * kWaves, 64 * kWaves, 1)
 FFTOC_TENSOR_LAUNCHER(launch_t16g2, tensor16_kernel, 32 * kWaves, 64 * kWaves, 2)
 FFTOC_TENSOR_LAUNCHER(launch_t16g4, tensor16_kernel, 64 * kWaves, 64 * kWaves, 4)
-FFTOC_TENSOR_LAUNCHER(launch_t32, tensor16xm_kernel, 8 * kWaves, 64 * kWaves, 2, 1)
-FFTOC_TENSOR_LAUNCHER(launch_t32g2, tensor16xm_kernel, 16 * kWaves, 64 * kWaves, 2, 2)
-FFTOC_TENSOR_LAUNCHER(launch_t32g4, tensor16xm_kernel, 32 * kWaves, 64 * kWaves, 2, 4)
-FFTOC_TENSOR_LAUNCHER(launch_t64, tensor16xm_kernel, 4 * kWaves, 64 * kWaves, 4, 1)
-FFTOC_TENSOR_LAUNCHER(launch_t64g2, tensor16xm_kernel, 8 * kWaves, 64 * kWaves, 4, 2)
-FFTOC_TENSOR_LAUNCHER(launch_t64g4, tensor16xm_kernel, 16 * kWaves, 64 * kWaves, 4, 4)
-FFTOC_TENSOR_LAUNCHER(launch_t128, tensor16xm_kernel, 2 * kWaves, 64 * kWaves, 8, 1)
-FFTOC_TENSOR_LAUNCHER(launch_t128g2, tensor16xm_kernel, 4 * kWaves, 64 * kWaves, 8, 2)
-FFTOC_TENSOR_LAUNCHER(launch_t128g4, tensor16xm_kernel, 8 * kWaves, 64 * kWaves, 8, 4)
-FFTOC_TENSOR_LAUNCHER(launch_t256, tensor256_kernel, kWaves, 64 * kWaves, 1)
-FFTOC_TENSOR_LAUNCHER(launch_t256g2, tensor256_kernel, 2 * kWaves, 64 * kWaves, 2)
-FFTOC_TENSOR_LAUNCHER(launch_t256g4, tensor256_kernel, 4 * kWaves, 64 * kWaves, 4)
+FFTOC_TENSOR_LAUNCHER(launch_t32, tensor16xm_kernel, 8 * kWaves, 64 * kWaves, 2, 1, KaratsubaDft16)
+FFTOC_TENSOR_LAUNCHER(launch_t32g2, tensor16xm_kernel, 16 * kWaves, 64 * kWaves, 2, 2, KaratsubaDft16)
+FFTOC_TENSOR_LAUNCHER(launch_t32g4, tensor16xm_kernel, 32 * kWaves, 64 * kWaves, 2, 4, KaratsubaDft16)
+FFTOC_TENSOR_LAUNCHER(launch_t64, tensor16xm_kernel, 4 * kWaves, 64 * kWaves, 4, 1, KaratsubaDft16)
+FFTOC_TENSOR_LAUNCHER(launch_t64g2, tensor16xm_kernel, 8 * kWaves, 64 * kWaves, 4, 2, KaratsubaDft16)
+FFTOC_TENSOR_LAUNCHER(launch_t64g2h, tensor16xm_kernel, 8 * kWaves, 64 * kWaves, 4, 2, TwoProductDft16)
+FFTOC_TENSOR_LAUNCHER(launch_t64g4, tensor16xm_kernel, 16 * kWaves, 64 * kWaves, 4, 4, KaratsubaDft16)
+FFTOC_TENSOR_LAUNCHER(launch_t128, tensor16xm_kernel, 2 * kWaves, 64 * kWaves, 8, 1, KaratsubaDft16)
+FFTOC_TENSOR_LAUNCHER(launch_t128g2, tensor16xm_kernel, 4 * kWaves, 64 * kWaves, 8, 2, KaratsubaDft16)
+FFTOC_TENSOR_LAUNCHER(launch_t128g2h, tensor16xm_kernel, 4 * kWaves, 64 * kWaves, 8, 2, TwoProductDft16)
+FFTOC_TENSOR_LAUNCHER(launch_t128g4, tensor16xm_kernel, 8 * kWaves, 64 * kWaves, 8, 4, KaratsubaDft16)
+FFTOC_TENSOR_LAUNCHER(launch_t256, tensor256_kernel, kWaves, 64 * kWaves, 1, KaratsubaCore)
+FFTOC_TENSOR_LAUNCHER(launch_t256g2, tensor256_kernel, 2 * kWaves, 64 * kWaves, 2, KaratsubaCore)
+FFTOC_TENSOR_LAUNCHER(launch_t256g2h, tensor256_kernel, 2 * kWaves, 64 * kWaves, 2, TwoProductCore)
+FFTOC_TENSOR_LAUNCHER(launch_t256g4, tensor256_kernel, 4 * kWaves, 64 * kWaves, 4, KaratsubaCore)
 FFTOC_TENSOR_LAUNCHER(launch_t512, tensor512_kernel, kWaves, 64 * kWaves, 1)
 FFTOC_TENSOR_LAUNCHER(launch_t512g2, tensor512_kernel, 2 * kWaves, 64 * kWaves, 2)
 FFTOC_TENSOR_LAUNCHER(launch_t4096, tensor4096_kernel, 1, 64 * k4096Waves)
-FFTOC_TENSOR_LAUNCHER(launch_t4096h, tensor4096h_kernel, 1, 256, 4)
+FFTOC_TENSOR_LAUNCHER(launch_t4096t, tensor4096t_kernel, 1, 64 * k4096Waves)
+FFTOC_TENSOR_LAUNCHER(launch_t4096h, tensor4096h_kernel, 1, 256, 4, KaratsubaCore)
+FFTOC_TENSOR_LAUNCHER(launch_t4096hh, tensor4096h_kernel, 1, 256, 4, TwoProductCore)
 #undef FFTOC_TENSOR_LAUNCHER
 
 constexpr unsigned TF = FAM_TENSOR | FAM_ANY;
@@ -865,12 +1159,16 @@ const Variant kT32[] = {{"tensor:16x2/16 per wave", TF, &launch_t32g2, 64},
                         {"tensor:16x2/32 per wave", TF, &launch_t32g4, 128}};
 const Variant kT64[] = {{"tensor:16x4/8 per wave", TF, &launch_t64g2, 32},
                         {"tensor:16x4/8 per wave/persistent", TF, &launch_tensor_persistent<Unit16xM<4>, 2>, 32},
+                        {"tensor:16x4/8 per wave/persistent/2-product", TF, &launch_tensor_persistent<Unit16xM<4, TwoProductDft16>, 2>, 32},
+                        {"tensor:16x4/8 per wave/2-product", TF, &launch_t64g2h, 32},
                         {"tensor:16x4/4 per wave", TF, &launch_t64, 16},
                         {"tensor:16x4/16 per wave", TF, &launch_t64g4, 64}};
 const Variant kT128[] = {{"tensor:16x8/4 per wave", TF, &launch_t128g2, 16},
+                         {"tensor:16x8/4 per wave/2-product", TF, &launch_t128g2h, 16},
                          {"tensor:16x8/2 per wave", TF, &launch_t128, 8},
                          {"tensor:16x8/8 per wave", TF, &launch_t128g4, 32}};
 const Variant kT256[] = {{"tensor:16x16/2 per wave", TF, &launch_t256g2, 8},
+                         {"tensor:16x16/2 per wave/2-product", TF, &launch_t256g2h, 8},
                          {"tensor:16x16/4 per wave", TF, &launch_t256g4, 16},
                          {"tensor:16x16/1 per wave", TF, &launch_t256, 4}};
 const Variant kT512[] = {{"tensor:16x2x16/1 per wave/persistent", TF, &launch_t512p, 4},
@@ -879,7 +1177,9 @@ const Variant kT512[] = {{"tensor:16x2x16/1 per wave/persistent", TF, &launch_t5
 // The hybrid form leads: 0.64 of HBM against 0.51 for three MFMA stages
 // (bound by the matrix cores plus the vector work between them).
 const Variant kT4096[] = {{"tensor:16x16x16/VALU stage A, 4 waves", TF, &launch_t4096h, 1},
-                          {"tensor:16x16x16/8 waves", TF, &launch_t4096, 1}};
+                          {"tensor:16x16x16/VALU stage A, 4 waves/2-product", TF, &launch_t4096hh, 1},
+                          {"tensor:16x16x16/8 waves", TF, &launch_t4096, 1},
+                          {"tensor:16x16x16/8 waves/2-product", TF, &launch_t4096t, 1}};
 
 template <size_t K>
 int set_out(const Variant (&v)[K], const Variant **out) {
