@@ -1,9 +1,13 @@
 // Example of fusing the transform into a caller's kernel with the in-LDS
 // functors of include/fftoc_lds.hpp (the reference's functor-on-shared-memory
-// form, includes/testing.cuh:39-43): each workgroup loads S signals into LDS
+// form, includes/testing.cuh:39-43): each workgroup loads its signals into LDS
 // applying an optional Hann window (pre-processing), transforms them in place
-// with fftoc::LdsFft and stores them.  Built as libfftoc_fused_example.so; the
-// GPU tests check it against the oracle.
+// with fftoc::LdsFft (Stockham radix passes) or fftoc::LdsTensorFft (matrix
+// cores) and stores them.  A second entry point is the reference's own timing
+// harness (includes/testing.cuh:16-57, 115-124): load once, run the functor R
+// times on the LDS-resident signals, store once, so that the difference of two
+// repeat counts is the on-chip transform time with HBM excluded.  Built as
+// libfftoc_fused_example.so; the GPU tests check it against the oracle.
 #include <hip/hip_runtime.h>
 
 #include "../../include/fftoc.h"
@@ -59,6 +63,117 @@ int launch(const void *in, void *out, int64_t batch, const void *tw, int inverse
   return hipGetLastError() == hipSuccess ? FFTOC_SUCCESS : FFTOC_ERROR_LAUNCH;
 }
 
+// Tensor functor per length: units of one wave (N <= 256) with several
+// signals each, or one four-wave unit (N = 4096); 256 threads per workgroup.
+template <int N, bool TP> struct TensorCfg;
+template <bool TP> struct TensorCfg<16, TP> { using F = fftoc::LdsTensorFft<16, 4, 16, TP>; };
+template <bool TP> struct TensorCfg<64, TP> { using F = fftoc::LdsTensorFft<64, 4, 8, TP>; };
+template <bool TP> struct TensorCfg<256, TP> { using F = fftoc::LdsTensorFft<256, 4, 2, TP>; };
+template <bool TP> struct TensorCfg<4096, TP> { using F = fftoc::LdsTensorFft<4096, 1, 1, TP>; };
+
+// Threads and signals per workgroup of either functor family.
+template <class F> struct Shape {
+  static constexpr int threads = F::max_threads_per_block;
+};
+template <int N, int S, class CT> struct Shape<LdsFft<N, S, CT>> {
+  static constexpr int threads = LdsFft<N, S, CT>::kThreads;
+};
+
+// Loads the workgroup's signals (s0 on) into LDS, optionally windowed.
+template <int N, int ELEMS, int THREADS>
+__device__ __forceinline__ void load_block(const double2 *in, double2 *buf, int64_t s0,
+                                           int64_t batch, int window) {
+  for (int e = threadIdx.x; e < ELEMS; e += THREADS) {
+    const int64_t s = s0 + e / N;
+    double2 v = s < batch ? in[s0 * N + e] : make_double2(0.0, 0.0);
+    if (window) {
+      const double w = 0.5 - 0.5 * cospi(2.0 * (double)(e % N) / (double)N);
+      v.x *= w;
+      v.y *= w;
+    }
+    buf[e] = v;
+  }
+}
+template <int N, int ELEMS, int THREADS>
+__device__ __forceinline__ void store_block(double2 *out, const double2 *buf, int64_t s0,
+                                            int64_t batch) {
+  for (int e = threadIdx.x; e < ELEMS; e += THREADS) {
+    const int64_t s = s0 + e / N;
+    if (s < batch) out[s0 * N + e] = buf[e];
+  }
+}
+
+template <int N, class F>
+__global__ __launch_bounds__(Shape<F>::threads) void windowed_tensor_kernel(
+    const double2 *in, double2 *out, int64_t batch, const double2 *__restrict__ tw, int inverse,
+    double scale, int window) {
+  constexpr int S = F::kElems / N;
+  __shared__ double2 buf[F::kElems];
+  const int64_t s0 = (int64_t)blockIdx.x * S;
+  load_block<N, F::kElems, Shape<F>::threads>(in, buf, s0, batch, window);
+  __syncthreads();
+  F f(buf, tw);
+  f(inverse != 0, scale);
+  __syncthreads();
+  store_block<N, F::kElems, Shape<F>::threads>(out, buf, s0, batch);
+}
+
+template <int N, bool TP>
+int launch_tensor(const void *in, void *out, int64_t batch, const void *tw, int inverse,
+                  double scale, int window, hipStream_t stream) {
+  using F = typename TensorCfg<N, TP>::F;
+  constexpr int S = F::kElems / N;
+  const int64_t grid = (batch + S - 1) / S;
+  if (grid > 0x7fffffffLL) return FFTOC_ERROR_INVALID_BATCH;
+  hipLaunchKernelGGL((windowed_tensor_kernel<N, F>), dim3((unsigned)grid), dim3(Shape<F>::threads),
+                     0, stream, static_cast<const double2 *>(in), static_cast<double2 *>(out),
+                     batch, static_cast<const double2 *>(tw), inverse, scale, window);
+  return hipGetLastError() == hipSuccess ? FFTOC_SUCCESS : FFTOC_ERROR_LAUNCH;
+}
+
+// The reference's timing harness with either functor family: one load, R
+// forward transforms of the LDS-resident signals (each output times `scale`,
+// 1/sqrt(N) keeping the values' magnitude), one store, in place.
+template <int N, class F>
+__global__ __launch_bounds__(Shape<F>::threads) void onchip_kernel(double2 *data, int64_t batch,
+                                                                   const double2 *__restrict__ tw,
+                                                                   int repeats, double scale) {
+  constexpr int S = F::kElems / N;
+  __shared__ double2 buf[F::kElems];
+  const int64_t s0 = (int64_t)blockIdx.x * S;
+  load_block<N, F::kElems, Shape<F>::threads>(data, buf, s0, batch, 0);
+  __syncthreads();
+  F f(buf, tw);
+  for (int r = 0; r < repeats; ++r) f(false, scale);
+  __syncthreads();
+  store_block<N, F::kElems, Shape<F>::threads>(data, buf, s0, batch);
+}
+
+template <int N, class F>
+int launch_onchip(void *data, int64_t batch, const void *tw, int repeats, double scale,
+                  hipStream_t stream) {
+  constexpr int S = F::kElems / N;
+  const int64_t grid = (batch + S - 1) / S;
+  if (grid > 0x7fffffffLL) return FFTOC_ERROR_INVALID_BATCH;
+  hipLaunchKernelGGL((onchip_kernel<N, F>), dim3((unsigned)grid), dim3(Shape<F>::threads), 0,
+                     stream, static_cast<double2 *>(data), batch,
+                     static_cast<const double2 *>(tw), repeats, scale);
+  return hipGetLastError() == hipSuccess ? FFTOC_SUCCESS : FFTOC_ERROR_LAUNCH;
+}
+
+template <int N>
+int onchip(int family, void *data, int64_t batch, const void *tw, int repeats, double scale,
+           hipStream_t s) {
+  if (family == 0) return launch_onchip<N, Fft<N>>(data, batch, tw, repeats, scale, s);
+  if constexpr (N == 16 || N == 64 || N == 256 || N == 4096) {
+    if (family == 1)
+      return launch_onchip<N, typename TensorCfg<N, false>::F>(data, batch, tw, repeats, scale, s);
+    if (family == 2)
+      return launch_onchip<N, typename TensorCfg<N, true>::F>(data, batch, tw, repeats, scale, s);
+  }
+  return FFTOC_ERROR_UNSUPPORTED;
+}
+
 }  // namespace
 
 extern "C" {
@@ -84,6 +199,56 @@ int fftoc_example_windowed_fft(const void *in, void *out, int64_t batch, int n, 
     case 1024: return launch<1024>(in, out, batch, tw, inverse, scale, window, s);
     case 2048: return launch<2048>(in, out, batch, tw, inverse, scale, window, s);
     case 4096: return launch<4096>(in, out, batch, tw, inverse, scale, window, s);
+    default: return FFTOC_ERROR_INVALID_SIZE;
+  }
+}
+
+// The matrix-core functors (fftoc::LdsTensorFft) in the same example: n in
+// {16, 64, 256, 4096}; two_product selects the two-real-product contraction.
+int fftoc_example_windowed_tensor_fft(const void *in, void *out, int64_t batch, int n,
+                                      const void *tw, int inverse, double scale, int window,
+                                      int two_product, void *stream) {
+  if (batch < 0) return FFTOC_ERROR_INVALID_BATCH;
+  if (batch == 0) return FFTOC_SUCCESS;
+  if (!in || !out || !tw) return FFTOC_ERROR_NULL_POINTER;
+  if (in == out) return FFTOC_ERROR_OVERLAP;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const bool tp = two_product != 0;
+  switch (n) {
+    case 16: return tp ? launch_tensor<16, true>(in, out, batch, tw, inverse, scale, window, s)
+                       : launch_tensor<16, false>(in, out, batch, tw, inverse, scale, window, s);
+    case 64: return tp ? launch_tensor<64, true>(in, out, batch, tw, inverse, scale, window, s)
+                       : launch_tensor<64, false>(in, out, batch, tw, inverse, scale, window, s);
+    case 256: return tp ? launch_tensor<256, true>(in, out, batch, tw, inverse, scale, window, s)
+                        : launch_tensor<256, false>(in, out, batch, tw, inverse, scale, window, s);
+    case 4096: return tp ? launch_tensor<4096, true>(in, out, batch, tw, inverse, scale, window, s)
+                         : launch_tensor<4096, false>(in, out, batch, tw, inverse, scale, window, s);
+    default: return FFTOC_ERROR_INVALID_SIZE;
+  }
+}
+
+// On-chip timing harness: `repeats` forward transforms (outputs times `scale`)
+// of the batch held in LDS between one load and one store, in place on data.
+// family 0 = fftoc::LdsFft (any n), 1 = fftoc::LdsTensorFft with Karatsuba, 2 =
+// with the two-product contraction (n in {16, 64, 256, 4096}).
+int fftoc_example_onchip_fft(void *data, int64_t batch, int n, const void *tw, int family,
+                             int repeats, double scale, void *stream) {
+  if (batch < 0 || repeats < 0) return FFTOC_ERROR_INVALID_BATCH;
+  if (batch == 0) return FFTOC_SUCCESS;
+  if (!data || !tw) return FFTOC_ERROR_NULL_POINTER;
+  if (family < 0 || family > 2) return FFTOC_ERROR_INVALID_FLAGS;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  switch (n) {
+    case 8: return onchip<8>(family, data, batch, tw, repeats, scale, s);
+    case 16: return onchip<16>(family, data, batch, tw, repeats, scale, s);
+    case 32: return onchip<32>(family, data, batch, tw, repeats, scale, s);
+    case 64: return onchip<64>(family, data, batch, tw, repeats, scale, s);
+    case 128: return onchip<128>(family, data, batch, tw, repeats, scale, s);
+    case 256: return onchip<256>(family, data, batch, tw, repeats, scale, s);
+    case 512: return onchip<512>(family, data, batch, tw, repeats, scale, s);
+    case 1024: return onchip<1024>(family, data, batch, tw, repeats, scale, s);
+    case 2048: return onchip<2048>(family, data, batch, tw, repeats, scale, s);
+    case 4096: return onchip<4096>(family, data, batch, tw, repeats, scale, s);
     default: return FFTOC_ERROR_INVALID_SIZE;
   }
 }

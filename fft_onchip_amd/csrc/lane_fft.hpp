@@ -28,31 +28,6 @@
 
 namespace fftoc {
 
-// DPP control words (GFX9 encoding).  row_ror:n makes lane l of a 16-lane
-// row read lane (l - n) mod 16 of the same row.
-constexpr int kDppQuadXor1 = 0xB1;  // quad_perm [1,0,3,2]: lane l reads l ^ 1
-constexpr int kDppQuadXor2 = 0x4E;  // quad_perm [2,3,0,1]: lane l reads l ^ 2
-constexpr int kDppRowRor4 = 0x124;  // reads (l - 4) mod 16
-constexpr int kDppRowRor8 = 0x128;  // reads (l + 8) mod 16 = l ^ 8
-constexpr int kDppRowRor12 = 0x12C; // reads (l + 4) mod 16
-
-template <int CTRL>
-__device__ __forceinline__ float dpp(float v) {
-  return __builtin_bit_cast(
-      float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xF, 0xF, true));
-}
-template <int CTRL>
-__device__ __forceinline__ double dpp(double v) {
-  const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
-  const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)u, CTRL, 0xF, 0xF, true);
-  const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(u >> 32), CTRL, 0xF, 0xF, true);
-  return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
-}
-template <int CTRL, class CT>
-__device__ __forceinline__ CT dpp_c(CT v) {
-  return mk<CT>(dpp<CTRL>(v.x), dpp<CTRL>(v.y));
-}
-
 // Value of lane l ^ 16 (ROWS = true) or l ^ 32 (ROWS = false) of a 32-bit
 // register.  permlane{16,32}_swap(x, x) returns (x with its odd rows / upper
 // half replaced by the even rows / lower half, x with its even rows / lower

@@ -24,195 +24,11 @@
 //  * Twiddles and DFT-matrix entries come from the FP64 table W_4096^j built
 //    on the host; only additions are done in-kernel (Br + Bi).
 #include "../../include/fftoc_device.hpp"
+#include "../../include/fftoc_mfma.hpp"
 #include "fftoc_internal.hpp"
-#include "lane_fft.hpp"
 
 namespace fftoc {
 namespace {
-
-typedef double v4d __attribute__((ext_vector_type(4)));
-
-__device__ __forceinline__ v4d mfma16(double a, double b, v4d c) {
-  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-}
-
-__device__ __forceinline__ v4d vzero() { return v4d{0.0, 0.0, 0.0, 0.0}; }
-
-// B fragment of the 16x16 DFT matrix M[k][col] = W_16^{k*col} in the
-// Karatsuba form; K-chunk c, slot g -> k = 4c + g.
-struct Frag {
-  double r[4], i[4], s[4];
-};
-
-__device__ __forceinline__ Frag dft16_frag(const c2 *__restrict__ tw, int lane) {
-  const int g = lane >> 4, j = lane & 15;
-  Frag f;
-#pragma unroll
-  for (int c = 0; c < 4; ++c) {
-    const int k = 4 * c + g;
-    const c2 w = tw[((k * j) & 15) << 8];  // W_16^{kj} = W_4096^{256 kj}
-    f.r[c] = w.x;
-    f.i[c] = w.y;
-    f.s[c] = w.x + w.y;
-  }
-  return f;
-}
-
-// Block-diagonal diag(F8, F8) with K order k = 4c + g.
-__device__ __forceinline__ Frag dft8x2_frag(const c2 *__restrict__ tw, int lane) {
-  const int g = lane >> 4, j = lane & 15;
-  Frag f;
-#pragma unroll
-  for (int c = 0; c < 4; ++c) {
-    const int k = 4 * c + g;
-    if ((k < 8) == (j < 8)) {
-      const c2 w = tw[(((k & 7) * (j & 7)) & 7) << 9];  // W_8^{..} = W_4096^{512 ..}
-      f.r[c] = w.x;
-      f.i[c] = w.y;
-      f.s[c] = w.x + w.y;
-    } else {
-      f.r[c] = 0.0;
-      f.i[c] = 0.0;
-      f.s[c] = 0.0;
-    }
-  }
-  return f;
-}
-
-// Complex 16x16x16 GEMM Y = X * M with 3 real MFMA chains (Karatsuba).
-// a[c] = A operand of K-chunk c as (re, im).
-__device__ __forceinline__ void kgemm(const c2 (&a)[4], const Frag &f, v4d &yr, v4d &yi) {
-  v4d p1 = vzero(), p2 = vzero(), p3 = vzero();
-#pragma unroll
-  for (int c = 0; c < 4; ++c) {
-    p1 = mfma16(a[c].x, f.r[c], p1);
-    p2 = mfma16(a[c].y, f.i[c], p2);
-    p3 = mfma16(a[c].x + a[c].y, f.s[c], p3);
-  }
-  yr = p1 - p2;
-  yi = p3 - p1 - p2;
-}
-
-// ---------------------------------------------------------------------------
-// Two-product DFT16 contraction.  The forward DFT16 matrix is C - iS with
-// C[k][m] = cos(2 pi k m / 16) even and S[k][m] = sin(2 pi k m / 16) odd in m,
-// so its 16 columns hold only 9 distinct cosine columns (m = 0..8) and 7
-// distinct sine columns (m = 1..7).  Packed side by side they make one real
-// 16 x 16 matrix P (columns 0..8 = C[.][0..8], columns 9..15 = S[.][1..7]),
-// and a complex product X * F needs two real MFMA products, Xr P and Xi P,
-// instead of Karatsuba's three:
-//   X[m]      = (Cr + Si) + i (Ci - Sr)      m = 1..7
-//   X[16 - m] = (Cr - Si) + i (Ci + Sr)      m = 1..7
-//   X[0], X[8] = the cosine sums alone,
-// with Cr, Ci (Sr, Si) the cosine (sine) sums of the real and imaginary parts.
-// P is symmetric in role: lane l holds P[4c + g][j] in K-chunk c both as the
-// B operand (data in A: packed frequency = column = lane j) and as the A
-// operand (data in B, P^T: packed frequency = row 4r + g = register r).
-struct HFrag {
-  double p[4];
-};
-
-__device__ __forceinline__ HFrag hdft16_frag(const c2 *__restrict__ tw, int lane) {
-  const int g = lane >> 4, j = lane & 15;
-  const int m = j <= 8 ? j : j - 8;
-  HFrag f;
-#pragma unroll
-  for (int c = 0; c < 4; ++c) {
-    const c2 w = tw[(((4 * c + g) * m) & 15) << 8];  // W_16^{km} = cos - i sin
-    f.p[c] = j <= 8 ? w.x : -w.y;
-  }
-  return f;
-}
-
-// Frequency of packed column j after the combine: j for the cosine columns,
-// 16 - (j - 8) for the sine columns.
-__device__ __forceinline__ int hphi(int j) { return j <= 8 ? j : 24 - j; }
-
-// Per-lane constants of the data-in-A combine: the lane's result is
-//   re = a u + b u' + c v + d v',  im = a v + b v' - c u - d u'
-// with (u, v) its own sums and (u', v') those of lane l ^ 8 (the other half of
-// its frequency).  Every coefficient is 0 or +-1, so the sums are exact.
-struct HCoef {
-  double a, b, c, d;
-};
-__device__ __forceinline__ HCoef hcoef(int j) {
-  HCoef k;
-  const bool cosine = j <= 8, edge = (j & 7) == 0;  // j = 0, 8: no sine partner
-  k.a = cosine ? 1.0 : 0.0;
-  k.b = cosine ? 0.0 : 1.0;
-  k.c = cosine ? 0.0 : -1.0;
-  k.d = (cosine && !edge) ? 1.0 : 0.0;
-  return k;
-}
-
-// DFT16 over K with the data as the A operand (rows = independent
-// transforms, K chunk c = k = 4c + g): y[r] = X[row 4r + g][hphi(j)].
-__device__ __forceinline__ void hgemm_a(const c2 (&a)[4], const HFrag &f, const HCoef &k,
-                                        c2 (&y)[4]) {
-  v4d u = vzero(), v = vzero();
-#pragma unroll
-  for (int c = 0; c < 4; ++c) {
-    u = mfma16(a[c].x, f.p[c], u);
-    v = mfma16(a[c].y, f.p[c], v);
-  }
-#pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    const double up = dpp<kDppRowRor8>(u[r]), vp = dpp<kDppRowRor8>(v[r]);
-    y[r].x = fma(k.d, vp, fma(k.c, v[r], fma(k.b, up, k.a * u[r])));
-    y[r].y = fma(-k.d, up, fma(-k.c, u[r], fma(k.b, vp, k.a * v[r])));
-  }
-}
-
-// DFT16 over K with the data as the B operand (columns = independent
-// transforms, K chunk c = k = 4c + g): o[q] = X[hk2(g, q)][column j].
-__device__ __forceinline__ void hgemm_b(const HFrag &f, const c2 (&b)[4], int g, c2 (&o)[4]) {
-  v4d u = vzero(), v = vzero();
-#pragma unroll
-  for (int c = 0; c < 4; ++c) {
-    u = mfma16(f.p[c], b[c].x, u);
-    v = mfma16(f.p[c], b[c].y, v);
-  }
-  // rows 4r + g: r = 0 cosine g, r = 1 cosine 4 + g, r = 2 sine g (cosine 8
-  // when g = 0), r = 3 sine 4 + g
-  const double e = g ? 1.0 : 0.0;
-  o[0] = make_double2(fma(e, v[2], u[0]), fma(-e, u[2], v[0]));
-  const c2 t = make_double2(u[0] - v[2], v[0] + u[2]);
-  o[1] = g ? t : make_double2(u[2], v[2]);
-  o[2] = make_double2(u[1] + v[3], v[1] - u[3]);
-  o[3] = make_double2(u[1] - v[3], v[1] + u[3]);
-}
-// Frequency of hgemm_b's output o[q] in a lane of group g.
-__device__ __forceinline__ int hk2(int g, int q) {
-  return q == 0 ? g : q == 1 ? (g ? 16 - g : 8) : q == 2 ? 4 + g : 12 - g;
-}
-
-// A DFT16 contraction with the data as the A operand, as a type: load() its
-// per-lane table values, run(a, y) -> y[r] = X[row 4r + g][frequency col].
-struct KaratsubaDft16 {
-  Frag f;
-  int col;
-  __device__ __forceinline__ void load(const c2 *__restrict__ tw, int lane) {
-    f = dft16_frag(tw, lane);
-    col = lane & 15;
-  }
-  __device__ __forceinline__ void run(const c2 (&a)[4], c2 (&y)[4]) const {
-    v4d yr, yi;
-    kgemm(a, f, yr, yi);
-#pragma unroll
-    for (int r = 0; r < 4; ++r) y[r] = make_double2(yr[r], yi[r]);
-  }
-};
-struct TwoProductDft16 {
-  HFrag f;
-  HCoef k;
-  int col;
-  __device__ __forceinline__ void load(const c2 *__restrict__ tw, int lane) {
-    f = hdft16_frag(tw, lane);
-    k = hcoef(lane & 15);
-    col = hphi(lane & 15);
-  }
-  __device__ __forceinline__ void run(const c2 (&a)[4], c2 (&y)[4]) const { hgemm_a(a, f, k, y); }
-};
 
 template <bool INV>
 __device__ __forceinline__ c2 ld_in(const c2 *p) {
@@ -559,94 +375,6 @@ int launch_tensor_persistent(const LaunchArgs &a) {
                        out, a.batch, tw, a.scale);
   return hipGetLastError() == hipSuccess ? FFTOC_SUCCESS : FFTOC_ERROR_LAUNCH;
 }
-
-// ---------------------------------------------------------------------------
-// 256-point core on one wave: input a[c] = x[64c + l] (n = 16 n1 + n2 with
-// n1 = 4c + g, n2 = j); output z[r] = X[4r + g + 16 j].
-// The inter-stage twiddles of core256 for this lane: w[r] = W_256^{n2 k1},
-// n2 = 4r + g, k1 = j.
-__device__ __forceinline__ void core256_twiddles(const c2 *__restrict__ tw, int lane, c2 (&w)[4]) {
-  const int g = lane >> 4, j = lane & 15;
-#pragma unroll
-  for (int r = 0; r < 4; ++r) w[r] = tw[((4 * r + g) * j) << 4];
-}
-
-__device__ __forceinline__ void core256(c2 (&a)[4], const Frag &f, const c2 (&w)[4], c2 (&z)[4]) {
-  v4d yr, yi;
-  kgemm(a, f, yr, yi);
-  // D: reg r = Y[n2 = 4r + g][k1 = j]; twiddle W_256^{n2 k1}
-  c2 b[4];
-#pragma unroll
-  for (int r = 0; r < 4; ++r) b[r] = cmul(make_double2(yr[r], yi[r]), w[r]);
-  // stage 2: A chunk r = b[r] (rows = k1 = j, K = n2 = 4r + g)
-  kgemm(b, f, yr, yi);
-#pragma unroll
-  for (int r = 0; r < 4; ++r) z[r] = make_double2(yr[r], yi[r]);
-}
-
-__device__ __forceinline__ void core256(c2 (&a)[4], const Frag &f, const c2 *__restrict__ tw,
-                                        int lane, c2 (&z)[4]) {
-  c2 w[4];
-  core256_twiddles(tw, lane, w);
-  core256(a, f, w, z);
-}
-
-// 256-point core with the two-product contraction: input a[c] = x[64c + l]
-// (n = 16 n1 + n2, n1 = 4c + g, n2 = j); stage 1 with the data in A gives
-// Y[n2 = 4r + g][k1 = hphi(j)] in register r, which after the twiddle
-// W_256^{n2 k1} is already the B operand of stage 2 (K = n2 = 4r + g, column
-// j); output z[q] = X[hphi(j) + 16 hk2(g, q)].  16 MFMAs against 24.
-__device__ __forceinline__ void core256h_twiddles(const c2 *__restrict__ tw, int lane,
-                                                  c2 (&w)[4]) {
-  const int g = lane >> 4, k1 = hphi(lane & 15);
-#pragma unroll
-  for (int r = 0; r < 4; ++r) w[r] = tw[((4 * r + g) * k1) << 4];
-}
-
-__device__ __forceinline__ void core256h(const c2 (&a)[4], const HFrag &f, const HCoef &k,
-                                         const c2 (&w)[4], int g, c2 (&z)[4]) {
-  c2 y[4];
-  hgemm_a(a, f, k, y);
-#pragma unroll
-  for (int r = 0; r < 4; ++r) y[r] = cmul(y[r], w[r]);
-  hgemm_b(f, y, g, z);
-}
-
-// A 256-point core as a type: its per-lane table values (load, once per
-// kernel), the transform of one row (run: a[c] = x[64c + l] in, z[q] out) and
-// the row position k1 + 16 k2 of z[q] (out_index).
-struct KaratsubaCore {  // three real products per DFT16 (24 MFMAs per core)
-  Frag f;
-  c2 w[4];
-  int g, j;
-  __device__ __forceinline__ void load(const c2 *__restrict__ tw, int lane) {
-    f = dft16_frag(tw, lane);
-    core256_twiddles(tw, lane, w);
-    g = lane >> 4;
-    j = lane & 15;
-  }
-  __device__ __forceinline__ void run(c2 (&a)[4], c2 (&z)[4]) const { core256(a, f, w, z); }
-  __device__ __forceinline__ int out_index(int q) const { return 4 * q + g + 16 * j; }
-};
-struct TwoProductCore {  // two real products per DFT16 (16 MFMAs per core)
-  HFrag f;
-  HCoef k;
-  c2 w[4];
-  int g, j;
-  __device__ __forceinline__ void load(const c2 *__restrict__ tw, int lane) {
-    f = hdft16_frag(tw, lane);
-    g = lane >> 4;
-    j = lane & 15;
-    k = hcoef(j);
-    core256h_twiddles(tw, lane, w);
-  }
-  __device__ __forceinline__ void run(c2 (&a)[4], c2 (&z)[4]) const { core256h(a, f, k, w, g, z); }
-  __device__ __forceinline__ int out_index(int q) const { return hphi(j) + 16 * hk2(g, q); }
-};
-
-// LDS element swizzle for the output transposes: XOR the low 4 bits of the
-// element index with bits 4..7 (16 x 16-byte slots per 256 B bank window).
-__device__ __forceinline__ int swz16(int e) { return e ^ ((e >> 4) & 15); }
 
 // N = 256: one signal per unit, two MFMA stages, output re-ordered through
 // 4 KiB of LDS per wave so every store instruction writes 1 KiB contiguously.

@@ -22,6 +22,9 @@
 // it synchronises the workgroup.  Element e of signal s is buf[s * N + e].
 #pragma once
 
+#include <type_traits>
+
+#include "fftoc_mfma.hpp"
 #include "fftoc_stockham.hpp"
 
 namespace fftoc {
@@ -55,6 +58,208 @@ struct LdsFft {
   // `scale` (1/N gives the normalised inverse).
   __device__ void operator()(bool inverse = false, real_t<CT> scale = 1) const {
     Kernel::run_lds(buf, tw, scale, inverse);
+  }
+};
+
+// ---------------------------------------------------------------------------
+// FP64 matrix-core (MFMA) in-LDS functors: the tensor path in the reference's
+// own functor form -- template <CT, Size, UPB, FPU> struct tensor_fft_N with
+// static threads / units_per_block / ffts_per_unit / max_threads_per_block,
+// constructed on the block's shared memory (its per-thread DFT-matrix and
+// twiddle values set up once there), operator() transforming each unit's
+// ffts_per_unit signals in place (includes/tensor_fft_64.cuh:12-77,
+// includes/tensor_fft_256.cuh:10-123, driven by includes/testing.cuh:39-43).
+//
+//   using F = fftoc::LdsTensorFft<256, 4, 2>;   // 4 units x 2 signals of 256
+//   __global__ __launch_bounds__(F::max_threads_per_block) void k(..., const double2 *tw) {
+//     __shared__ double2 buf[F::kElems];        // natural order, signal-major
+//     ... fill buf ...; __syncthreads();
+//     F f(buf, tw);                              // loads the lane constants
+//     f();                                       // forward, in place
+//     __syncthreads(); ... use buf ...
+//   }
+//
+// Unit u (u = threadIdx.x / threads, 0 <= u < units_per_block) owns the
+// signals at buf + u * N * ffts_per_unit.  A unit is one wave (N = 16, 64,
+// 256) or four waves (N = 4096, 64 KiB of LDS per signal).  Every thread of
+// the block calls operator(); units of one wave need no barrier, the N = 4096
+// unit synchronises the block.  `tw` is the library's FP64 table
+// (fftoc_twiddle_table).  TWO_PRODUCT selects the two-real-product DFT16
+// contraction instead of Karatsuba's three (the library's "2-product"
+// variants).  operator()(inverse, scale): forward or inverse DFT, each output
+// multiplied by scale.
+template <int N, int UNITS = 1, int FPU = 1, bool TWO_PRODUCT = false>
+struct LdsTensorFft;
+
+namespace lds_detail {
+__device__ __forceinline__ double2 ld_dir(const double2 *p, bool inverse) {
+  const double2 v = *p;
+  return inverse ? make_double2(v.y, v.x) : v;
+}
+__device__ __forceinline__ void st_dir(double2 *p, double2 v, bool inverse, double scale) {
+  v = make_double2(v.x * scale, v.y * scale);
+  *p = inverse ? make_double2(v.y, v.x) : v;
+}
+template <bool TWO_PRODUCT>
+using Dft16 = std::conditional_t<TWO_PRODUCT, TwoProductDft16, KaratsubaDft16>;
+template <bool TWO_PRODUCT>
+using Core256 = std::conditional_t<TWO_PRODUCT, TwoProductCore, KaratsubaCore>;
+}  // namespace lds_detail
+
+// N = 16: one DFT16 per matrix row, 16 signals per matrix product.
+template <int UNITS, int FPU, bool TWO_PRODUCT>
+struct LdsTensorFft<16, UNITS, FPU, TWO_PRODUCT> {
+  static_assert(FPU % 16 == 0, "N = 16: ffts_per_unit must be a multiple of 16");
+  static constexpr int threads = 64, units_per_block = UNITS, ffts_per_unit = FPU;
+  static constexpr int max_threads_per_block = UNITS * threads;
+  static constexpr int kElems = 16 * FPU * UNITS;
+  double2 *buf;
+  lds_detail::Dft16<TWO_PRODUCT> d;
+  __device__ LdsTensorFft(double2 *smem, const double2 *tw)
+      : buf(smem + 16 * FPU * (threadIdx.x / threads)) {
+    d.load(tw, threadIdx.x & 63);
+  }
+  __device__ void operator()(bool inverse = false, double scale = 1.0) const {
+    const int lane = threadIdx.x & 63, g = lane >> 4, j = lane & 15;
+    for (int t = 0; t < FPU / 16; ++t) {
+      double2 *x = buf + 256 * t;
+      c2 a[4], y[4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) a[c] = lds_detail::ld_dir(x + 16 * j + 4 * c + g, inverse);
+      d.run(a, y);
+      wave_lds_sync();  // the wave's reads of the tile precede its writes
+#pragma unroll
+      for (int r = 0; r < 4; ++r) lds_detail::st_dir(x + 16 * (4 * r + g) + d.col, y[r], inverse, scale);
+      wave_lds_sync();
+    }
+  }
+};
+
+// N = 64 = 16 x 4: four signals per matrix product (rows = (signal, n2)),
+// the DFT16 over n1 on the matrix cores, twiddle W_64^{n2 k1} and DFT4 over n2
+// in registers.
+template <int UNITS, int FPU, bool TWO_PRODUCT>
+struct LdsTensorFft<64, UNITS, FPU, TWO_PRODUCT> {
+  static_assert(FPU % 4 == 0, "N = 64: ffts_per_unit must be a multiple of 4");
+  static constexpr int threads = 64, units_per_block = UNITS, ffts_per_unit = FPU;
+  static constexpr int max_threads_per_block = UNITS * threads;
+  static constexpr int kElems = 64 * FPU * UNITS;
+  double2 *buf;
+  lds_detail::Dft16<TWO_PRODUCT> d;
+  c2 w[4];  // W_64^{n2 k1} of register r (n2 = r, k1 = the lane's frequency)
+  __device__ LdsTensorFft(double2 *smem, const double2 *tw)
+      : buf(smem + 64 * FPU * (threadIdx.x / threads)) {
+    d.load(tw, threadIdx.x & 63);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) w[r] = tw[(r * d.col) * (kTwN / 64)];
+  }
+  __device__ void operator()(bool inverse = false, double scale = 1.0) const {
+    const int lane = threadIdx.x & 63, g = lane >> 4, j = lane & 15;
+    for (int t = 0; t < FPU / 4; ++t) {
+      double2 *x = buf + 256 * t;
+      // row j <-> (signal j % 4, n2 = j / 4); K = n1 = 4c + g; n = 4 n1 + n2
+      c2 a[4], y[4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c)
+        a[c] = lds_detail::ld_dir(x + 64 * (j & 3) + 4 * (4 * c + g) + (j >> 2), inverse);
+      d.run(a, y);  // y[r]: signal g, n2 = r, k1 = d.col
+#pragma unroll
+      for (int r = 1; r < 4; ++r) y[r] = cmul(y[r], w[r]);
+      dft4(y[0], y[1], y[2], y[3]);
+      wave_lds_sync();
+#pragma unroll
+      for (int k2 = 0; k2 < 4; ++k2)
+        lds_detail::st_dir(x + 64 * g + d.col + 16 * k2, y[k2], inverse, scale);
+      wave_lds_sync();
+    }
+  }
+};
+
+// N = 256 = 16 x 16: the two-stage core, one signal per pass of the wave.
+template <int UNITS, int FPU, bool TWO_PRODUCT>
+struct LdsTensorFft<256, UNITS, FPU, TWO_PRODUCT> {
+  static constexpr int threads = 64, units_per_block = UNITS, ffts_per_unit = FPU;
+  static constexpr int max_threads_per_block = UNITS * threads;
+  static constexpr int kElems = 256 * FPU * UNITS;
+  double2 *buf;
+  lds_detail::Core256<TWO_PRODUCT> core;
+  __device__ LdsTensorFft(double2 *smem, const double2 *tw)
+      : buf(smem + 256 * FPU * (threadIdx.x / threads)) {
+    core.load(tw, threadIdx.x & 63);
+  }
+  __device__ void operator()(bool inverse = false, double scale = 1.0) const {
+    const int lane = threadIdx.x & 63;
+    for (int t = 0; t < FPU; ++t) {
+      double2 *x = buf + 256 * t;
+      c2 a[4], z[4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) a[c] = lds_detail::ld_dir(x + 64 * c + lane, inverse);
+      core.run(a, z);
+      wave_lds_sync();
+#pragma unroll
+      for (int r = 0; r < 4; ++r) lds_detail::st_dir(x + core.out_index(r), z[r], inverse, scale);
+      wave_lds_sync();
+    }
+  }
+};
+
+// N = 4096 = 16 x 256: stage A (VALU, thread t = column m: a DFT16 over n1 in
+// registers, twiddle W_4096^{m k1}, back into the column), stage B (MFMA: the
+// 256-point core of each row k1, four rows per wave, results back into the
+// row in core order), then a gather into natural order.
+template <int UNITS, int FPU, bool TWO_PRODUCT>
+struct LdsTensorFft<4096, UNITS, FPU, TWO_PRODUCT> {
+  static constexpr int threads = 256, units_per_block = UNITS, ffts_per_unit = FPU;
+  static constexpr int max_threads_per_block = UNITS * threads;
+  static constexpr int kElems = 4096 * FPU * UNITS;
+  double2 *buf;
+  lds_detail::Core256<TWO_PRODUCT> core;
+  TwBase<16, double2> ta;
+  __device__ LdsTensorFft(double2 *smem, const double2 *tw)
+      : buf(smem + 4096 * FPU * (threadIdx.x / threads)) {
+    core.load(tw, threadIdx.x & 63);
+    ta.fetch(tw, threadIdx.x % threads);
+  }
+  // position of X[k1 + 16 k2] inside row k1 (XOR of the low four bits: the
+  // row's writes and the gather's reads spread over the LDS banks)
+  __device__ static int rowpos(int k1, int k2) { return k1 * 256 + (k2 ^ (((k2 >> 4) ^ k1) & 15)); }
+  __device__ void operator()(bool inverse = false, double scale = 1.0) const {
+    const int t = threadIdx.x % threads, lane = t & 63, w = t >> 6;
+    for (int sgl = 0; sgl < FPU; ++sgl) {
+      double2 *x = buf + 4096 * sgl;
+      {
+        c2 v[16];
+#pragma unroll
+        for (int n1 = 0; n1 < 16; ++n1) v[n1] = lds_detail::ld_dir(x + 256 * n1 + t, inverse);
+        dft<16>(v);
+        apply_twiddle_base<16>(v, ta);
+#pragma unroll
+        for (int k1 = 0; k1 < 16; ++k1) x[256 * k1 + t] = v[k1];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int k1 = 4 * w + q;
+        c2 a[4], z[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) a[c] = x[256 * k1 + 64 * c + lane];
+        core.run(a, z);
+        wave_lds_sync();  // the wave alone reads and writes row k1
+#pragma unroll
+        for (int r = 0; r < 4; ++r) x[rowpos(k1, core.out_index(r))] = z[r];
+      }
+      __syncthreads();
+      c2 o[16];
+#pragma unroll
+      for (int c = 0; c < 16; ++c) {
+        const int e = 256 * c + t;
+        o[c] = x[rowpos(e & 15, e >> 4)];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int c = 0; c < 16; ++c) lds_detail::st_dir(x + 256 * c + t, o[c], inverse, scale);
+      __syncthreads();
+    }
   }
 };
 

@@ -92,6 +92,24 @@ def test_fused_example_library_loads_and_checks_arguments():
     assert f(None, None, 4, 64, None, 0, 1.0, 0, None) == 3      # FFTOC_ERROR_NULL_POINTER
     assert f(16, 16, 4, 64, 32, 0, 1.0, 0, None) == 5            # FFTOC_ERROR_OVERLAP
     assert f(16, 4096, 4, 48, 32, 0, 1.0, 0, None) == 1          # FFTOC_ERROR_INVALID_SIZE
+    lib = ctypes.CDLL(path)
+    t = lib.fftoc_example_windowed_tensor_fft
+    t.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_void_p,
+                  ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_void_p]
+    t.restype = ctypes.c_int
+    assert t(None, None, 0, 64, None, 0, 1.0, 0, 0, None) == 0
+    assert t(None, None, -1, 64, None, 0, 1.0, 0, 0, None) == 2
+    assert t(16, 16, 4, 64, 32, 0, 1.0, 0, 1, None) == 5
+    assert t(16, 4096, 4, 128, 32, 0, 1.0, 0, 0, None) == 1     # no tensor functor at 128
+    o = lib.fftoc_example_onchip_fft
+    o.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
+                  ctypes.c_int, ctypes.c_double, ctypes.c_void_p]
+    o.restype = ctypes.c_int
+    assert o(None, 0, 64, None, 0, 1, 1.0, None) == 0
+    assert o(None, 4, 64, None, 0, 1, 1.0, None) == 3
+    assert o(16, 4, 64, 32, 3, 1, 1.0, None) == 4                # unknown family
+    assert o(16, 4, 64, 32, 0, -1, 1.0, None) == 2               # negative repeat count
+    assert o(16, 4, 100, 32, 0, 1, 1.0, None) == 1
 
 
 def test_plan_rejects_jobs_writing_into_each_other():

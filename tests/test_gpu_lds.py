@@ -1,7 +1,9 @@
 """GPU parity of the device-callable in-LDS functors (include/fftoc_lds.hpp)
 through the fused example library: Hann window applied while loading into
-LDS, fftoc::LdsFft in place, store.  Checked against the oracle applied to
-the windowed signals, FP64 tolerance 1e-13 relative L2 per signal."""
+LDS, fftoc::LdsFft (radix passes) or fftoc::LdsTensorFft (matrix cores, both
+contractions) in place, store; and the on-chip timing harness (one load, R
+transforms in LDS, one store).  Checked against the oracle applied to the
+windowed signals, FP64 tolerance 1e-13 relative L2 per signal."""
 import ctypes
 import os
 
@@ -31,6 +33,15 @@ def example():
     f.restype = ctypes.c_int
     tw = ctypes.c_void_p()
     assert fo.lib().fftoc_twiddle_table(ctypes.byref(tw), None) == 0 and tw.value
+    t = lib.fftoc_example_windowed_tensor_fft
+    t.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_void_p,
+                  ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_void_p]
+    t.restype = ctypes.c_int
+    o = lib.fftoc_example_onchip_fft
+    o.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
+                  ctypes.c_int, ctypes.c_double, ctypes.c_void_p]
+    o.restype = ctypes.c_int
+    f.tensor, f.onchip = t, o
     return f, tw
 
 
@@ -68,3 +79,55 @@ def test_twiddle_table_entry_point():
     t = download_ptr(a.value, (4096,), np.complex128)
     assert np.max(np.abs(t - np.exp(-2j * np.pi * np.arange(4096) / 4096))) < 1e-15
     assert fo.lib().fftoc_twiddle_table(None, None) == 3  # FFTOC_ERROR_NULL_POINTER
+
+
+# signals per workgroup of the example's tensor functors (fused_example.hip)
+TENSOR_PER_GROUP = {16: 64, 64: 32, 256: 8, 4096: 1}
+
+
+@pytest.mark.parametrize("n", sorted(TENSOR_PER_GROUP))
+@pytest.mark.parametrize("two_product", [False, True])
+@pytest.mark.parametrize("window,inverse", [(False, False), (True, True)])
+def test_in_lds_tensor_functor_matches_oracle(example, n, two_product, window, inverse):
+    """fftoc::LdsTensorFft<N, UNITS, FPU, TWO_PRODUCT> (the reference's
+    tensor_fft_N functor form on the matrix cores), several units per
+    workgroup, a partial last workgroup."""
+    f, tw = example
+    batch = 2 * TENSOR_PER_GROUP[n] + 1
+    x = signals(n, batch, seed=800 + n)
+    xw = x * (0.5 - 0.5 * np.cos(2 * np.pi * np.arange(n) / n)) if window else x
+    scale = 1.0 / n if inverse else 1.0
+    print(f"[lds tensor] n={n} two_product={two_product} window={window} inverse={inverse}", flush=True)
+    src = fo.DeviceBuffer.from_array(x)
+    dst = fo.DeviceBuffer(x.nbytes)
+    assert f.tensor(src.ptr, dst.ptr, batch, n, tw, int(inverse), scale, int(window),
+                    int(two_product), None) == 0
+    fo.synchronize()
+    got = dst.download(x.shape)
+    ref = oracle.reference(xw, inverse=inverse)
+    if inverse:
+        ref = oracle.DD(ref.hi / n, ref.lo / n)
+    err = np.max(oracle.rel_l2(got, ref, axis=-1))
+    assert err <= TOL, (n, two_product, window, inverse, err)
+
+
+@pytest.mark.parametrize("n,family", [(8, 0), (64, 0), (128, 0), (256, 0), (512, 0), (4096, 0),
+                                      (16, 1), (64, 1), (256, 1), (4096, 1), (64, 2), (4096, 2)])
+def test_onchip_harness_repeats_in_lds(example, n, family):
+    """The on-chip timing harness: R = 1 and 2 transforms of the LDS-resident
+    signals, each output times 1/sqrt(N), equal the oracle applied R times."""
+    f, tw = example
+    batch = 67
+    x = signals(n, batch, seed=900 + n)
+    scale = 1.0 / np.sqrt(n)
+    ref = x
+    for repeats in (1, 2):
+        print(f"[onchip] n={n} family={family} repeats={repeats}", flush=True)
+        buf = fo.DeviceBuffer.from_array(x)
+        assert f.onchip(buf.ptr, batch, n, tw, family, repeats, scale, None) == 0
+        fo.synchronize()
+        got = buf.download(x.shape)
+        r = oracle.reference(ref)
+        ref = r.as_longdouble().astype(np.complex128) * scale  # input of the next repeat
+        err = np.max(oracle.rel_l2(got, oracle.DD(r.hi * scale, r.lo * scale), axis=-1))
+        assert err <= TOL, (n, family, repeats, err)
