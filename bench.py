@@ -134,6 +134,93 @@ def run_gpu(fo, wl: Workload, steps: int, warmup: int, host_x: np.ndarray | None
     }, x, sample
 
 
+# The reference's on-chip rows (scripts/results/configs.csv): microseconds per
+# in-shared-memory pass over 32 FFTs per SM on a 108-SM GPU (3456 FFTs), best
+# matrix-core functor per length, cuFFTDx at N = 4096 (no matrix-core row).
+ONCHIP_REF = {8: (19, "MMA FPU=8 UPB=2", 0.2355),
+              64: (195, "MMA FPU=1 UPB=32", 1.7070),
+              128: (236, "MMA FPU=1 UPB=16", 3.8533),
+              256: (261, "MMA FPU=1 UPB=8", 8.3804),
+              512: (267, "MMA FPU=2 UPB=1", 35.714),
+              4096: (274, "cuFFTDx", 253.27)}
+ONCHIP_REF_FFTS = 32 * 108
+ONCHIP_FAMILIES = {0: "radix (fftoc::LdsFft)", 1: "tensor Karatsuba (fftoc::LdsTensorFft)",
+                   2: "tensor two-product (fftoc::LdsTensorFft<.., true>)"}
+
+
+def run_onchip(fo, reps: int = 3):
+    """On-chip timing with the reference's method (includes/testing.cuh:94-127):
+    one launch loads the batch into LDS, runs the in-LDS functor R times and
+    stores once; R = 100 and R = 1100 are timed with HIP events on the launch
+    stream and (t1100 - t100) / 1000 is the time of one pass over the batch,
+    HBM traffic excluded.  Every functor family at N = 8, 64, 128, 256, 512,
+    4096; batches of 32 FFTs per CU (the reference's shape), and one and four
+    LDS-fulls of signals per CU (160 KiB = 10240 points each), the best of
+    which is reported as the saturated rate."""
+    import ctypes
+    from fft_onchip_amd import hip
+
+    lib = ctypes.CDLL(os.path.join(ROOT, "fft_onchip_amd", "libfftoc_fused_example.so"))
+    run = lib.fftoc_example_onchip_fft
+    run.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
+                    ctypes.c_int, ctypes.c_double, ctypes.c_void_p]
+    run.restype = ctypes.c_int
+    tw = ctypes.c_void_p()
+    if fo.lib().fftoc_twiddle_table(ctypes.byref(tw), None) != 0:
+        raise RuntimeError("fftoc_twiddle_table failed")
+    cus = hip.cu_count()
+    stream = hip.Stream()
+    out = {"method": "(t[R=1100] - t[R=100]) / 1000 per pass; HIP events on the launch stream; "
+                     f"median of {reps}", "cus": cus, "rows": {}}
+    shapes = {"32_per_cu": lambda n: 32 * cus, "lds_x1": lambda n: max(1, 10240 * cus // n),
+              "lds_x4": lambda n: max(1, 4 * 10240 * cus // n)}
+    for n, (row, cfg, ref_us) in ONCHIP_REF.items():
+        ref_rate = ONCHIP_REF_FFTS / (ref_us * 1e-6)
+        entry = {"reference": {"row": f"scripts/results/configs.csv:{row}",
+                               "config": cfg, "us_per_pass": ref_us, "ffts_per_pass": ONCHIP_REF_FFTS,
+                               "ffts_per_s": ref_rate}}
+        flops = 5.0 * n * math.log2(n)
+        g = np.random.Generator(np.random.Philox(31 + n))
+        big = max(fb(n) for fb in shapes.values())
+        xs = g.uniform(-1, 1, (big, n)) + 1j * g.uniform(-1, 1, (big, n))
+        for fam, fam_name in ONCHIP_FAMILIES.items():
+            if fam == 2 and n == 8:
+                continue  # the two-product contraction starts at N = 16
+            res = {"functor": fam_name}
+            best = 0.0
+            for shape, fb in shapes.items():
+                batch = fb(n)
+                buf = hip.DeviceBuffer.from_array(np.ascontiguousarray(xs[:batch]))
+
+                def timed(r):
+                    a, b = hip.Event(), hip.Event()
+                    a.record(stream)
+                    rc = run(buf.ptr, batch, n, tw, fam, r, 1.0 / math.sqrt(n), stream.handle)
+                    b.record(stream)
+                    stream.synchronize()
+                    if rc != 0:
+                        raise RuntimeError(f"fftoc_example_onchip_fft n={n} family={fam}: {rc}")
+                    return a.elapsed_ms(b)
+
+                timed(100)  # warm-up (clocks, code load)
+                t100 = statistics.median(timed(100) for _ in range(reps))
+                t1100 = statistics.median(timed(1100) for _ in range(reps))
+                buf.free()
+                us = (t1100 - t100) * 1e3 / 1000.0
+                rate = batch / (us * 1e-6) if us > 0 else float("nan")
+                res[shape] = {"batch": batch, "us_per_pass": round(us, 4), "ffts_per_s": rate,
+                              "GFLOPps": rate * flops / 1e9}
+                best = max(best, rate)
+            res["ffts_per_s"] = best
+            res["GFLOPps"] = best * flops / 1e9
+            res["vs_reference"] = round(best / ref_rate, 3)
+            entry[f"family{fam}"] = res
+            print(f"[bench] onchip n={n} {fam_name}: {best:.4g} FFT/s ({best / ref_rate:.2f}x reference)",
+                  file=sys.stderr, flush=True)
+        out["rows"][str(n)] = entry
+    return out
+
+
 def run_precision(fo, steps: int, warmup: int):
     """C4: FP64 vs FP32 vs 3-term split-BF16 MFMA stage GEMMs on the
     dynamic-range stress inputs (seed 11), N = 64 x 65536 and 4096 x 16384.
@@ -521,6 +608,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--quick", action="store_true", help="headline workload only")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--onchip", action="store_true", help="with --quick: the on-chip leg too")
     ap.add_argument("--only", metavar="PATH:N[:PREC]",
                     help="time one workload (path best|tensor|legacy8|legacy16, length N at its "
                          "benchmark batch, precision fp64|fp32|bf16x3) and print its result; "
@@ -585,10 +673,15 @@ def main():
                   # the tensor path with all three DFT16 stages on the matrix cores
                   Workload("fp64_n4096_b16384_tensor_3mfma", 4096, 16384, 7, "normal",
                            "variant:tensor:16x16x16/8 waves"),
+                  # the two-real-product DFT16 contraction (packed cosine/sine columns)
+                  Workload("fp64_n4096_b16384_tensor_2product", 4096, 16384, 7, "normal",
+                           "variant:tensor:16x16x16/VALU stage A, 4 waves/2-product"),
                   Workload("fp64_n4096_b16384_legacy16", 4096, 16384, 7, "normal", "legacy16"),
                   Workload("fp64_n4096_b16384_legacy8", 4096, 16384, 7, "normal", "legacy8")]
         for path in ("best", "tensor", "legacy8"):
             others.append(Workload(f"fp64_n64_b65536_{path}", 64, 65536, 0, "uniform", path))
+        others.append(Workload("fp64_n64_b65536_tensor_2product", 64, 65536, 0, "uniform",
+                               "variant:tensor:16x4/8 per wave/persistent/2-product"))
         for i, n in enumerate((8, 16, 128, 256, 512)):
             for path in ("best", "tensor"):
                 others.append(Workload(f"fp64_n{n}_b{(1 << 24) // n}_{path}", n, (1 << 24) // n,
@@ -639,6 +732,8 @@ def main():
         print(f"[bench] batched plan: {extra['batched_mixed_c5']}", file=sys.stderr, flush=True)
         extra["served_c5"] = run_served(fo)
         print(f"[bench] served: {extra['served_c5']}", file=sys.stderr, flush=True)
+    if not args.quick or args.onchip:
+        extra["onchip"] = run_onchip(fo)
 
     cpu = None
     if not args.no_cpu:

@@ -65,11 +65,18 @@ int launch(const void *in, void *out, int64_t batch, const void *tw, int inverse
 
 // Tensor functor per length: units of one wave (N <= 256) with several
 // signals each, or one four-wave unit (N = 4096); 256 threads per workgroup.
+// N = 8 has the Karatsuba contraction only.
 template <int N, bool TP> struct TensorCfg;
+template <bool TP> struct TensorCfg<8, TP> { using F = fftoc::LdsTensorFft<8, 4, 64, false>; };
 template <bool TP> struct TensorCfg<16, TP> { using F = fftoc::LdsTensorFft<16, 4, 16, TP>; };
 template <bool TP> struct TensorCfg<64, TP> { using F = fftoc::LdsTensorFft<64, 4, 8, TP>; };
+template <bool TP> struct TensorCfg<128, TP> { using F = fftoc::LdsTensorFft<128, 4, 4, TP>; };
 template <bool TP> struct TensorCfg<256, TP> { using F = fftoc::LdsTensorFft<256, 4, 2, TP>; };
+template <bool TP> struct TensorCfg<512, TP> { using F = fftoc::LdsTensorFft<512, 4, 1, TP>; };
 template <bool TP> struct TensorCfg<4096, TP> { using F = fftoc::LdsTensorFft<4096, 1, 1, TP>; };
+template <int N, bool TP>
+constexpr bool kHasTensor = (N == 8 && !TP) || N == 16 || N == 64 || N == 128 || N == 256 ||
+                            N == 512 || N == 4096;
 
 // Threads and signals per workgroup of either functor family.
 template <class F> struct Shape {
@@ -131,6 +138,15 @@ int launch_tensor(const void *in, void *out, int64_t batch, const void *tw, int 
   return hipGetLastError() == hipSuccess ? FFTOC_SUCCESS : FFTOC_ERROR_LAUNCH;
 }
 
+template <int N>
+int tensor(bool tp, const void *in, void *out, int64_t batch, const void *tw, int inverse,
+           double scale, int window, hipStream_t s) {
+  if constexpr (kHasTensor<N, true>)
+    if (tp) return launch_tensor<N, true>(in, out, batch, tw, inverse, scale, window, s);
+  if (tp) return FFTOC_ERROR_UNSUPPORTED;
+  return launch_tensor<N, false>(in, out, batch, tw, inverse, scale, window, s);
+}
+
 // The reference's timing harness with either functor family: one load, R
 // forward transforms of the LDS-resident signals (each output times `scale`,
 // 1/sqrt(N) keeping the values' magnitude), one store, in place.
@@ -165,12 +181,12 @@ template <int N>
 int onchip(int family, void *data, int64_t batch, const void *tw, int repeats, double scale,
            hipStream_t s) {
   if (family == 0) return launch_onchip<N, Fft<N>>(data, batch, tw, repeats, scale, s);
-  if constexpr (N == 16 || N == 64 || N == 256 || N == 4096) {
+  if constexpr (kHasTensor<N, false>)
     if (family == 1)
       return launch_onchip<N, typename TensorCfg<N, false>::F>(data, batch, tw, repeats, scale, s);
+  if constexpr (kHasTensor<N, true>)
     if (family == 2)
       return launch_onchip<N, typename TensorCfg<N, true>::F>(data, batch, tw, repeats, scale, s);
-  }
   return FFTOC_ERROR_UNSUPPORTED;
 }
 
@@ -204,7 +220,8 @@ int fftoc_example_windowed_fft(const void *in, void *out, int64_t batch, int n, 
 }
 
 // The matrix-core functors (fftoc::LdsTensorFft) in the same example: n in
-// {16, 64, 256, 4096}; two_product selects the two-real-product contraction.
+// {8, 16, 64, 128, 256, 512, 4096}; two_product selects the two-real-product
+// contraction (not at n = 8).
 int fftoc_example_windowed_tensor_fft(const void *in, void *out, int64_t batch, int n,
                                       const void *tw, int inverse, double scale, int window,
                                       int two_product, void *stream) {
@@ -215,14 +232,13 @@ int fftoc_example_windowed_tensor_fft(const void *in, void *out, int64_t batch, 
   hipStream_t s = static_cast<hipStream_t>(stream);
   const bool tp = two_product != 0;
   switch (n) {
-    case 16: return tp ? launch_tensor<16, true>(in, out, batch, tw, inverse, scale, window, s)
-                       : launch_tensor<16, false>(in, out, batch, tw, inverse, scale, window, s);
-    case 64: return tp ? launch_tensor<64, true>(in, out, batch, tw, inverse, scale, window, s)
-                       : launch_tensor<64, false>(in, out, batch, tw, inverse, scale, window, s);
-    case 256: return tp ? launch_tensor<256, true>(in, out, batch, tw, inverse, scale, window, s)
-                        : launch_tensor<256, false>(in, out, batch, tw, inverse, scale, window, s);
-    case 4096: return tp ? launch_tensor<4096, true>(in, out, batch, tw, inverse, scale, window, s)
-                         : launch_tensor<4096, false>(in, out, batch, tw, inverse, scale, window, s);
+    case 8: return tensor<8>(tp, in, out, batch, tw, inverse, scale, window, s);
+    case 16: return tensor<16>(tp, in, out, batch, tw, inverse, scale, window, s);
+    case 64: return tensor<64>(tp, in, out, batch, tw, inverse, scale, window, s);
+    case 128: return tensor<128>(tp, in, out, batch, tw, inverse, scale, window, s);
+    case 256: return tensor<256>(tp, in, out, batch, tw, inverse, scale, window, s);
+    case 512: return tensor<512>(tp, in, out, batch, tw, inverse, scale, window, s);
+    case 4096: return tensor<4096>(tp, in, out, batch, tw, inverse, scale, window, s);
     default: return FFTOC_ERROR_INVALID_SIZE;
   }
 }
@@ -230,7 +246,8 @@ int fftoc_example_windowed_tensor_fft(const void *in, void *out, int64_t batch, 
 // On-chip timing harness: `repeats` forward transforms (outputs times `scale`)
 // of the batch held in LDS between one load and one store, in place on data.
 // family 0 = fftoc::LdsFft (any n), 1 = fftoc::LdsTensorFft with Karatsuba, 2 =
-// with the two-product contraction (n in {16, 64, 256, 4096}).
+// with the two-product contraction (tensor n in {8, 16, 64, 128, 256, 512,
+// 4096}, two-product not at 8).
 int fftoc_example_onchip_fft(void *data, int64_t batch, int n, const void *tw, int family,
                              int repeats, double scale, void *stream) {
   if (batch < 0 || repeats < 0) return FFTOC_ERROR_INVALID_BATCH;

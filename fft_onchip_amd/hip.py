@@ -41,6 +41,8 @@ def lib():
         h.hipEventSynchronize.argtypes = [vp]
         h.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), vp, vp]
         h.hipGetDeviceCount.argtypes = [ctypes.POINTER(ctypes.c_int)]
+        h.hipGetDevice.argtypes = [ctypes.POINTER(ctypes.c_int)]
+        h.hipDeviceGetAttribute.argtypes = [ctypes.POINTER(ctypes.c_int), ctypes.c_int, ctypes.c_int]
         h.hipGetErrorString.argtypes = [ctypes.c_int]
         h.hipGetErrorString.restype = ctypes.c_char_p
         h.hipMemGetInfo.argtypes = [ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)]
@@ -58,6 +60,18 @@ def device_count() -> int:
     n = ctypes.c_int(0)
     rc = lib().hipGetDeviceCount(ctypes.byref(n))
     return n.value if rc == 0 else 0
+
+
+_ATTR_MULTIPROCESSOR_COUNT = 63  # hipDeviceAttributeMultiprocessorCount (hip_runtime_api.h)
+
+
+def cu_count() -> int:
+    """Compute units of the current device."""
+    dev, v = ctypes.c_int(0), ctypes.c_int(0)
+    check(lib().hipGetDevice(ctypes.byref(dev)), "hipGetDevice")
+    check(lib().hipDeviceGetAttribute(ctypes.byref(v), _ATTR_MULTIPROCESSOR_COUNT, dev.value),
+          "hipDeviceGetAttribute")
+    return v.value
 
 
 def synchronize() -> None:

@@ -175,6 +175,156 @@ struct LdsTensorFft<64, UNITS, FPU, TWO_PRODUCT> {
   }
 };
 
+// N = 8: 32 signals per matrix product (the A operand holds two signals per
+// row, the DFT matrix is diag(F8, F8)); Karatsuba contraction only.
+template <int UNITS, int FPU, bool TWO_PRODUCT>
+struct LdsTensorFft<8, UNITS, FPU, TWO_PRODUCT> {
+  static_assert(!TWO_PRODUCT, "N = 8: the Karatsuba contraction only");
+  static_assert(FPU % 32 == 0, "N = 8: ffts_per_unit must be a multiple of 32");
+  static constexpr int threads = 64, units_per_block = UNITS, ffts_per_unit = FPU;
+  static constexpr int max_threads_per_block = UNITS * threads;
+  static constexpr int kElems = 8 * FPU * UNITS;
+  double2 *buf;
+  Frag f;
+  __device__ LdsTensorFft(double2 *smem, const double2 *tw)
+      : buf(smem + 8 * FPU * (threadIdx.x / threads)) {
+    f = dft8x2_frag(tw, threadIdx.x & 63);
+  }
+  __device__ void operator()(bool inverse = false, double scale = 1.0) const {
+    const int lane = threadIdx.x & 63, g = lane >> 4, j = lane & 15;
+    for (int t = 0; t < FPU / 32; ++t) {
+      double2 *x = buf + 256 * t;
+      // row j: signals j (K chunks 0, 1) and 16 + j (chunks 2, 3)
+      c2 a[4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c)
+        a[c] = lds_detail::ld_dir(x + 8 * ((c < 2 ? 0 : 16) + j) + 4 * (c & 1) + g, inverse);
+      v4d yr, yi;
+      kgemm(a, f, yr, yi);
+      wave_lds_sync();
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        lds_detail::st_dir(x + 8 * ((j < 8 ? 0 : 16) + 4 * r + g) + (j & 7),
+                           make_double2(yr[r], yi[r]), inverse, scale);
+      wave_lds_sync();
+    }
+  }
+};
+
+// N = 128 = 16 x 8: two signals per matrix product (rows = (signal, n2)),
+// the DFT16 over n1 on the matrix cores, twiddle W_128^{n2 k1}, DFT8 over n2
+// as a DFT4 in registers and one radix-2 step with the lane 32 apart.
+template <int UNITS, int FPU, bool TWO_PRODUCT>
+struct LdsTensorFft<128, UNITS, FPU, TWO_PRODUCT> {
+  static_assert(FPU % 2 == 0, "N = 128: ffts_per_unit must be even");
+  static constexpr int threads = 64, units_per_block = UNITS, ffts_per_unit = FPU;
+  static constexpr int max_threads_per_block = UNITS * threads;
+  static constexpr int kElems = 128 * FPU * UNITS;
+  double2 *buf;
+  lds_detail::Dft16<TWO_PRODUCT> d;
+  c2 w[4];  // W_128^{n2 k1}, n2 = 2r + g / 2
+  __device__ LdsTensorFft(double2 *smem, const double2 *tw)
+      : buf(smem + 128 * FPU * (threadIdx.x / threads)) {
+    const int lane = threadIdx.x & 63;
+    d.load(tw, lane);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) w[r] = tw[((2 * r + (lane >> 5)) * d.col) * (kTwN / 128)];
+  }
+  __device__ void operator()(bool inverse = false, double scale = 1.0) const {
+    const int lane = threadIdx.x & 63, g = lane >> 4, j = lane & 15, h = g >> 1;
+    for (int t = 0; t < FPU / 2; ++t) {
+      double2 *x = buf + 256 * t;
+      // row j <-> (signal j % 2, n2 = j / 2); K = n1 = 4c + g; n = 8 n1 + n2
+      c2 a[4], y[4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c)
+        a[c] = lds_detail::ld_dir(x + 128 * (j & 1) + 8 * (4 * c + g) + (j >> 1), inverse);
+      d.run(a, y);  // y[r]: signal g % 2, n2 = 2r + h, k1 = d.col
+#pragma unroll
+      for (int r = 0; r < 4; ++r) y[r] = cmul(y[r], w[r]);
+      // DIT over n2 = 2m + h: E/O = DFT4 over m, X[k] = E[k] + W8^k O[k],
+      // X[k + 4] = E[k] - W8^k O[k]
+      dft4(y[0], y[1], y[2], y[3]);
+      if (h == 1) {
+        y[1] = mul_w8_1(y[1]);
+        y[2] = mul_mi(y[2]);
+        y[3] = mul_w8_3(y[3]);
+      }
+      c2 p[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        p[r].x = __shfl_xor(y[r].x, 32);
+        p[r].y = __shfl_xor(y[r].y, 32);
+      }
+      wave_lds_sync();
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const c2 v = (h == 0) ? cadd(y[k], p[k]) : csub(p[k], y[k]);
+        lds_detail::st_dir(x + 128 * (g & 1) + d.col + 16 * (4 * h + k), v, inverse, scale);
+      }
+      wave_lds_sync();
+    }
+  }
+};
+
+// N = 512 = 16 x 2 x 16: one signal per pass of the wave.  n = 32 n1 + 16 h +
+// n2'; a DFT16 over n1 for h = 0, 1, twiddle W_512^{n2 k1}, a radix-2 over h
+// with W_32^{n2' k2a}, a DFT16 over n2' for k2a = 0, 1; X[k1 + 16 k2a + 32 k2b].
+template <int UNITS, int FPU, bool TWO_PRODUCT>
+struct LdsTensorFft<512, UNITS, FPU, TWO_PRODUCT> {
+  static constexpr int threads = 64, units_per_block = UNITS, ffts_per_unit = FPU;
+  static constexpr int max_threads_per_block = UNITS * threads;
+  static constexpr int kElems = 512 * FPU * UNITS;
+  using D = lds_detail::Dft16<TWO_PRODUCT>;
+  double2 *buf;
+  D d;
+  c2 w1[2][4];  // W_512^{n2 k1}, n2 = 16 h + 4 r + g, k1 = d.col
+  c2 w2[4];     // W_32^{n2' k2a = 1}, n2' = 4 r + g
+  __device__ LdsTensorFft(double2 *smem, const double2 *tw)
+      : buf(smem + 512 * FPU * (threadIdx.x / threads)) {
+    const int lane = threadIdx.x & 63, g = lane >> 4;
+    d.load(tw, lane);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+#pragma unroll
+      for (int h = 0; h < 2; ++h) w1[h][r] = tw[((16 * h + 4 * r + g) * d.col) * (kTwN / 512)];
+      w2[r] = tw[(4 * r + g) * (kTwN / 32)];
+    }
+  }
+  __device__ void operator()(bool inverse = false, double scale = 1.0) const {
+    const int lane = threadIdx.x & 63, g = lane >> 4, j = lane & 15;
+    for (int t = 0; t < FPU; ++t) {
+      double2 *x = buf + 512 * t;
+      c2 yh[2][4];
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        c2 a[4];  // row j = n2' , K = n1 = 4c + g
+#pragma unroll
+        for (int c = 0; c < 4; ++c) a[c] = lds_detail::ld_dir(x + 32 * (4 * c + g) + 16 * h + j, inverse);
+        d.run(a, yh[h]);  // register r: n2' = 4r + g, k1 = d.col
+#pragma unroll
+        for (int r = 0; r < 4; ++r) yh[h][r] = cmul(yh[h][r], w1[h][r]);
+      }
+      c2 b[2][4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        b[0][r] = cadd(yh[0][r], yh[1][r]);
+        b[1][r] = cmul(csub(yh[0][r], yh[1][r]), w2[r]);
+      }
+      wave_lds_sync();
+#pragma unroll
+      for (int k2a = 0; k2a < 2; ++k2a) {
+        c2 z[4];  // rows = the lanes' k1 columns of stage 1, K = n2'
+        d.run(b[k2a], z);  // register r: row 4r + g (k1 = freq(4r + g)), k2b = d.col
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          lds_detail::st_dir(x + D::freq(4 * r + g) + 16 * k2a + 32 * d.col, z[r], inverse, scale);
+      }
+      wave_lds_sync();
+    }
+  }
+};
+
 // N = 256 = 16 x 16: the two-stage core, one signal per pass of the wave.
 template <int UNITS, int FPU, bool TWO_PRODUCT>
 struct LdsTensorFft<256, UNITS, FPU, TWO_PRODUCT> {

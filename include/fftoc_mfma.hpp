@@ -179,6 +179,8 @@ __device__ __forceinline__ int hk2(int g, int q) {
 // A DFT16 contraction with the data as the A operand, as a type: load() its
 // per-lane table values, run(a, y) -> y[r] = X[row 4r + g][frequency col].
 struct KaratsubaDft16 {
+  // frequency held by output column (lane j) or row i of a product
+  __device__ static __forceinline__ int freq(int i) { return i; }
   Frag f;
   int col;
   __device__ __forceinline__ void load(const c2 *__restrict__ tw, int lane) {
@@ -193,6 +195,7 @@ struct KaratsubaDft16 {
   }
 };
 struct TwoProductDft16 {
+  __device__ static __forceinline__ int freq(int i) { return hphi(i); }
   HFrag f;
   HCoef k;
   int col;
