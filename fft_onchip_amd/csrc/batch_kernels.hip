@@ -20,13 +20,18 @@ constexpr int kBatchThreads = 256;
 constexpr int kLdsElems = lds_elems(4096);
 
 // Tile forms: a Stockham tile read from HBM directly (Tile) or staged
-// through LDS (Staged), or the lane-shuffle tile (Lanes, N = 8 ... 64).
-enum class TForm { Tile, Staged, Lanes };
+// through LDS (Staged), one tile after another; the Stockham tile pipelined
+// over the routine's tiles (Stream: the next tile's loads in flight during
+// this tile's passes); the lane-shuffle tile, pipelined (Lanes, N = 8 ... 64).
+enum class TForm { Tile, Staged, Stream, Lanes };
 
 template <class SP, int S, TForm F, class CT>
 struct RoutineKernel {
   using type = StockhamKernel<SP, S, 2, F == TForm::Staged, CT>;
 };
+// which kernels a routine streams with run_stream (the others run run_tile
+// per tile)
+constexpr bool streams(TForm f) { return f == TForm::Stream || f == TForm::Lanes; }
 template <class SP, int S, class CT>
 struct RoutineKernel<SP, S, TForm::Lanes, CT> {
   using type = LaneTile<SP::N, S * SP::N / kBatchThreads, 2, CT>;
@@ -59,7 +64,19 @@ struct RoutineKernel<SP, S, TForm::Lanes, CT> {
   X(20, 8, "8/lanes/S128", (Split<8>), 128, TForm::Lanes) \
   X(21, 16, "16/lanes/S64", (Split<16>), 64, TForm::Lanes) \
   X(22, 32, "32/lanes/S32", (Split<32>), 32, TForm::Lanes) \
-  X(23, 64, "64/lanes/S16", (Split<64>), 16, TForm::Lanes)
+  X(23, 64, "64/lanes/S16", (Split<64>), 16, TForm::Lanes) \
+  X(24, 64, "8x8/stream/S32", (Split<8, 8>), 32, TForm::Stream) \
+  X(25, 64, "4x4x4/stream/S16", (Split<4, 4, 4>), 16, TForm::Stream) \
+  X(26, 128, "8x4x4/stream/S16", (Split<8, 4, 4>), 16, TForm::Stream) \
+  X(27, 128, "16x8/stream/S32", (Split<16, 8>), 32, TForm::Stream) \
+  X(28, 256, "8x8x4/stream/S8", (Split<8, 8, 4>), 8, TForm::Stream) \
+  X(29, 256, "16x16/stream/S16", (Split<16, 16>), 16, TForm::Stream) \
+  X(30, 512, "8x8x8/stream/S4", (Split<8, 8, 8>), 4, TForm::Stream) \
+  X(31, 1024, "16x16x4/stream/S4", (Split<16, 16, 4>), 4, TForm::Stream) \
+  X(32, 1024, "8x8x16/stream/S4", (Split<8, 8, 16>), 4, TForm::Stream) \
+  X(33, 2048, "16x16x8/stream/S2", (Split<16, 16, 8>), 2, TForm::Stream) \
+  X(34, 2048, "8x16x16/stream/S2", (Split<8, 16, 16>), 2, TForm::Stream) \
+  X(35, 4096, "16x16x16/stream/S1", (Split<16, 16, 16>), 1, TForm::Stream)
 
 #define FFTOC_UNPAREN(...) __VA_ARGS__
 
@@ -93,31 +110,91 @@ using global_ptr = __attribute__((address_space(1))) T *;
 template <class T>
 using lds_ptr = __attribute__((address_space(3))) T *;
 
-template <class K, class CT>
-__device__ __noinline__ void tile_call(global_ptr<const CT> in, global_ptr<CT> out, int64_t batch,
-                                       int64_t tile, lds_ptr<CT> lds, global_ptr<const CT> tw,
-                                       real_t<CT> scale, bool inv) {
-  K::run_tile((const CT *)in, (CT *)out, batch, tile, (CT *)lds, (const CT *)tw, scale, inv);
+// Where a workgroup stands in the item sequence: the item it runs (the same
+// in every thread) and, in thread 0, the item it has claimed ahead.
+struct Cursor {
+  int64_t item;
+  long long claim;
+};
+
+// Thread 0's claimed item to every thread (two barriers: the slot is read by
+// all before it is written again).
+__device__ __forceinline__ int64_t publish_claim(long long claim, lds_ptr<long long> s_item) {
+  if (threadIdx.x == 0) *s_item = claim;
+  __syncthreads();
+  const int64_t item = (int64_t)uniform_u64((uint64_t)*s_item);
+  __syncthreads();
+  return item;
+}
+
+// One routine call streams tiles: those of item cur.item of its job (items
+// [item_begin, item_end) of tpi tiles each, relative tile (item - item_begin)
+// * tpi on).  With a claim counter, at the end of an item the claimed item is
+// published and the next one claimed; while the published item belongs to the
+// same job the stream runs on into it, so the next tile's loads overlap this
+// tile's passes across item boundaries too.  Returns the cursor at the first
+// item outside the job.  Without a counter (ctr null) the stream ends with
+// the item.
+template <class K, TForm F, class CT>
+__device__ __noinline__ Cursor tile_stream(global_ptr<const CT> in, global_ptr<CT> out, int64_t batch,
+                                           Cursor cur, int64_t item_begin, int64_t item_end,
+                                           int64_t total, int64_t ntiles, int tpi,
+                                           global_ptr<unsigned long long> ctr,
+                                           lds_ptr<long long> s_item, lds_ptr<CT> lds,
+                                           global_ptr<const CT> tw, real_t<CT> scale, bool inv) {
+  // the stream's bounds drive its barriers: made uniform
+  int64_t item = (int64_t)uniform_u64((uint64_t)cur.item);
+  item_begin = (int64_t)uniform_u64((uint64_t)item_begin);
+  item_end = (int64_t)uniform_u64((uint64_t)item_end);
+  total = (int64_t)uniform_u64((uint64_t)total);
+  ntiles = (int64_t)uniform_u64((uint64_t)ntiles);
+  tpi = __builtin_amdgcn_readfirstlane(tpi);
+  const bool claims = __builtin_amdgcn_readfirstlane(ctr != nullptr ? 1 : 0) != 0;
+  long long claim = cur.claim;
+  int64_t t = (item - item_begin) * tpi;
+  int64_t t_end = (t + tpi < ntiles) ? t + tpi : ntiles;
+  auto next = [&]() -> int64_t {
+    if (++t < t_end) return t;
+    if (!claims) return -1;
+    item = publish_claim(claim, s_item);
+    if (threadIdx.x == 0 && item < total)  // a global (not flat) atomic: no LDS wait waits for it
+      claim = (long long)__hip_atomic_fetch_add(ctr, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (item >= item_end) return -1;
+    t = (item - item_begin) * tpi;
+    t_end = (t + tpi < ntiles) ? t + tpi : ntiles;
+    return t;
+  };
+  if constexpr (streams(F)) {
+    K::run_stream((const CT *)in, (CT *)out, batch, t, (CT *)lds, (const CT *)tw, scale, inv, next);
+  } else {
+    for (int64_t tile = t; tile >= 0; tile = next()) {
+      K::run_tile((const CT *)in, (CT *)out, batch, tile, (CT *)lds, (const CT *)tw, scale, inv);
+      __syncthreads();  // LDS is reused by the next tile
+    }
+  }
+  return Cursor{item, claim};
 }
 
 template <class CT>
-__device__ __forceinline__ void run_routine(int id, const JobDev &J, int64_t tile, CT *lds,
-                                            const CT *__restrict__ tw) {
+__device__ __forceinline__ Cursor run_routine(int id, const JobDev &J, Cursor cur, int64_t item_begin,
+                                              int64_t item_end, int64_t total,
+                                              unsigned long long *ctr, long long *s_item, CT *lds,
+                                              const CT *__restrict__ tw) {
   const CT *in = static_cast<const CT *>(J.in);
   CT *out = static_cast<CT *>(J.out);
   const real_t<CT> scale = (real_t<CT>)J.scale;
   const bool inv = J.inverse != 0;
   switch (id) {
-#define FFTOC_CASE(ID, NN, NAME, SPLIT, S, FORM)                                            \
-  case ID:                                                                                \
-    tile_call<typename RoutineKernel<FFTOC_UNPAREN SPLIT, S, FORM, CT>::type, CT>(          \
-        (global_ptr<const CT>)in, (global_ptr<CT>)out, J.batch, tile, (lds_ptr<CT>)lds,     \
-        (global_ptr<const CT>)tw, scale, inv);                                               \
-    break;
+#define FFTOC_CASE(ID, NN, NAME, SPLIT, S, FORM)                                                  \
+  case ID:                                                                                      \
+    return tile_stream<typename RoutineKernel<FFTOC_UNPAREN SPLIT, S, FORM, CT>::type, FORM, CT>( \
+        (global_ptr<const CT>)in, (global_ptr<CT>)out, J.batch, cur, item_begin, item_end, total,  \
+        J.ntiles, J.tiles_per_item, (global_ptr<unsigned long long>)ctr,                           \
+        (lds_ptr<long long>)s_item, (lds_ptr<CT>)lds, (global_ptr<const CT>)tw, scale, inv);
     FFTOC_ROUTINES(FFTOC_CASE)
 #undef FFTOC_CASE
     default:
-      break;
+      return Cursor{item_end, cur.claim};
   }
 }
 
@@ -128,39 +205,34 @@ __global__ __launch_bounds__(kBatchThreads) __attribute__((amdgpu_waves_per_eu(2
                                                               const f2 *__restrict__ tw32) {
   __shared__ c2 lds[kLdsElems];
   __shared__ long long s_item;
+  // Thread 0 claims one item ahead of the one running, so the claim's round
+  // trip to the counter is hidden behind the tile work.
+  Cursor cur{0, 0};
+  if (threadIdx.x == 0) cur.claim = (long long)atomicAdd(&ctr[0], 1ull);
+  cur.item = publish_claim(cur.claim, (lds_ptr<long long>)&s_item);
+  if (threadIdx.x == 0 && cur.item < total_items) cur.claim = (long long)atomicAdd(&ctr[0], 1ull);
   int jcur = 0;
-  // Thread 0 claims the next item while the current one runs, so the claim's
-  // round trip to the counter is hidden behind the tile work.
-  long long next = 0;
-  if (threadIdx.x == 0) next = (long long)atomicAdd(&ctr[0], 1ull);
   for (;;) {
-    if (threadIdx.x == 0) s_item = next;
-    __syncthreads();
     // readfirstlane: the value is the same in every lane, and the compiler
     // then knows it, so the loop exit below is a uniform branch (a loop it
     // took to exit per lane would run this loop's barriers on split lanes)
-    const int64_t item = (int64_t)uniform_u64((uint64_t)s_item);
-    __syncthreads();  // s_item is read by all before thread 0 writes it again
+    const int64_t item = (int64_t)uniform_u64((uint64_t)cur.item);
     if (item >= total_items) break;
     while (jcur + 1 < njobs && jobs[jcur + 1].item_begin <= item) ++jcur;
     const JobDev &J = jobs[jcur];
-    // the tile loop holds barriers: its bounds and the routine made uniform
+    const int64_t ib = (int64_t)uniform_u64((uint64_t)J.item_begin);
     const int64_t tpi = __builtin_amdgcn_readfirstlane(J.tiles_per_item);
     const int64_t ntiles = (int64_t)uniform_u64((uint64_t)J.ntiles);
-    const int64_t t0 = (item - (int64_t)uniform_u64((uint64_t)J.item_begin)) * tpi;
-    const int64_t t1 = (t0 + tpi < ntiles) ? t0 + tpi : ntiles;
+    const int64_t ie = ib + (ntiles + tpi - 1) / tpi;
     const int routine = __builtin_amdgcn_readfirstlane(J.routine);
     const bool c64 = __builtin_amdgcn_readfirstlane(J.c64) != 0;
-    // issued after the job lookup: a wait for that lookup's loads would also
-    // wait for this older atomic
-    if (threadIdx.x == 0) next = (long long)atomicAdd(&ctr[0], 1ull);
-    for (int64_t t = t0; t < t1; ++t) {
-      if (c64)
-        run_routine<f2>(routine, J, t, reinterpret_cast<f2 *>(lds), tw32);
-      else
-        run_routine<c2>(routine, J, t, lds, tw64);
-      __syncthreads();  // LDS is reused by the next tile
-    }
+    cur.item = item;
+    if (c64)
+      cur = run_routine<f2>(routine, J, cur, ib, ie, total_items, ctr, &s_item,
+                            reinterpret_cast<f2 *>(lds), tw32);
+    else
+      cur = run_routine<c2>(routine, J, cur, ib, ie, total_items, ctr, &s_item, lds, tw64);
+    __syncthreads();  // LDS is reused by the next routine
   }
   // The last workgroup out resets the counters for the next launch / replay.
   if (threadIdx.x == 0) {
@@ -425,22 +497,20 @@ void serve_kernel(ServerDev sd) {
       continue;
     }
     const JobDev J = s_job;
-    // the tile loop's bounds and the routine, read from LDS, made uniform:
-    // the loop holds barriers, so its branches must be the same in every lane
+    // the item and the routine, read from LDS, made uniform: the tile
+    // stream holds barriers, so its branches must be the same in every lane
     const int64_t item = (int64_t)uniform_u64((uint64_t)s_item);
-    const int64_t tpi = __builtin_amdgcn_readfirstlane(J.tiles_per_item);
-    const int64_t ntiles = (int64_t)uniform_u64((uint64_t)J.ntiles);
     const int routine = __builtin_amdgcn_readfirstlane(J.routine);
     const bool c64 = __builtin_amdgcn_readfirstlane(J.c64) != 0;
-    const int64_t t0 = item * tpi;
-    const int64_t t1 = (t0 + tpi < ntiles) ? t0 + tpi : ntiles;
-    for (int64_t t = t0; t < t1; ++t) {
-      if (c64)
-        run_routine<f2>(routine, J, t, reinterpret_cast<f2 *>(lds), static_cast<const f2 *>(sd.tw32));
-      else
-        run_routine<c2>(routine, J, t, lds, static_cast<const c2 *>(sd.tw64));
-      __syncthreads();
-    }
+    // the one item (tiles item * tiles_per_item on, request-relative), pipelined
+    const Cursor one{item, 0};
+    if (c64)
+      run_routine<f2>(routine, J, one, 0, item + 1, item + 1, nullptr, &s_item,
+                      reinterpret_cast<f2 *>(lds), static_cast<const f2 *>(sd.tw32));
+    else
+      run_routine<c2>(routine, J, one, 0, item + 1, item + 1, nullptr, &s_item, lds,
+                      static_cast<const c2 *>(sd.tw64));
+    __syncthreads();
     if (threadIdx.x == 0) {
       const uint64_t r = s_req;
       ++pending;
@@ -528,13 +598,14 @@ int launch_server(const ServerDev &sd, int grid, hipStream_t stream) {
 
 int batch_routine_count() { return kNumRoutines; }
 int batch_routine_default(int n, bool c64) {
-  // complex64 N=128: more lanes per signal read longer runs of 8-byte values
-  if (c64 && n == 128) return 19;
-  if (n == 8) return 20;   // lane-shuffle tiles: no LDS round trips
-  if (n == 16) return 21;
-  if (n == 32) return 22;
-  for (int id = 0; id < kNumRoutines; ++id)
-    if (kRoutines[id].n == n) return id;
+  // Untuned choice per length (index log2 n - 3) and data type, from single
+  // 1 GiB jobs on MI355X: lane-shuffle tiles up to N = 32, pipelined streams
+  // above; complex128 at N = 1024 and 2048 runs the plain tile loop (its
+  // pipelined form needs more registers than a wave has and spills).
+  static constexpr int kC128[10] = {20, 21, 22, 25, 26, 28, 30, 15, 16, 35};
+  static constexpr int kC64[10] = {20, 21, 22, 25, 26, 29, 30, 32, 33, 35};
+  for (int k = 0; k < 10; ++k)
+    if (n == (8 << k)) return c64 ? kC64[k] : kC128[k];
   return -1;
 }
 int batch_routine_n(int id) { return (id >= 0 && id < kNumRoutines) ? kRoutines[id].n : 0; }

@@ -152,16 +152,33 @@ struct LaneTile {
   static constexpr int S = THREADS * C / N;
   static_assert(S * N == THREADS * C, "whole signals per wave");
 
-  __device__ static void run_tile(const CT *in, CT *out, int64_t batch, int64_t tile, CT *,
-                                  const CT *__restrict__ tw, real_t<CT> scale, bool inv = false) {
-    const bool sw = INVM == 0 ? false : INVM == 1 ? true : inv;
-    const int lane = threadIdx.x & 63;
-    const int64_t total = batch * N;
-    const int64_t base = tile * (S * N) + (int64_t)(threadIdx.x >> 6) * (C * 64);
-    const LaneFft<N, CT> f(tw, lane);
-    const int p = lane & (N - 1);
-    const int to = bit_reverse<N>(p) - p;  // the group's output index minus its input index
-    CT v[C];
+  // Prefetch form: plain loads (past the end: the last element), fixed up
+  // where the values are used, so no use is tied to the loads; issued whether
+  // or not there is a next tile (`live` false: all from the first element),
+  // so the loop's vector-memory operations are the same on every pass.
+  __device__ __forceinline__ static void load_raw(CT (&v)[C], const CT *in, int64_t total,
+                                                  int64_t base, int lane, bool live = true) {
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      const int64_t e = base + c * 64 + lane;
+      v[c] = ld_stream(live ? in + (e < total ? e : total - 1) : in);
+    }
+  }
+  __device__ __forceinline__ static void prep(CT (&v)[C], int64_t total, int64_t base, int lane,
+                                              bool sw) {
+#pragma unroll
+    for (int c = 0; c < C; ++c) pin_vgpr(v[c]);
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      if (base + c * 64 + lane >= total) v[c] = mk<CT>(0, 0);
+      if (sw) v[c] = cswap(v[c]);
+    }
+  }
+  __device__ static int64_t base_of(int64_t tile) {
+    return tile * (S * N) + (int64_t)(threadIdx.x >> 6) * (C * 64);
+  }
+  __device__ static void load(CT (&v)[C], const CT *in, int64_t total, int64_t base, int lane,
+                              bool sw) {
 #pragma unroll
     for (int c = 0; c < C; ++c) {
       // past the end: read the last element (no branch) and use zero
@@ -170,8 +187,9 @@ struct LaneTile {
       if (e >= total) v[c] = mk<CT>(0, 0);
       if (sw) v[c] = cswap(v[c]);
     }
-#pragma unroll
-    for (int c = 0; c < C; ++c) v[c] = f(v[c]);
+  }
+  __device__ static void store(const CT (&v)[C], CT *out, int64_t total, int64_t base, int lane,
+                               int to, real_t<CT> scale, bool sw) {
 #pragma unroll
     for (int c = 0; c < C; ++c) {
       const int64_t e = base + c * 64 + lane;  // a whole N-lane group is valid or not
@@ -179,6 +197,56 @@ struct LaneTile {
       if (scale != (real_t<CT>)1) r = cscale(r, scale);
       if (sw) r = cswap(r);
       if (e < total) st_stream(out + e + to, r);
+    }
+  }
+
+  __device__ static void run_tile(const CT *in, CT *out, int64_t batch, int64_t tile, CT *,
+                                  const CT *__restrict__ tw, real_t<CT> scale, bool inv = false) {
+    const bool sw = INVM == 0 ? false : INVM == 1 ? true : inv;
+    const int lane = threadIdx.x & 63;
+    const int64_t total = batch * N;
+    const int64_t base = base_of(tile);
+    const LaneFft<N, CT> f(tw, lane);
+    const int p = lane & (N - 1);
+    const int to = bit_reverse<N>(p) - p;  // the group's output index minus its input index
+    CT v[C];
+    load(v, in, total, base, lane, sw);
+#pragma unroll
+    for (int c = 0; c < C; ++c) v[c] = f(v[c]);
+    store(v, out, total, base, lane, to, scale, sw);
+  }
+
+  // Pipelined stream of tiles (the interface of StockhamKernel::run_stream):
+  // `tile`, then the tiles next() returns until it returns a negative index;
+  // the next tile's loads are in flight during this tile's butterflies and
+  // stores.
+  template <class Next>
+  __device__ __forceinline__ static void run_stream(const CT *in, CT *out, int64_t batch,
+                                                    int64_t tile, CT *, const CT *__restrict__ tw,
+                                                    real_t<CT> scale, bool inv, Next &&next) {
+    const bool sw = INVM == 0 ? false : INVM == 1 ? true : inv;
+    const int lane = threadIdx.x & 63;
+    const int64_t total = batch * N;
+    const LaneFft<N, CT> f(tw, lane);
+    const int p = lane & (N - 1);
+    const int to = bit_reverse<N>(p) - p;
+    // two buffers in turn: a copy of the prefetched values would wait for them
+    CT a[C], b[C];
+    load_raw(a, in, total, base_of(tile), lane);
+    int64_t nt = next();
+    auto step = [&](CT (&cur)[C], CT (&pre)[C]) -> bool {
+      const bool more = nt >= 0;  // uniform over the workgroup
+      load_raw(pre, in, total, base_of(more ? nt : 0), lane, more);
+      prep(cur, total, base_of(tile), lane, sw);
+#pragma unroll
+      for (int c = 0; c < C; ++c) cur[c] = f(cur[c]);
+      const int64_t nnt = more ? next() : -1;
+      store(cur, out, total, base_of(tile), lane, to, scale, sw);
+      tile = nt;
+      nt = nnt;
+      return more;
+    };
+    while (step(a, b) && step(b, a)) {
     }
   }
 };

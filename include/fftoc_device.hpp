@@ -256,6 +256,14 @@ __device__ __forceinline__ void st_stream(CT *p, CT v) {
   __builtin_nontemporal_store(t, reinterpret_cast<vec2 *>(p));
 }
 
+// Marks where a prefetched value is first used: the compiler may not move a
+// use above this point (it would wait for the load there), and the value stays
+// in its registers until here.
+template <class CT>
+__device__ __forceinline__ void pin_vgpr(CT &v) {
+  asm volatile("" : "+v"(v.x), "+v"(v.y));
+}
+
 // A wave-uniform 64-bit value in scalar registers.  readfirstlane returns a
 // signed 32-bit int: each half goes through uint32_t, or the low half would
 // sign-extend into the high one when its bit 31 is set.

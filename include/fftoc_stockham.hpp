@@ -68,6 +68,35 @@ struct StockhamKernel {
     }
   }
 
+  // Pass-0 inputs of a prefetch: plain loads from a signal that exists (the
+  // caller clamps it), so no use of the values is tied to the loads; prep()
+  // then zeroes a signal past the end and applies the direction, where the
+  // values are used.  The loads are issued whether or not there is a next
+  // tile (`live` false: all from `dummy`, one element that exists), so every
+  // pass through the loop issues the same vector-memory operations and the
+  // compiler's wait counts stay exact.
+  __device__ __forceinline__ static void load_raw(CT (&reg)[RM], const CT *gin, int t, bool live,
+                                                  const CT *dummy) {
+    constexpr int R = SP::R[0];
+    constexpr int NB = RM / R;
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+#pragma unroll
+      for (int m = 0; m < R; ++m)
+        reg[i * R + m] = ld_stream(live ? gin + (t + i * T + m * (N / R)) : dummy);
+    }
+  }
+  __device__ __forceinline__ static void prep(CT (&reg)[RM], bool valid, bool inv) {
+#pragma unroll
+    for (int i = 0; i < RM; ++i) pin_vgpr(reg[i]);
+#pragma unroll
+    for (int i = 0; i < RM; ++i) {
+      CT v = valid ? reg[i] : mk<CT>(0, 0);
+      if (swap_io(inv)) v = cswap(v);
+      reg[i] = v;
+    }
+  }
+
   // Pass-0 inputs straight from HBM (non-staged kernels).
   __device__ static void load_first(CT (&reg)[RM], const CT *gin, int t, bool valid, bool inv) {
     constexpr int R = SP::R[0];
@@ -307,7 +336,8 @@ struct StockhamKernel {
     }
     sync();
   }
-  __device__ __forceinline__ static void scatter_out(const CT (&reg)[RM], CT *gout, int t, bool valid, R_ scale) {
+  __device__ __forceinline__ static void scatter_out(const CT (&reg)[RM], CT *gout, int t, bool valid, R_ scale,
+                                                     bool inv = false) {
     constexpr int p = SP::P - 1;
     constexpr int R = SP::R[p];
     constexpr int L = SP::L(p);
@@ -320,7 +350,7 @@ struct StockhamKernel {
       for (int q = 0; q < R; ++q) {
         CT v = reg[i * R + q];
         if (scale != 1.0) v = cscale(v, scale);
-        if (swap_io(false)) v = cswap(v);
+        if (swap_io(inv)) v = cswap(v);
         if (valid) st_stream(gout + (j - k) * R + k + q * L, v);
       }
     }
@@ -333,54 +363,73 @@ struct StockhamKernel {
      ...);
   }
 
-  // Persistent, pipelined: tiles blockIdx.x, blockIdx.x + gridDim.x, ...  The
-  // base twiddles are read once, before the loop, so the passes after the
-  // first issue no vector-memory loads: the next tile's loads, issued right
-  // after the current tile's first pass, then overlap all of its later passes
-  // (the vector-memory counter retires in order, so any load issued after
-  // them -- a twiddle read -- would wait for them).  The next tile's first
-  // pass runs before the current tile's stores are issued, so its wait for
-  // those loads does not wait for these stores either.
-  __device__ __forceinline__ static void run_pipelined_persistent(const CT *in, CT *out, int64_t batch,
-                                                  const CT *__restrict__ tw, R_ scale) {
-    static_assert(!STAGED && SP::P >= 2 && INVM != 2, "pipelined form needs an LDS pass");
-    __shared__ CT lds[lds_elems(S * N)];
+  // Pipelined stream of tiles: `tile` first, then the tiles next() returns
+  // until it returns a negative index (next() is called by every thread at
+  // the same point, once per tile, and returns the same value in all; it may
+  // hold workgroup barriers).  The base twiddles are read once, before the
+  // loop, so the passes after the first issue no vector-memory loads: the
+  // next tile's loads, issued right after the current tile's first pass, then
+  // overlap all of its later passes (the vector-memory counter retires in
+  // order, so any load issued after them -- a twiddle read -- would wait for
+  // them).  The next tile's first pass runs before the current tile's stores
+  // are issued, so its wait for those loads does not wait for these stores;
+  // next() for the tile after it is called there too, when no load is in
+  // flight.  `lds` holds lds_elems(S * N) elements.
+  template <class Next>
+  __device__ __forceinline__ static void run_stream(const CT *in, CT *out, int64_t batch, int64_t tile,
+                                                    CT *lds, const CT *__restrict__ tw, R_ scale,
+                                                    bool inv, Next &&next) {
+    static_assert(!STAGED && SP::P >= 2, "pipelined form needs an LDS pass");
     const int tid = threadIdx.x;
     const int t = tid % T;
     const int sl = tid / T;
     const int sbase = sl * N;
-    const int64_t ntiles = (batch + S - 1) / S;
-    int64_t tile = blockIdx.x;
-    if (tile >= ntiles) return;  // uniform over the workgroup
     CT hw[kTwCount];
     fetch_all_tw(std::make_integer_sequence<int, SP::P - 1>{}, hw, tw, t);
     CT reg[RM];
     {
       const int64_t sig = tile * S + sl;
-      load_first(reg, in + sig * N, t, sig < batch, false);
+      load_raw(reg, in + (sig < batch ? sig : batch - 1) * N, t, true, in);
+      prep(reg, sig < batch, inv);
     }
     butterflies<0>(reg, hw);
     scatter_lds<0>(reg, lds, t, sbase);
+    int64_t nt = next();
     for (;;) {
       const int64_t sig = tile * S + sl;
-      const int64_t next = tile + gridDim.x;
-      const bool more = next < ntiles;  // uniform over the workgroup
+      const bool more = nt >= 0;  // uniform over the workgroup
+      const int64_t nsig = nt * S + sl;
       CT nxt[RM];
-      if (more) {
-        const int64_t nsig = next * S + sl;
-        load_first(nxt, in + nsig * N, t, nsig < batch, false);
-      }
+      load_raw(nxt, in + (nsig < batch ? nsig : batch - 1) * N, t, more, in);
       middle_passes(std::make_integer_sequence<int, SP::P - 2>{}, reg, lds, t, sbase, hw);
       gather<SP::P - 1>(reg, lds, t, sbase);  // LDS free for the next tile after this
       butterflies<SP::P - 1>(reg, hw);
+      int64_t nnt = -1;
       if (more) {
+        prep(nxt, nsig < batch, inv);
         butterflies<0>(nxt, hw);
         scatter_lds<0>(nxt, lds, t, sbase);
+        nnt = next();
       }
-      scatter_out(reg, out + sig * N, t, sig < batch, scale);
+      scatter_out(reg, out + sig * N, t, sig < batch, scale, inv);
       if (!more) break;
-      tile = next;
+      tile = nt;
+      nt = nnt;
     }
+  }
+
+  // Persistent, pipelined: tiles blockIdx.x, blockIdx.x + gridDim.x, ...
+  __device__ __forceinline__ static void run_pipelined_persistent(const CT *in, CT *out, int64_t batch,
+                                                  const CT *__restrict__ tw, R_ scale) {
+    static_assert(INVM != 2, "direction fixed at compile time");
+    __shared__ CT lds[lds_elems(S * N)];
+    const int64_t ntiles = (batch + S - 1) / S;
+    int64_t tile = blockIdx.x;
+    if (tile >= ntiles) return;  // uniform over the workgroup
+    run_stream(in, out, batch, tile, lds, tw, scale, false, [&]() -> int64_t {
+      tile += gridDim.x;
+      return tile < ntiles ? tile : -1;
+    });
   }
 
   // Persistent workgroups: tiles blockIdx.x, blockIdx.x + gridDim.x, ...;
