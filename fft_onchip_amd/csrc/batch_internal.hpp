@@ -98,5 +98,10 @@ int batch_routine_signals(int id);
 int batch_resident_blocks();
 int launch_batch(const JobDev *d_jobs, int njobs, int64_t total_items, unsigned long long *d_ctr,
                  const void *tw64, const void *tw32, int grid, hipStream_t stream);
+// batch_one.hip: the kernel specialized to one routine and data type (every
+// job of the launch runs that routine on that data type)
+int64_t batch_one_resident(int routine, bool c64);
+int launch_batch_one(int routine, bool c64, const JobDev *d_jobs, int njobs, int64_t total_items,
+                     unsigned long long *d_ctr, const void *tw, int grid, hipStream_t stream);
 
 }  // namespace fftoc

@@ -9,194 +9,10 @@
 // and one LDS allocation serve every length.  The last workgroup to finish resets
 // the counters, so the launch can be replayed from a hipGraph without any
 // memset node.
-#include "batch_internal.hpp"
-#include "lane_fft.hpp"
-#include "../../include/fftoc_stockham.hpp"
+#include "batch_routines.hpp"
 
 namespace fftoc {
 namespace {
-
-constexpr int kBatchThreads = 256;
-constexpr int kLdsElems = lds_elems(4096);
-
-// Tile forms: a Stockham tile read from HBM directly (Tile) or staged
-// through LDS (Staged), one tile after another; the Stockham tile pipelined
-// over the routine's tiles (Stream: the next tile's loads in flight during
-// this tile's passes); the lane-shuffle tile, pipelined (Lanes, N = 8 ... 64).
-enum class TForm { Tile, Staged, Stream, Lanes };
-
-template <class SP, int S, TForm F, class CT>
-struct RoutineKernel {
-  using type = StockhamKernel<SP, S, 2, F == TForm::Staged, CT>;
-};
-// which kernels a routine streams with run_stream (the others run run_tile
-// per tile)
-constexpr bool streams(TForm f) { return f == TForm::Stream || f == TForm::Lanes; }
-template <class SP, int S, class CT>
-struct RoutineKernel<SP, S, TForm::Lanes, CT> {
-  using type = LaneTile<SP::N, S * SP::N / kBatchThreads, 2, CT>;
-};
-
-// Routine table: id -> (n, split, signals per tile, form).  Two or three
-// candidates per length; the plan autotuner picks among them, and
-// batch_routine_default gives the untuned choice per data type.
-#define FFTOC_ROUTINES(X)                              \
-  X(0, 8, "4x2/staged/S128", (Split<4, 2>), 128, TForm::Staged)  \
-  X(1, 8, "8/staged/S256", (Split<8>), 256, TForm::Staged)        \
-  X(2, 16, "4x4/staged/S64", (Split<4, 4>), 64, TForm::Staged)    \
-  X(3, 16, "16/staged/S256", (Split<16>), 256, TForm::Staged)     \
-  X(4, 32, "8x4/staged/S64", (Split<8, 4>), 64, TForm::Staged)    \
-  X(5, 32, "4x4x2/staged/S32", (Split<4, 4, 2>), 32, TForm::Staged) \
-  X(6, 64, "8x8/tile/S32", (Split<8, 8>), 32, TForm::Tile)     \
-  X(7, 64, "4x4x4/tile/S16", (Split<4, 4, 4>), 16, TForm::Tile) \
-  X(8, 128, "16x8/tile/S32", (Split<16, 8>), 32, TForm::Tile)  \
-  X(9, 128, "8x16/tile/S32", (Split<8, 16>), 32, TForm::Tile)  \
-  X(10, 256, "16x16/tile/S16", (Split<16, 16>), 16, TForm::Tile) \
-  X(11, 256, "8x8x4/tile/S8", (Split<8, 8, 4>), 8, TForm::Tile) \
-  X(12, 512, "8x8x8/tile/S4", (Split<8, 8, 8>), 4, TForm::Tile) \
-  X(13, 512, "16x16x2/tile/S8", (Split<16, 16, 2>), 8, TForm::Tile) \
-  X(14, 1024, "16x16x4/tile/S4", (Split<16, 16, 4>), 4, TForm::Tile) \
-  X(15, 1024, "8x8x16/tile/S4", (Split<8, 8, 16>), 4, TForm::Tile) \
-  X(16, 2048, "16x16x8/tile/S2", (Split<16, 16, 8>), 2, TForm::Tile) \
-  X(17, 2048, "8x16x16/tile/S2", (Split<8, 16, 16>), 2, TForm::Tile) \
-  X(18, 4096, "16x16x16/tile/S1", (Split<16, 16, 16>), 1, TForm::Tile) \
-  X(19, 128, "8x4x4/tile/S16", (Split<8, 4, 4>), 16, TForm::Tile) \
-  X(20, 8, "8/lanes/S128", (Split<8>), 128, TForm::Lanes) \
-  X(21, 16, "16/lanes/S64", (Split<16>), 64, TForm::Lanes) \
-  X(22, 32, "32/lanes/S32", (Split<32>), 32, TForm::Lanes) \
-  X(23, 64, "64/lanes/S16", (Split<64>), 16, TForm::Lanes) \
-  X(24, 64, "8x8/stream/S32", (Split<8, 8>), 32, TForm::Stream) \
-  X(25, 64, "4x4x4/stream/S16", (Split<4, 4, 4>), 16, TForm::Stream) \
-  X(26, 128, "8x4x4/stream/S16", (Split<8, 4, 4>), 16, TForm::Stream) \
-  X(27, 128, "16x8/stream/S32", (Split<16, 8>), 32, TForm::Stream) \
-  X(28, 256, "8x8x4/stream/S8", (Split<8, 8, 4>), 8, TForm::Stream) \
-  X(29, 256, "16x16/stream/S16", (Split<16, 16>), 16, TForm::Stream) \
-  X(30, 512, "8x8x8/stream/S4", (Split<8, 8, 8>), 4, TForm::Stream) \
-  X(31, 1024, "16x16x4/stream/S4", (Split<16, 16, 4>), 4, TForm::Stream) \
-  X(32, 1024, "8x8x16/stream/S4", (Split<8, 8, 16>), 4, TForm::Stream) \
-  X(33, 2048, "16x16x8/stream/S2", (Split<16, 16, 8>), 2, TForm::Stream) \
-  X(34, 2048, "8x16x16/stream/S2", (Split<8, 16, 16>), 2, TForm::Stream) \
-  X(35, 4096, "16x16x16/stream/S1", (Split<16, 16, 16>), 1, TForm::Stream)
-
-#define FFTOC_UNPAREN(...) __VA_ARGS__
-
-#define FFTOC_CHECK_THREADS(ID, NN, NAME, SPLIT, S, FORM)                                     \
-  static_assert(RoutineKernel<FFTOC_UNPAREN SPLIT, S, FORM, c2>::type::THREADS == kBatchThreads, \
-                "tile routine " #ID " must use 256 threads");                                 \
-  static_assert(FFTOC_UNPAREN SPLIT::N == NN, "tile routine " #ID " length");                 \
-  static_assert(S * NN <= 4096, "tile routine " #ID " LDS");
-FFTOC_ROUTINES(FFTOC_CHECK_THREADS)
-#undef FFTOC_CHECK_THREADS
-
-struct RoutineInfo {
-  int n;
-  const char *name;
-  int signals_per_tile;
-};
-
-#define FFTOC_INFO(ID, NN, NAME, SPLIT, S, FORM) {NN, NAME, S},
-constexpr RoutineInfo kRoutines[] = {FFTOC_ROUTINES(FFTOC_INFO)};
-#undef FFTOC_INFO
-constexpr int kNumRoutines = sizeof(kRoutines) / sizeof(kRoutines[0]);
-
-// Each tile routine is a separate (not inlined) function so that its
-// registers are allocated on their own; the kernel needs the maximum over the
-// routines, not their union.  Its pointer parameters carry their address
-// spaces (global, LDS): through generic pointers the routine would access HBM
-// with flat instructions, whose LDS counter makes every LDS wait also wait for
-// outstanding HBM traffic.
-template <class T>
-using global_ptr = __attribute__((address_space(1))) T *;
-template <class T>
-using lds_ptr = __attribute__((address_space(3))) T *;
-
-// Where a workgroup stands in the item sequence: the item it runs (the same
-// in every thread) and, in thread 0, the item it has claimed ahead.
-struct Cursor {
-  int64_t item;
-  long long claim;
-};
-
-// Thread 0's claimed item to every thread (two barriers: the slot is read by
-// all before it is written again).
-__device__ __forceinline__ int64_t publish_claim(long long claim, lds_ptr<long long> s_item) {
-  if (threadIdx.x == 0) *s_item = claim;
-  __syncthreads();
-  const int64_t item = (int64_t)uniform_u64((uint64_t)*s_item);
-  __syncthreads();
-  return item;
-}
-
-// One routine call streams tiles: those of item cur.item of its job (items
-// [item_begin, item_end) of tpi tiles each, relative tile (item - item_begin)
-// * tpi on).  With a claim counter, at the end of an item the claimed item is
-// published and the next one claimed; while the published item belongs to the
-// same job the stream runs on into it, so the next tile's loads overlap this
-// tile's passes across item boundaries too.  Returns the cursor at the first
-// item outside the job.  Without a counter (ctr null) the stream ends with
-// the item.
-template <class K, TForm F, class CT>
-__device__ __noinline__ Cursor tile_stream(global_ptr<const CT> in, global_ptr<CT> out, int64_t batch,
-                                           Cursor cur, int64_t item_begin, int64_t item_end,
-                                           int64_t total, int64_t ntiles, int tpi,
-                                           global_ptr<unsigned long long> ctr,
-                                           lds_ptr<long long> s_item, lds_ptr<CT> lds,
-                                           global_ptr<const CT> tw, real_t<CT> scale, bool inv) {
-  // the stream's bounds drive its barriers: made uniform
-  int64_t item = (int64_t)uniform_u64((uint64_t)cur.item);
-  item_begin = (int64_t)uniform_u64((uint64_t)item_begin);
-  item_end = (int64_t)uniform_u64((uint64_t)item_end);
-  total = (int64_t)uniform_u64((uint64_t)total);
-  ntiles = (int64_t)uniform_u64((uint64_t)ntiles);
-  tpi = __builtin_amdgcn_readfirstlane(tpi);
-  const bool claims = __builtin_amdgcn_readfirstlane(ctr != nullptr ? 1 : 0) != 0;
-  long long claim = cur.claim;
-  int64_t t = (item - item_begin) * tpi;
-  int64_t t_end = (t + tpi < ntiles) ? t + tpi : ntiles;
-  auto next = [&]() -> int64_t {
-    if (++t < t_end) return t;
-    if (!claims) return -1;
-    item = publish_claim(claim, s_item);
-    if (threadIdx.x == 0 && item < total)  // a global (not flat) atomic: no LDS wait waits for it
-      claim = (long long)__hip_atomic_fetch_add(ctr, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (item >= item_end) return -1;
-    t = (item - item_begin) * tpi;
-    t_end = (t + tpi < ntiles) ? t + tpi : ntiles;
-    return t;
-  };
-  if constexpr (streams(F)) {
-    K::run_stream((const CT *)in, (CT *)out, batch, t, (CT *)lds, (const CT *)tw, scale, inv, next);
-  } else {
-    for (int64_t tile = t; tile >= 0; tile = next()) {
-      K::run_tile((const CT *)in, (CT *)out, batch, tile, (CT *)lds, (const CT *)tw, scale, inv);
-      __syncthreads();  // LDS is reused by the next tile
-    }
-  }
-  return Cursor{item, claim};
-}
-
-template <class CT>
-__device__ __forceinline__ Cursor run_routine(int id, const JobDev &J, Cursor cur, int64_t item_begin,
-                                              int64_t item_end, int64_t total,
-                                              unsigned long long *ctr, long long *s_item, CT *lds,
-                                              const CT *__restrict__ tw) {
-  const CT *in = static_cast<const CT *>(J.in);
-  CT *out = static_cast<CT *>(J.out);
-  const real_t<CT> scale = (real_t<CT>)J.scale;
-  const bool inv = J.inverse != 0;
-  switch (id) {
-#define FFTOC_CASE(ID, NN, NAME, SPLIT, S, FORM)                                                  \
-  case ID:                                                                                      \
-    return tile_stream<typename RoutineKernel<FFTOC_UNPAREN SPLIT, S, FORM, CT>::type, FORM, CT>( \
-        (global_ptr<const CT>)in, (global_ptr<CT>)out, J.batch, cur, item_begin, item_end, total,  \
-        J.ntiles, J.tiles_per_item, (global_ptr<unsigned long long>)ctr,                           \
-        (lds_ptr<long long>)s_item, (lds_ptr<CT>)lds, (global_ptr<const CT>)tw, scale, inv);
-    FFTOC_ROUTINES(FFTOC_CASE)
-#undef FFTOC_CASE
-    default:
-      return Cursor{item_end, cur.claim};
-  }
-}
 
 __global__ __launch_bounds__(kBatchThreads) __attribute__((amdgpu_waves_per_eu(2))) void batch_kernel(const JobDev *__restrict__ jobs,
                                                               int njobs, int64_t total_items,
@@ -205,12 +21,11 @@ __global__ __launch_bounds__(kBatchThreads) __attribute__((amdgpu_waves_per_eu(2
                                                               const f2 *__restrict__ tw32) {
   __shared__ c2 lds[kLdsElems];
   __shared__ long long s_item;
-  // Thread 0 claims one item ahead of the one running, so the claim's round
-  // trip to the counter is hidden behind the tile work.
-  Cursor cur{0, 0};
-  if (threadIdx.x == 0) cur.claim = (long long)atomicAdd(&ctr[0], 1ull);
-  cur.item = publish_claim(cur.claim, (lds_ptr<long long>)&s_item);
-  if (threadIdx.x == 0 && cur.item < total_items) cur.claim = (long long)atomicAdd(&ctr[0], 1ull);
+  // Workgroup b starts with item b and claims the items after the grid's
+  // first ones from the counter (claim_item), one item ahead of the one
+  // running (claim_ahead): the start waits for no atomic.
+  Cursor cur{(int64_t)blockIdx.x, 0, claim_ahead(blockIdx.x, total_items) ? 1 : 0};
+  if (cur.ahead && threadIdx.x == 0) cur.claim = claim_item((global_ptr<unsigned long long>)ctr);
   int jcur = 0;
   for (;;) {
     // readfirstlane: the value is the same in every lane, and the compiler
@@ -503,7 +318,7 @@ void serve_kernel(ServerDev sd) {
     const int routine = __builtin_amdgcn_readfirstlane(J.routine);
     const bool c64 = __builtin_amdgcn_readfirstlane(J.c64) != 0;
     // the one item (tiles item * tiles_per_item on, request-relative), pipelined
-    const Cursor one{item, 0};
+    const Cursor one{item, 0, 0};
     if (c64)
       run_routine<f2>(routine, J, one, 0, item + 1, item + 1, nullptr, &s_item,
                       reinterpret_cast<f2 *>(lds), static_cast<const f2 *>(sd.tw32));
@@ -599,11 +414,12 @@ int launch_server(const ServerDev &sd, int grid, hipStream_t stream) {
 int batch_routine_count() { return kNumRoutines; }
 int batch_routine_default(int n, bool c64) {
   // Untuned choice per length (index log2 n - 3) and data type, from single
-  // 1 GiB jobs on MI355X: lane-shuffle tiles up to N = 32, pipelined streams
-  // above; complex128 at N = 1024 and 2048 runs the plain tile loop (its
-  // pipelined form needs more registers than a wave has and spills).
-  static constexpr int kC128[10] = {20, 21, 22, 25, 26, 28, 30, 15, 16, 35};
-  static constexpr int kC64[10] = {20, 21, 22, 25, 26, 29, 30, 32, 33, 35};
+  // 1 GiB jobs on MI355X (the kernel specialized to the routine): lane-shuffle
+  // tiles up to N = 32, pipelined streams above (complex128 N = 256: the plain
+  // tile loop), and at N = 4096 complex128 the matrix-core stream with the
+  // two-product contraction.
+  static constexpr int kC128[10] = {20, 21, 22, 25, 26, 10, 30, 32, 34, 37};
+  static constexpr int kC64[10] = {20, 21, 22, 25, 26, 29, 30, 31, 33, 35};
   for (int k = 0; k < 10; ++k)
     if (n == (8 << k)) return c64 ? kC64[k] : kC128[k];
   return -1;

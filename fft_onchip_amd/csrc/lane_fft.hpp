@@ -151,17 +151,20 @@ struct LaneTile {
   static constexpr int THREADS = 256;
   static constexpr int S = THREADS * C / N;
   static_assert(S * N == THREADS * C, "whole signals per wave");
+  static_assert(C * 64 <= kTwN, "prefetch dummies inside the twiddle table");
 
   // Prefetch form: plain loads (past the end: the last element), fixed up
   // where the values are used, so no use is tied to the loads; issued whether
-  // or not there is a next tile (`live` false: all from the first element),
-  // so the loop's vector-memory operations are the same on every pass.
+  // or not there is a next tile (`live` false: from `dummy`, an array of at
+  // least 64 C elements that stays in cache, at distinct offsets), so the
+  // loop's vector-memory operations are the same on every pass.
   __device__ __forceinline__ static void load_raw(CT (&v)[C], const CT *in, int64_t total,
-                                                  int64_t base, int lane, bool live = true) {
+                                                  int64_t base, int lane, bool live,
+                                                  const CT *dummy) {
 #pragma unroll
     for (int c = 0; c < C; ++c) {
       const int64_t e = base + c * 64 + lane;
-      v[c] = ld_stream(live ? in + (e < total ? e : total - 1) : in);
+      v[c] = ld_stream(live ? in + (e < total ? e : total - 1) : dummy + (c * 64 + lane));
     }
   }
   __device__ __forceinline__ static void prep(CT (&v)[C], int64_t total, int64_t base, int lane,
@@ -232,11 +235,11 @@ struct LaneTile {
     const int to = bit_reverse<N>(p) - p;
     // two buffers in turn: a copy of the prefetched values would wait for them
     CT a[C], b[C];
-    load_raw(a, in, total, base_of(tile), lane);
+    load_raw(a, in, total, base_of(tile), lane, true, tw);
     int64_t nt = next();
     auto step = [&](CT (&cur)[C], CT (&pre)[C]) -> bool {
       const bool more = nt >= 0;  // uniform over the workgroup
-      load_raw(pre, in, total, base_of(more ? nt : 0), lane, more);
+      load_raw(pre, in, total, base_of(more ? nt : 0), lane, more, tw);
       prep(cur, total, base_of(tile), lane, sw);
 #pragma unroll
       for (int c = 0; c < C; ++c) cur[c] = f(cur[c]);

@@ -21,6 +21,8 @@ struct fftoc_plan_t {
   unsigned long long *d_ctr = nullptr;  // [0] next item, [1] finished workgroups
   int64_t total_items = 0;
   int grid = 0;
+  int one_routine = -1;                 // the routine of every job, if they share
+  bool one_c64 = false;                 // one (routine, data type); else -1
   int routine[13][2];                   // [log2 n][complex64] -> routine id
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -137,7 +139,20 @@ int layout(fftoc_plan_t *p) {
     if (hipMemcpy(p->d_jobs, p->host.data(), bytes, hipMemcpyHostToDevice) != hipSuccess)
       return FFTOC_ERROR_DEVICE;
   }
-  const int64_t resident = batch_resident_blocks();
+  // jobs sharing one routine and data type run the kernel specialized to it
+  p->one_routine = -1;
+  for (size_t k = 0; k < p->host.size(); ++k) {
+    const JobDev &d = p->host[k];
+    if (k == 0) {
+      p->one_routine = d.routine;
+      p->one_c64 = d.c64 != 0;
+    } else if (d.routine != p->one_routine || (d.c64 != 0) != p->one_c64) {
+      p->one_routine = -1;
+      break;
+    }
+  }
+  const int64_t resident = p->one_routine >= 0 ? batch_one_resident(p->one_routine, p->one_c64)
+                                               : batch_resident_blocks();
   p->grid = (int)std::max<int64_t>(1, std::min<int64_t>(resident, p->total_items));
   if (p->exec) {
     (void)hipGraphExecDestroy(p->exec);
@@ -152,6 +167,9 @@ int layout(fftoc_plan_t *p) {
 
 int launch(fftoc_plan_t *p, hipStream_t s) {
   if (p->total_items == 0) return FFTOC_SUCCESS;
+  if (p->one_routine >= 0)
+    return launch_batch_one(p->one_routine, p->one_c64, p->d_jobs, (int)p->host.size(),
+                            p->total_items, p->d_ctr, p->one_c64 ? p->tw32 : p->tw64, p->grid, s);
   return launch_batch(p->d_jobs, (int)p->host.size(), p->total_items, p->d_ctr, p->tw64, p->tw32,
                       p->grid, s);
 }

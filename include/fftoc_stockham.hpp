@@ -72,18 +72,20 @@ struct StockhamKernel {
   // caller clamps it), so no use of the values is tied to the loads; prep()
   // then zeroes a signal past the end and applies the direction, where the
   // values are used.  The loads are issued whether or not there is a next
-  // tile (`live` false: all from `dummy`, one element that exists), so every
-  // pass through the loop issues the same vector-memory operations and the
-  // compiler's wait counts stay exact.
+  // tile (`live` false: from the same offsets of `dummy`, an array of at
+  // least N elements that stays in cache -- the twiddle table -- rather than
+  // all from one element, which every workgroup's last tile hit at once), so
+  // every pass through the loop issues the same vector-memory operations and
+  // the compiler's wait counts stay exact.
   __device__ __forceinline__ static void load_raw(CT (&reg)[RM], const CT *gin, int t, bool live,
                                                   const CT *dummy) {
     constexpr int R = SP::R[0];
     constexpr int NB = RM / R;
+    const CT *base = live ? gin : dummy;
 #pragma unroll
     for (int i = 0; i < NB; ++i) {
 #pragma unroll
-      for (int m = 0; m < R; ++m)
-        reg[i * R + m] = ld_stream(live ? gin + (t + i * T + m * (N / R)) : dummy);
+      for (int m = 0; m < R; ++m) reg[i * R + m] = ld_stream(base + (t + i * T + m * (N / R)));
     }
   }
   __device__ __forceinline__ static void prep(CT (&reg)[RM], bool valid, bool inv) {
@@ -381,26 +383,30 @@ struct StockhamKernel {
                                                     bool inv, Next &&next) {
     static_assert(!STAGED && SP::P >= 2, "pipelined form needs an LDS pass");
     const int tid = threadIdx.x;
-    const int t = tid % T;
-    const int sl = tid / T;
-    const int sbase = sl * N;
+    const int t0 = tid % T, sl0 = tid / T, sbase0 = sl0 * N;
+    const int t = t0, sl = sl0, sbase = sbase0;
     CT hw[kTwCount];
     fetch_all_tw(std::make_integer_sequence<int, SP::P - 1>{}, hw, tw, t);
     CT reg[RM];
     {
       const int64_t sig = tile * S + sl;
-      load_raw(reg, in + (sig < batch ? sig : batch - 1) * N, t, true, in);
+      load_raw(reg, in + (sig < batch ? sig : batch - 1) * N, t, true, tw);
       prep(reg, sig < batch, inv);
     }
     butterflies<0>(reg, hw);
     scatter_lds<0>(reg, lds, t, sbase);
     int64_t nt = next();
     for (;;) {
+      // the thread's indices redefined on every pass: the compiler then
+      // computes each LDS and HBM address where it is used instead of
+      // hoisting them all out of the loop, where they stayed live and spilled
+      int t = t0, sl = sl0, sbase = sbase0;
+      asm volatile("" : "+v"(t), "+v"(sl), "+v"(sbase));
       const int64_t sig = tile * S + sl;
       const bool more = nt >= 0;  // uniform over the workgroup
       const int64_t nsig = nt * S + sl;
       CT nxt[RM];
-      load_raw(nxt, in + (nsig < batch ? nsig : batch - 1) * N, t, more, in);
+      load_raw(nxt, in + (nsig < batch ? nsig : batch - 1) * N, t, more, tw);
       middle_passes(std::make_integer_sequence<int, SP::P - 2>{}, reg, lds, t, sbase, hw);
       gather<SP::P - 1>(reg, lds, t, sbase);  // LDS free for the next tile after this
       butterflies<SP::P - 1>(reg, hw);
