@@ -315,6 +315,49 @@ def run_batched(fo, steps: int, warmup: int, njobs: int = 64, max_batch: int = 4
             "per_call_GBps": round(nbytes / (ms_plain * 1e-3) / 1e9, 1)}
 
 
+def run_plan_single(fo, steps: int, warmup: int, n: int = 4096, batch: int = 16384, c64: bool = False):
+    """One 1 GiB job (N = 4096 x 16384, complex128) as a plan -- autotuned
+    (fftoc_plan_autotune picks the routine: a plan whose jobs share one
+    routine runs the kernel specialized to it), one launch from a hipGraph --
+    against the same transform as one fftoc_fft call, same buffers."""
+    from fft_onchip_amd import hip
+    es = 8 if c64 else 16
+    flags = fo.DATA_C64 if c64 else 0
+    x = gen_signals(n, batch, 7, "normal")
+    if c64:
+        x = x.astype(np.complex64)
+    src = hip.DeviceBuffer.from_array(x)
+    dst = hip.DeviceBuffer(x.nbytes)
+    stream = hip.Stream()
+    plan = fo.Plan([(src, dst, batch, n, flags)])
+    plan.autotune(reps=3, stream=stream)
+
+    def timed(run):
+        for _ in range(warmup):
+            run()
+        stream.synchronize()
+        a, b = hip.Event(), hip.Event()
+        a.record(stream)
+        for _ in range(steps):
+            run()
+        b.record(stream)
+        stream.synchronize()
+        return a.elapsed_ms(b) / steps
+
+    ms_plan = timed(lambda: plan.execute(stream=stream, graph=True))
+    ms_call = timed(lambda: fo.fft(src, batch, n, out=dst, flags=flags, stream=stream))
+    routine = plan.routine(n, c64=c64)
+    plan.close()
+    src.free()
+    dst.free()
+    nbytes = 2 * n * batch * es
+    return {"n": n, "batch": batch, "dtype": "c64" if c64 else "c128", "routine": routine,
+            "plan_ms": round(ms_plan, 4), "plan_GBps": round(nbytes / (ms_plan * 1e-3) / 1e9, 1),
+            "per_call_ms": round(ms_call, 4),
+            "per_call_GBps": round(nbytes / (ms_call * 1e-3) / 1e9, 1),
+            "plan_over_per_call": round(ms_call / ms_plan, 3)}
+
+
 def run_served(fo, seconds: float = 1.0, rate: float = 20000.0, saturated: int = 4000):
     """Poisson request stream through the device-resident server (C5)."""
     from fft_onchip_amd import hip
@@ -730,6 +773,8 @@ def main():
         extra["batched_small_jobs"] = run_batched(fo, args.steps, args.warmup, njobs=1024,
                                                   max_batch=16)
         print(f"[bench] batched plan: {extra['batched_mixed_c5']}", file=sys.stderr, flush=True)
+        extra["plan_single_n4096"] = run_plan_single(fo, args.steps, args.warmup)
+        print(f"[bench] single-job plan: {extra['plan_single_n4096']}", file=sys.stderr, flush=True)
         extra["served_c5"] = run_served(fo)
         print(f"[bench] served: {extra['served_c5']}", file=sys.stderr, flush=True)
     if not args.quick or args.onchip:

@@ -92,6 +92,11 @@ int launch_batch_noalloc(const NoAllocJobs &jobs, int njobs, int64_t total_items
 
 int batch_routine_count();
 int batch_routine_default(int n, bool c64);  // untuned routine of a length / data type
+// the routine of a mixed-size kernel (or the server) for routine id: id
+// itself, or for a wide routine (512 threads, one-routine kernels only) the
+// kBatchThreads routine of its length
+int batch_routine_mixed(int id, bool c64);
+int batch_routine_threads(int id);
 int batch_routine_n(int id);
 const char *batch_routine_name(int id);
 int batch_routine_signals(int id);

@@ -412,17 +412,32 @@ int launch_server(const ServerDev &sd, int grid, hipStream_t stream) {
 }
 
 int batch_routine_count() { return kNumRoutines; }
-int batch_routine_default(int n, bool c64) {
-  // Untuned choice per length (index log2 n - 3) and data type, from single
-  // 1 GiB jobs on MI355X (the kernel specialized to the routine): lane-shuffle
-  // tiles up to N = 32, pipelined streams above (complex128 N = 256: the plain
-  // tile loop), and at N = 4096 complex128 the matrix-core stream with the
-  // two-product contraction.
-  static constexpr int kC128[10] = {20, 21, 22, 25, 26, 10, 30, 32, 34, 37};
-  static constexpr int kC64[10] = {20, 21, 22, 25, 26, 29, 30, 31, 33, 35};
+// Untuned choice per length (index log2 n - 3) and data type, from single
+// 1 GiB jobs on MI355X: lane-shuffle tiles up to N = 32, pipelined streams
+// above (complex128 N = 256: the plain tile loop; N = 4096: the matrix-core
+// stream, two-product contraction).  All are kBatchThreads routines, so a
+// job's untuned result is the same alone (one-routine kernel) and in a mix
+// (mixed-size kernel); the wide routines are there for set_routine and the
+// autotuner of plans whose jobs share one routine.
+constexpr int kMixedC128[10] = {20, 21, 22, 25, 26, 10, 30, 32, 39, 37};
+constexpr int kMixedC64[10] = {20, 21, 22, 25, 26, 29, 30, 31, 33, 35};
+int length_index(int n) {
   for (int k = 0; k < 10; ++k)
-    if (n == (8 << k)) return c64 ? kC64[k] : kC128[k];
+    if (n == (8 << k)) return k;
   return -1;
+}
+
+int batch_routine_default(int n, bool c64) {
+  const int k = length_index(n);
+  return k < 0 ? -1 : c64 ? kMixedC64[k] : kMixedC128[k];
+}
+int batch_routine_mixed(int id, bool c64) {
+  if (id < 0 || id >= kNumRoutines || kRoutines[id].threads == kBatchThreads) return id;
+  const int k = length_index(kRoutines[id].n);
+  return k < 0 ? -1 : c64 ? kMixedC64[k] : kMixedC128[k];
+}
+int batch_routine_threads(int id) {
+  return (id >= 0 && id < kNumRoutines) ? kRoutines[id].threads : 0;
 }
 int batch_routine_n(int id) { return (id >= 0 && id < kNumRoutines) ? kRoutines[id].n : 0; }
 const char *batch_routine_name(int id) {

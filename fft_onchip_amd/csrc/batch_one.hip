@@ -18,8 +18,8 @@ constexpr int routine_lds(TForm f, int n, int s, bool c128) {
   return lds_elems(s * n);
 }
 
-template <class SP, int S, TForm F, class CT>
-__global__ __launch_bounds__(kBatchThreads) __attribute__((amdgpu_waves_per_eu(2))) void
+template <class SP, int S, TForm F, class CT, int THREADS = kRoutineThreads<SP, S, F, CT>>
+__global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(2))) void
 batch_one_kernel(const JobDev *__restrict__ jobs, int njobs, int64_t total_items,
                  unsigned long long *ctr, const CT *__restrict__ tw) {
   using K = typename RoutineKernel<SP, S, F, CT>::type;
@@ -79,8 +79,8 @@ int64_t batch_one_resident(int routine, bool c64) {
     if (v > 0) return v;
   }
   int per_cu = 0, cus = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kOneKernels[routine][t], kBatchThreads,
-                                                   0) != hipSuccess ||
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kOneKernels[routine][t],
+                                                   kRoutines[routine].threads, 0) != hipSuccess ||
       per_cu <= 0)
     per_cu = 1;
   // at most two workgroups per CU, as the mixed-size kernel: more (the light
@@ -100,8 +100,8 @@ int launch_batch_one(int routine, bool c64, const JobDev *d_jobs, int njobs, int
                      unsigned long long *d_ctr, const void *tw, int grid, hipStream_t stream) {
   if (routine < 0 || routine >= kNumRoutines || grid <= 0) return FFTOC_ERROR_INVALID_BATCH;
   void *args[] = {&d_jobs, &njobs, &total_items, &d_ctr, &tw};
-  if (hipLaunchKernel(kOneKernels[routine][c64 ? 1 : 0], dim3(grid), dim3(kBatchThreads), args, 0,
-                      stream) != hipSuccess)
+  if (hipLaunchKernel(kOneKernels[routine][c64 ? 1 : 0], dim3(grid),
+                      dim3(kRoutines[routine].threads), args, 0, stream) != hipSuccess)
     return FFTOC_ERROR_LAUNCH;
   return hipGetLastError() == hipSuccess ? FFTOC_SUCCESS : FFTOC_ERROR_LAUNCH;
 }

@@ -203,13 +203,27 @@ struct RoutineKernel<SP, S, TForm::Lanes, CT> {
   X(34, 2048, "8x16x16/stream/S2", (Split<8, 16, 16>), 2, TForm::Stream) \
   X(35, 4096, "16x16x16/stream/S1", (Split<16, 16, 16>), 1, TForm::Stream) \
   X(36, 4096, "tensor:16x16x16/stream/S1", (Split<16, 16, 16>), 1, TForm::TensorK) \
-  X(37, 4096, "tensor:16x16x16/stream/2-product/S1", (Split<16, 16, 16>), 1, TForm::TensorH)
+  X(37, 4096, "tensor:16x16x16/stream/2-product/S1", (Split<16, 16, 16>), 1, TForm::TensorH) \
+  X(38, 1024, "8x8x8x2/stream/S2", (Split<8, 8, 8, 2>), 2, TForm::Stream) \
+  X(39, 2048, "8x8x8x4/stream/S1", (Split<8, 8, 8, 4>), 1, TForm::Stream) \
+  X(40, 4096, "8x8x8x8/stream/S1 (512 threads)", (Split<8, 8, 8, 8>), 1, TForm::Stream)
 
 #define FFTOC_UNPAREN(...) __VA_ARGS__
 
+// Routines run on workgroups of kBatchThreads threads, except wide routines
+// (kWideThreads: the N = 4096 radix-8 stream, eight values per thread), which
+// only the one-routine kernels run: mixed plans and the server use a
+// kBatchThreads routine of the same length instead (batch_routine_mixed).
+constexpr int kWideThreads = 512;
+template <class SP, int S, TForm F, class CT>
+constexpr int kRoutineThreads = RoutineKernel<SP, S, F, CT>::type::THREADS;
+
 #define FFTOC_CHECK_THREADS(ID, NN, NAME, SPLIT, S, FORM)                                     \
-  static_assert(RoutineKernel<FFTOC_UNPAREN SPLIT, S, FORM, c2>::type::THREADS == kBatchThreads, \
-                "tile routine " #ID " must use 256 threads");                                 \
+  static_assert((kRoutineThreads<FFTOC_UNPAREN SPLIT, S, FORM, c2> == kBatchThreads ||         \
+                 kRoutineThreads<FFTOC_UNPAREN SPLIT, S, FORM, c2> == kWideThreads) &&         \
+                    kRoutineThreads<FFTOC_UNPAREN SPLIT, S, FORM, c2> ==                       \
+                        kRoutineThreads<FFTOC_UNPAREN SPLIT, S, FORM, f2>,                     \
+                "tile routine " #ID ": 256 or 512 threads");                                  \
   static_assert(FFTOC_UNPAREN SPLIT::N == NN, "tile routine " #ID " length");                 \
   static_assert(S * NN <= 4096, "tile routine " #ID " LDS");
 FFTOC_ROUTINES(FFTOC_CHECK_THREADS)
@@ -219,9 +233,11 @@ struct RoutineInfo {
   int n;
   const char *name;
   int signals_per_tile;
+  int threads;
 };
 
-#define FFTOC_INFO(ID, NN, NAME, SPLIT, S, FORM) {NN, NAME, S},
+#define FFTOC_INFO(ID, NN, NAME, SPLIT, S, FORM) \
+  {NN, NAME, S, kRoutineThreads<FFTOC_UNPAREN SPLIT, S, FORM, c2>},
 constexpr RoutineInfo kRoutines[] = {FFTOC_ROUTINES(FFTOC_INFO)};
 #undef FFTOC_INFO
 constexpr int kNumRoutines = sizeof(kRoutines) / sizeof(kRoutines[0]);
@@ -335,6 +351,22 @@ __device__ __noinline__ Cursor tile_stream(global_ptr<const CT> in, global_ptr<C
                                   s_item, lds, tw, scale, inv);
 }
 
+// The call of the mixed-size kernels; a wide routine is never dispatched
+// there (the host maps it to a kBatchThreads routine).
+template <class K, TForm F, class CT>
+__device__ __forceinline__ Cursor mixed_call(global_ptr<const CT> in, global_ptr<CT> out,
+                                             int64_t batch, Cursor cur, int64_t item_begin,
+                                             int64_t item_end, int64_t total, int64_t ntiles,
+                                             int tpi, global_ptr<unsigned long long> ctr,
+                                             lds_ptr<long long> s_item, lds_ptr<CT> lds,
+                                             global_ptr<const CT> tw, real_t<CT> scale, bool inv) {
+  if constexpr (K::THREADS == kBatchThreads)
+    return tile_stream<K, F, CT>(in, out, batch, cur, item_begin, item_end, total, ntiles, tpi, ctr,
+                                 s_item, lds, tw, scale, inv);
+  else
+    return Cursor{item_end, cur.claim, cur.ahead};
+}
+
 template <class CT>
 __device__ __forceinline__ Cursor run_routine(int id, const JobDev &J, Cursor cur, int64_t item_begin,
                                               int64_t item_end, int64_t total,
@@ -347,7 +379,7 @@ __device__ __forceinline__ Cursor run_routine(int id, const JobDev &J, Cursor cu
   switch (id) {
 #define FFTOC_CASE(ID, NN, NAME, SPLIT, S, FORM)                                                  \
   case ID:                                                                                      \
-    return tile_stream<typename RoutineKernel<FFTOC_UNPAREN SPLIT, S, FORM, CT>::type, FORM, CT>( \
+    return mixed_call<typename RoutineKernel<FFTOC_UNPAREN SPLIT, S, FORM, CT>::type, FORM, CT>( \
         (global_ptr<const CT>)in, (global_ptr<CT>)out, J.batch, cur, item_begin, item_end, total,  \
         J.ntiles, J.tiles_per_item, (global_ptr<unsigned long long>)ctr,                           \
         (lds_ptr<long long>)s_item, (lds_ptr<CT>)lds, (global_ptr<const CT>)tw, scale, inv);

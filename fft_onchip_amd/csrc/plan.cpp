@@ -119,13 +119,33 @@ bool heavier(const fftoc_job &a, const fftoc_job &b) {
 
 // Lays the jobs out as work items and uploads the descriptors.
 int layout(fftoc_plan_t *p) {
+  // Jobs sharing one routine and data type run the kernel specialized to it;
+  // otherwise the mixed-size kernel runs, with kBatchThreads routines only.
+  p->one_routine = -1;
+  bool first = true;
+  for (int idx : p->order) {
+    const fftoc_job &j = p->jobs[idx];
+    if (j.batch == 0) continue;
+    const bool c64 = (j.flags & FFTOC_DATA_C64) != 0;
+    const int r = p->routine[lg(j.n)][c64 ? 1 : 0];
+    if (first) {
+      p->one_routine = r;
+      p->one_c64 = c64;
+      first = false;
+    } else if (r != p->one_routine || c64 != p->one_c64) {
+      p->one_routine = -1;
+      break;
+    }
+  }
   p->host.clear();
   p->total_items = 0;
   for (int idx : p->order) {
     const fftoc_job &j = p->jobs[idx];
     if (j.batch == 0) continue;
-    const JobDev d =
-        make_jobdev(j, p->routine[lg(j.n)][(j.flags & FFTOC_DATA_C64) ? 1 : 0], p->total_items);
+    const bool c64 = (j.flags & FFTOC_DATA_C64) != 0;
+    int r = p->routine[lg(j.n)][c64 ? 1 : 0];
+    if (p->one_routine < 0) r = batch_routine_mixed(r, c64);
+    const JobDev d = make_jobdev(j, r, p->total_items);
     p->total_items += job_items(d);
     p->host.push_back(d);
   }
@@ -138,18 +158,6 @@ int layout(fftoc_plan_t *p) {
     if (hipMalloc(&p->d_jobs, bytes) != hipSuccess) return FFTOC_ERROR_ALLOC;
     if (hipMemcpy(p->d_jobs, p->host.data(), bytes, hipMemcpyHostToDevice) != hipSuccess)
       return FFTOC_ERROR_DEVICE;
-  }
-  // jobs sharing one routine and data type run the kernel specialized to it
-  p->one_routine = -1;
-  for (size_t k = 0; k < p->host.size(); ++k) {
-    const JobDev &d = p->host[k];
-    if (k == 0) {
-      p->one_routine = d.routine;
-      p->one_c64 = d.c64 != 0;
-    } else if (d.routine != p->one_routine || (d.c64 != 0) != p->one_c64) {
-      p->one_routine = -1;
-      break;
-    }
   }
   const int64_t resident = p->one_routine >= 0 ? batch_one_resident(p->one_routine, p->one_c64)
                                                : batch_resident_blocks();
@@ -434,7 +442,8 @@ int fftoc_batch_fft_noalloc(const fftoc_job *jobs, int njobs, void *stream) {
   int64_t total = 0;
   for (int k = 0; k < live; ++k) {
     const fftoc_job &j = jobs[order[k]];
-    desc.j[k] = make_jobdev(j, batch_routine_default(j.n, (j.flags & FFTOC_DATA_C64) != 0), total);
+    const bool c64 = (j.flags & FFTOC_DATA_C64) != 0;
+    desc.j[k] = make_jobdev(j, batch_routine_mixed(batch_routine_default(j.n, c64), c64), total);
     total += job_items(desc.j[k]);
   }
   const int grid = (int)std::min<int64_t>(batch_resident_blocks(), total);
