@@ -92,6 +92,7 @@ int launch_batch_noalloc(const NoAllocJobs &jobs, int njobs, int64_t total_items
 
 int batch_routine_count();
 int batch_routine_default(int n, bool c64);  // untuned routine of a length / data type
+int batch_routine_server(int n, bool c64);   // the serving kernel's routine
 // the routine of a mixed-size kernel (or the server) for routine id: id
 // itself, or for a wide routine (512 threads, one-routine kernels only) the
 // kBatchThreads routine of its length
@@ -103,10 +104,12 @@ int batch_routine_signals(int id);
 int batch_resident_blocks();
 int launch_batch(const JobDev *d_jobs, int njobs, int64_t total_items, unsigned long long *d_ctr,
                  const void *tw64, const void *tw32, int grid, hipStream_t stream);
-// batch_one.hip: the kernel specialized to one routine and data type (every
-// job of the launch runs that routine on that data type)
-int64_t batch_one_resident(int routine, bool c64);
-int launch_batch_one(int routine, bool c64, const JobDev *d_jobs, int njobs, int64_t total_items,
-                     unsigned long long *d_ctr, const void *tw, int grid, hipStream_t stream);
+// batch_one*.hip: the kernel specialized to one routine and data type (every
+// job of the launch runs that routine on that data type) and direction (dir
+// 0: every job forward, 1: every job inverse, 2: each job its own)
+int64_t batch_one_resident(int routine, bool c64, int dir);
+int launch_batch_one(int routine, bool c64, int dir, const JobDev *d_jobs, int njobs,
+                     int64_t total_items, unsigned long long *d_ctr, const void *tw, int grid,
+                     hipStream_t stream);
 
 }  // namespace fftoc

@@ -412,15 +412,18 @@ int launch_server(const ServerDev &sd, int grid, hipStream_t stream) {
 }
 
 int batch_routine_count() { return kNumRoutines; }
-// Untuned choice per length (index log2 n - 3) and data type, from single
-// 1 GiB jobs on MI355X: lane-shuffle tiles up to N = 32, pipelined streams
-// above (complex128 N = 256: the plain tile loop; N = 4096: the matrix-core
-// stream, two-product contraction).  All are kBatchThreads routines, so a
-// job's untuned result is the same alone (one-routine kernel) and in a mix
-// (mixed-size kernel); the wide routines are there for set_routine and the
-// autotuner of plans whose jobs share one routine.
-constexpr int kMixedC128[10] = {20, 21, 22, 25, 26, 10, 30, 32, 39, 37};
-constexpr int kMixedC64[10] = {20, 21, 22, 25, 26, 29, 30, 31, 33, 35};
+// Untuned choices per length (index log2 n - 3) and data type, from single
+// 1 GiB jobs on MI355X.  kPlan*: plans, whose jobs sharing one routine run the
+// kernel specialized to it (lane-shuffle tiles up to N = 32, then mostly the
+// plain tile loops, which gain most from a compile-time direction).  All are
+// kBatchThreads routines, so a job's untuned result is the same alone and in
+// a mix (the mixed-size kernel); the wide routines are there for set_routine
+// and the autotuner.  kServe*: the serving kernel, one item per routine call
+// with the direction chosen at run time, where the pipelined streams run best.
+constexpr int kPlanC128[10] = {20, 21, 22, 7, 19, 10, 13, 15, 16, 18};
+constexpr int kPlanC64[10] = {20, 21, 22, 25, 26, 10, 13, 15, 33, 35};
+constexpr int kServeC128[10] = {20, 21, 22, 25, 26, 10, 30, 32, 39, 37};
+constexpr int kServeC64[10] = {20, 21, 22, 25, 26, 29, 30, 31, 33, 35};
 int length_index(int n) {
   for (int k = 0; k < 10; ++k)
     if (n == (8 << k)) return k;
@@ -429,12 +432,15 @@ int length_index(int n) {
 
 int batch_routine_default(int n, bool c64) {
   const int k = length_index(n);
-  return k < 0 ? -1 : c64 ? kMixedC64[k] : kMixedC128[k];
+  return k < 0 ? -1 : c64 ? kPlanC64[k] : kPlanC128[k];
+}
+int batch_routine_server(int n, bool c64) {
+  const int k = length_index(n);
+  return k < 0 ? -1 : c64 ? kServeC64[k] : kServeC128[k];
 }
 int batch_routine_mixed(int id, bool c64) {
   if (id < 0 || id >= kNumRoutines || kRoutines[id].threads == kBatchThreads) return id;
-  const int k = length_index(kRoutines[id].n);
-  return k < 0 ? -1 : c64 ? kMixedC64[k] : kMixedC128[k];
+  return batch_routine_default(kRoutines[id].n, c64);
 }
 int batch_routine_threads(int id) {
   return (id >= 0 && id < kNumRoutines) ? kRoutines[id].threads : 0;

@@ -32,8 +32,10 @@ enum class TForm { Tile, Staged, Stream, Lanes, TensorK, TensorH };
 // registers) and are in flight during stage B and the stores; the stream's
 // next() comes after stage B, before the stores, when those loads have had a
 // stage B's time to arrive.
-template <class CORE>
+template <class CORE, int INVM = 2>
 struct TensorTile4096 {
+  // INVM: 0 forward, 1 inverse, 2 the run-time `inv` argument
+  __device__ static bool swap_io(bool inv) { return INVM == 2 ? inv : INVM == 1; }
   static constexpr int THREADS = 256;
   __device__ static int zpos(int k1, int m) { return k1 * 256 + (m ^ k1); }
   __device__ static int rowpos(int k1, int k2) { return k1 * 256 + (k2 ^ (((k2 >> 4) ^ k1) & 15)); }
@@ -45,7 +47,7 @@ struct TensorTile4096 {
 #pragma unroll
     for (int n1 = 0; n1 < 16; ++n1) {
       pin_vgpr(v[n1]);
-      if (inv) v[n1] = cswap(v[n1]);
+      if (swap_io(inv)) v[n1] = cswap(v[n1]);
     }
     dft<16>(v);
     apply_twiddle_base<16>(v, ta);
@@ -130,7 +132,7 @@ struct TensorTile4096 {
         for (int c = 0; c < 8; ++c) {
           c2 r = o[c];
           if (scale != 1.0) r = cscale(r, scale);
-          if (inv) r = cswap(r);
+          if (swap_io(inv)) r = cswap(r);
           st_stream(y + 256 * (8 * h + c) + t, r);
         }
       }
@@ -142,24 +144,27 @@ struct TensorTile4096 {
   }
 };
 
-template <class SP, int S, TForm F, class CT>
+// DIR: the direction, 0 forward, 1 inverse, 2 chosen at run time per job (a
+// run-time choice costs some 6 % at N = 4096: the selects sit on every
+// loaded and stored value); the mixed-size kernels run DIR = 2.
+template <class SP, int S, TForm F, class CT, int DIR = 2>
 struct RoutineKernel {
-  using type = StockhamKernel<SP, S, 2, F == TForm::Staged, CT>;
+  using type = StockhamKernel<SP, S, DIR, F == TForm::Staged, CT>;
 };
-template <class SP, int S>
-struct RoutineKernel<SP, S, TForm::TensorK, c2> {
-  using type = TensorTile4096<KaratsubaCore>;
+template <class SP, int S, int DIR>
+struct RoutineKernel<SP, S, TForm::TensorK, c2, DIR> {
+  using type = TensorTile4096<KaratsubaCore, DIR>;
 };
-template <class SP, int S>
-struct RoutineKernel<SP, S, TForm::TensorH, c2> {
-  using type = TensorTile4096<TwoProductCore>;
+template <class SP, int S, int DIR>
+struct RoutineKernel<SP, S, TForm::TensorH, c2, DIR> {
+  using type = TensorTile4096<TwoProductCore, DIR>;
 };
 // which kernels a routine streams with run_stream (the others run run_tile
 // per tile)
 constexpr bool streams(TForm f) { return f != TForm::Tile && f != TForm::Staged; }
-template <class SP, int S, class CT>
-struct RoutineKernel<SP, S, TForm::Lanes, CT> {
-  using type = LaneTile<SP::N, S * SP::N / kBatchThreads, 2, CT>;
+template <class SP, int S, class CT, int DIR>
+struct RoutineKernel<SP, S, TForm::Lanes, CT, DIR> {
+  using type = LaneTile<SP::N, S * SP::N / kBatchThreads, DIR, CT>;
 };
 
 // Routine table: id -> (n, split, signals per tile, form).  Two or three

@@ -23,6 +23,7 @@ struct fftoc_plan_t {
   int grid = 0;
   int one_routine = -1;                 // the routine of every job, if they share
   bool one_c64 = false;                 // one (routine, data type); else -1
+  int one_dir = 2;                      // 0 / 1: every job forward / inverse
   int routine[13][2];                   // [log2 n][complex64] -> routine id
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -122,19 +123,24 @@ int layout(fftoc_plan_t *p) {
   // Jobs sharing one routine and data type run the kernel specialized to it;
   // otherwise the mixed-size kernel runs, with kBatchThreads routines only.
   p->one_routine = -1;
+  p->one_dir = 2;
   bool first = true;
   for (int idx : p->order) {
     const fftoc_job &j = p->jobs[idx];
     if (j.batch == 0) continue;
     const bool c64 = (j.flags & FFTOC_DATA_C64) != 0;
     const int r = p->routine[lg(j.n)][c64 ? 1 : 0];
+    const int dir = (j.flags & FFTOC_INVERSE) ? 1 : 0;
     if (first) {
       p->one_routine = r;
       p->one_c64 = c64;
+      p->one_dir = dir;
       first = false;
     } else if (r != p->one_routine || c64 != p->one_c64) {
       p->one_routine = -1;
       break;
+    } else if (dir != p->one_dir) {
+      p->one_dir = 2;
     }
   }
   p->host.clear();
@@ -159,8 +165,9 @@ int layout(fftoc_plan_t *p) {
     if (hipMemcpy(p->d_jobs, p->host.data(), bytes, hipMemcpyHostToDevice) != hipSuccess)
       return FFTOC_ERROR_DEVICE;
   }
-  const int64_t resident = p->one_routine >= 0 ? batch_one_resident(p->one_routine, p->one_c64)
-                                               : batch_resident_blocks();
+  const int64_t resident = p->one_routine >= 0
+                               ? batch_one_resident(p->one_routine, p->one_c64, p->one_dir)
+                               : batch_resident_blocks();
   p->grid = (int)std::max<int64_t>(1, std::min<int64_t>(resident, p->total_items));
   if (p->exec) {
     (void)hipGraphExecDestroy(p->exec);
@@ -176,8 +183,9 @@ int layout(fftoc_plan_t *p) {
 int launch(fftoc_plan_t *p, hipStream_t s) {
   if (p->total_items == 0) return FFTOC_SUCCESS;
   if (p->one_routine >= 0)
-    return launch_batch_one(p->one_routine, p->one_c64, p->d_jobs, (int)p->host.size(),
-                            p->total_items, p->d_ctr, p->one_c64 ? p->tw32 : p->tw64, p->grid, s);
+    return launch_batch_one(p->one_routine, p->one_c64, p->one_dir, p->d_jobs,
+                            (int)p->host.size(), p->total_items, p->d_ctr,
+                            p->one_c64 ? p->tw32 : p->tw64, p->grid, s);
   return launch_batch(p->d_jobs, (int)p->host.size(), p->total_items, p->d_ctr, p->tw64, p->tw32,
                       p->grid, s);
 }

@@ -215,8 +215,8 @@ int fftoc_server_create(size_t hbm_budget_bytes, int max_inflight, fftoc_server 
   s->tw64 = tw64;
   s->tw32 = tw32;
   for (int k = 0; k < 13; ++k) {
-    s->routine[k][0] = batch_routine_mixed(batch_routine_default(1 << k, false), false);
-    s->routine[k][1] = batch_routine_mixed(batch_routine_default(1 << k, true), true);
+    s->routine[k][0] = batch_routine_server(1 << k, false);
+    s->routine[k][1] = batch_routine_server(1 << k, true);
   }
   s->nslots = max_inflight;
   s->slots.assign(max_inflight, Slot{});
