@@ -299,19 +299,19 @@ int fftoc_plan_autotune(fftoc_plan p, int reps, void *stream_p) {
     (void)hipEventDestroy(e0);
     return FFTOC_ERROR_DEVICE;
   }
+  // Mean over `reps` back-to-back launches after a warm-up launch: plans run
+  // in loops, and a launch timed alone after a synchronisation starts from an
+  // idle GPU, which favours kernels with more workgroups.
   auto time_plan = [&](double *us) -> int {
     if ((rc = launch(p, s)) != FFTOC_SUCCESS) return rc;  // warm-up
-    std::vector<float> t;
-    for (int r = 0; r < reps; ++r) {
-      float ms = 0;
-      if (hipEventRecord(e0, s) != hipSuccess || launch(p, s) != FFTOC_SUCCESS ||
-          hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
-          hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
-        return FFTOC_ERROR_DEVICE;
-      t.push_back(ms);
-    }
-    std::sort(t.begin(), t.end());
-    *us = 1000.0 * t[t.size() / 2];
+    float ms = 0;
+    if (hipEventRecord(e0, s) != hipSuccess) return FFTOC_ERROR_DEVICE;
+    for (int r = 0; r < reps; ++r)
+      if ((rc = launch(p, s)) != FFTOC_SUCCESS) return rc;
+    if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+        hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+      return FFTOC_ERROR_DEVICE;
+    *us = 1000.0 * ms / reps;
     return FFTOC_SUCCESS;
   };
   // coordinate descent over the (length, data type) classes present

@@ -197,7 +197,10 @@ int fftoc_plan_execute(fftoc_plan plan, void *stream);
 /* The same launch replayed from a hipGraph captured on first use. */
 int fftoc_plan_execute_graph(fftoc_plan plan, void *stream);
 /* Times the candidate tile routines of every (length, data type) in the plan
- * on its own buffers and keeps the fastest (overwrites the outputs). */
+ * on its own buffers -- `reps` back-to-back launches each, after one warm-up
+ * launch -- and keeps the fastest (overwrites the outputs).  A plan whose
+ * jobs share one routine, data type and direction runs a kernel specialized
+ * to them, and its candidates include the 512-thread routines. */
 int fftoc_plan_autotune(fftoc_plan plan, int reps, void *stream);
 /* Name of the tile routine the plan uses for (n, data type of flags). */
 int fftoc_plan_routine(fftoc_plan plan, int n, uint32_t flags, const char **name);
