@@ -148,17 +148,10 @@ ONCHIP_FAMILIES = {0: "radix (fftoc::LdsFft)", 1: "tensor Karatsuba (fftoc::LdsT
                    2: "tensor two-product (fftoc::LdsTensorFft<.., true>)"}
 
 
-def run_onchip(fo, reps: int = 3):
-    """On-chip timing with the reference's method (includes/testing.cuh:94-127):
-    one launch loads the batch into LDS, runs the in-LDS functor R times and
-    stores once; R = 100 and R = 1100 are timed with HIP events on the launch
-    stream and (t1100 - t100) / 1000 is the time of one pass over the batch,
-    HBM traffic excluded.  Every functor family at N = 8, 64, 128, 256, 512,
-    4096; batches of 32 FFTs per CU (the reference's shape), and one and four
-    LDS-fulls of signals per CU (160 KiB = 10240 points each), the best of
-    which is reported as the saturated rate."""
+def onchip_library(fo):
+    """The example library's on-chip harness entry point and the device's
+    FP64 twiddle table."""
     import ctypes
-    from fft_onchip_amd import hip
 
     lib = ctypes.CDLL(os.path.join(ROOT, "fft_onchip_amd", "libfftoc_fused_example.so"))
     run = lib.fftoc_example_onchip_fft
@@ -168,6 +161,145 @@ def run_onchip(fo, reps: int = 3):
     tw = ctypes.c_void_p()
     if fo.lib().fftoc_twiddle_table(ctypes.byref(tw), None) != 0:
         raise RuntimeError("fftoc_twiddle_table failed")
+    return run, tw
+
+
+def onchip_pass_us(run, tw, stream, xs: np.ndarray, fam: int, reps: int) -> float:
+    """Microseconds of one in-LDS pass over the signals xs (batch, n):
+    (t[R=1100] - t[R=100]) / 1000, medians of `reps` launches each."""
+    from fft_onchip_amd import hip
+
+    batch, n = xs.shape
+    buf = hip.DeviceBuffer.from_array(np.ascontiguousarray(xs))
+
+    def timed(r):
+        a, b = hip.Event(), hip.Event()
+        a.record(stream)
+        rc = run(buf.ptr, batch, n, tw, fam, r, 1.0 / math.sqrt(n), stream.handle)
+        b.record(stream)
+        stream.synchronize()
+        if rc != 0:
+            raise RuntimeError(f"fftoc_example_onchip_fft n={n} family={fam}: {rc}")
+        return a.elapsed_ms(b)
+
+    timed(100)  # warm-up (clocks, code load)
+    t100 = statistics.median(timed(100) for _ in range(reps))
+    t1100 = statistics.median(timed(1100) for _ in range(reps))
+    buf.free()
+    return (t1100 - t100) * 1e3 / 1000.0
+
+
+# The reference's saturation sweep (scripts/results/saturation.csv, from
+# src/saturation.cu:15-62): microseconds per on-chip pass over k FFTs per SM
+# on 108 SMs, matrix-core functor and cuFFTDx; N -> {k: (line, tensor us,
+# cuFFTDx us)}.
+SATURATION_REF = {
+    8: {4: (2, 0.959488, 1.06394), 16: (5, 0.95744, 1.06189), 64: (17, 0.96256, 1.64454),
+        256: (65, 1.41517, 4.82099)},
+    64: {4: (67, 0.390144, 1.55443), 16: (70, 1.21037, 1.54931), 64: (82, 4.5865, 4.51891),
+         256: (130, 18.0838, 17.4725)},
+    128: {4: (132, 1.0537, 1.6087), 16: (135, 2.0265, 2.56614), 64: (147, 7.83667, 8.67533),
+          256: (195, 31.0579, 33.6507)},
+    256: {4: (197, 2.23232, 1.66605), 16: (200, 4.3479, 4.58957), 64: (212, 16.9667, 17.1561),
+          256: (260, 67.542, 67.4499)},
+    512: {4: (262, 5.29408, 2.77914), 16: (265, 18.3081, 9.05421), 64: (277, 71.3052, 34.902),
+          256: (325, 283.205, 138.338)},
+}
+
+
+def run_saturation(fo, reps: int = 3):
+    """The reference's saturation sweep (src/saturation.cu:15-62) on this
+    GPU: on-chip rate of the radix and Karatsuba matrix-core functors at k
+    FFTs per CU, k = 4, 16, 64, 256, beside the reference's rows at the same
+    k per SM (its FFT/s = k * 108 / time)."""
+    from fft_onchip_amd import hip
+
+    run, tw = onchip_library(fo)
+    cus = hip.cu_count()
+    stream = hip.Stream()
+    out = {"method": "on-chip pass time as extra.onchip; k FFTs per CU (k * CUs signals)",
+           "cus": cus, "rows": {}}
+    for n, ref in SATURATION_REF.items():
+        g = np.random.Generator(np.random.Philox(41 + n))
+        kmax = max(ref)
+        xs = g.uniform(-1, 1, (kmax * cus, n)) + 1j * g.uniform(-1, 1, (kmax * cus, n))
+        rows = {}
+        for k, (line, t_tensor, t_dx) in ref.items():
+            r = {"reference": {"row": f"scripts/results/saturation.csv:{line}",
+                               "tensor_ffts_per_s": k * 108 / (t_tensor * 1e-6),
+                               "cufftdx_ffts_per_s": k * 108 / (t_dx * 1e-6)}}
+            for fam in (0, 1):
+                us = onchip_pass_us(run, tw, stream, xs[:k * cus], fam, reps)
+                r[f"family{fam}_ffts_per_s"] = k * cus / (us * 1e-6) if us > 0 else float("nan")
+            best_ref = max(r["reference"]["tensor_ffts_per_s"], r["reference"]["cufftdx_ffts_per_s"])
+            r["best_vs_reference"] = round(max(r["family0_ffts_per_s"], r["family1_ffts_per_s"])
+                                           / best_ref, 3)
+            rows[str(k)] = r
+        out["rows"][str(n)] = rows
+        print(f"[bench] saturation n={n}: " + ", ".join(
+            f"k={k} {v['best_vs_reference']}x" for k, v in rows.items()), file=sys.stderr, flush=True)
+    return out
+
+
+def run_per_cu(fo, rounds: int = 3, launches: int = 20):
+    """Workgroups per CU as a measured dimension (the reference's blocks-per-
+    SM sweep, src/saturation.cu:15-62, here over HBM-resident batches): the
+    variant fftoc_fft picks for each workload, at its occupancy and at every
+    smaller cap (FFTOC_WGS_PER_CU), back-to-back launches over buffers that
+    rotate past the 256 MB last-level cache; median over rounds that visit
+    the caps in turn.  `default_cap` is the cap fftoc_fft applies."""
+    from fft_onchip_amd import hip
+
+    lib = fo.lib()
+    stream = hip.Stream()
+    e0, e1 = hip.Event(), hip.Event()
+    out = {}
+    for n, batch in ((4096, 16384), (1024, 16384), (512, 32768), (256, 65536), (64, 65536)):
+        nbytes = n * batch * 16
+        pairs = max(1, min(8, math.ceil(4 * MALL_BYTES / (2 * nbytes))))
+        x = gen_signals(n, batch, 3, "uniform")
+        ins = [hip.DeviceBuffer.from_array(x) for _ in range(pairs)]
+        outs = [hip.DeviceBuffer(nbytes) for _ in range(pairs)]
+        v = lib.fftoc_get_variant(n, 0, batch)
+        occ = lib.fftoc_variant_occupancy(n, 0, v)
+        caps = list(range(0, min(occ, 16)))
+        gbps = {c: [] for c in caps}
+        for _ in range(rounds):
+            for c in caps:
+                fl = fo.make_flags(per_cu=c)
+                for i in range(3):
+                    fo.fft_variant(ins[i % pairs], batch, n, v, out=outs[i % pairs], flags=fl,
+                                   stream=stream)
+                e0.record(stream)
+                for i in range(launches):
+                    fo.fft_variant(ins[i % pairs], batch, n, v, out=outs[i % pairs], flags=fl,
+                                   stream=stream)
+                e1.record(stream)
+                e1.synchronize()
+                gbps[c].append(2 * nbytes * launches / (e0.elapsed_ms(e1) * 1e-3) / 1e9)
+        for b in ins + outs:
+            b.free()
+        out[f"fp64_n{n}_b{batch}"] = {
+            "variant": fo.variants(n)[v], "occupancy": occ,
+            "default_cap": lib.fftoc_get_workgroups_per_cu(n, 0, batch),
+            "GBps": {("occupancy" if c == 0 else str(c)): round(statistics.median(t), 1)
+                     for c, t in gbps.items()}}
+        print(f"[bench] per-CU n={n}: {out[f'fp64_n{n}_b{batch}']}", file=sys.stderr, flush=True)
+    return out
+
+
+def run_onchip(fo, reps: int = 3):
+    """On-chip timing with the reference's method (includes/testing.cuh:94-127):
+    one launch loads the batch into LDS, runs the in-LDS functor R times and
+    stores once; R = 100 and R = 1100 are timed with HIP events on the launch
+    stream and (t1100 - t100) / 1000 is the time of one pass over the batch,
+    HBM traffic excluded.  Every functor family at N = 8, 64, 128, 256, 512,
+    4096; batches of 32 FFTs per CU (the reference's shape), and one and four
+    LDS-fulls of signals per CU (160 KiB = 10240 points each), the best of
+    which is reported as the saturated rate."""
+    from fft_onchip_amd import hip
+
+    run, tw = onchip_library(fo)
     cus = hip.cu_count()
     stream = hip.Stream()
     out = {"method": "(t[R=1100] - t[R=100]) / 1000 per pass; HIP events on the launch stream; "
@@ -190,23 +322,7 @@ def run_onchip(fo, reps: int = 3):
             best = 0.0
             for shape, fb in shapes.items():
                 batch = fb(n)
-                buf = hip.DeviceBuffer.from_array(np.ascontiguousarray(xs[:batch]))
-
-                def timed(r):
-                    a, b = hip.Event(), hip.Event()
-                    a.record(stream)
-                    rc = run(buf.ptr, batch, n, tw, fam, r, 1.0 / math.sqrt(n), stream.handle)
-                    b.record(stream)
-                    stream.synchronize()
-                    if rc != 0:
-                        raise RuntimeError(f"fftoc_example_onchip_fft n={n} family={fam}: {rc}")
-                    return a.elapsed_ms(b)
-
-                timed(100)  # warm-up (clocks, code load)
-                t100 = statistics.median(timed(100) for _ in range(reps))
-                t1100 = statistics.median(timed(1100) for _ in range(reps))
-                buf.free()
-                us = (t1100 - t100) * 1e3 / 1000.0
+                us = onchip_pass_us(run, tw, stream, xs[:batch], fam, reps)
                 rate = batch / (us * 1e-6) if us > 0 else float("nan")
                 res[shape] = {"batch": batch, "us_per_pass": round(us, 4), "ffts_per_s": rate,
                               "GFLOPps": rate * flops / 1e9}
@@ -785,6 +901,9 @@ def main():
         print(f"[bench] served: {extra['served_c5']}", file=sys.stderr, flush=True)
     if not args.quick or args.onchip:
         extra["onchip"] = run_onchip(fo)
+        extra["saturation"] = run_saturation(fo)
+    if not args.quick:
+        extra["workgroups_per_cu"] = run_per_cu(fo)
 
     cpu = None
     if not args.no_cpu:

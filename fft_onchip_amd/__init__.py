@@ -77,9 +77,11 @@ def _stream(s) -> int | None:
 
 
 def make_flags(inverse: bool = False, normalize: bool = False, precision: str = "fp64",
-               tile: int = 0) -> int:
+               tile: int = 0, per_cu: int = 0) -> int:
+    """Flag bits: direction, 1/N scaling, stage precision, FFTOC_TILE(tile)
+    and FFTOC_WGS_PER_CU(per_cu) (0 = the tuned choice)."""
     f = (INVERSE if inverse else 0) | (NORMALIZE if normalize else 0) | _PREC[precision]
-    return f | ((int(tile) & 0xFF) << 16)
+    return f | ((int(tile) & 0xFF) << 16) | ((int(per_cu) & 0x3F) << 24)
 
 
 def _call(fn_name: str, x, out, batch: int, n: int | None, flags: int, stream) -> None:
@@ -153,6 +155,25 @@ def fft_variant(x, batch: int, n: int, variant: int, out=None, flags: int = 0,
         raise FftocError(rc, "fftoc_fft_variant")
 
 
+def variant_occupancy(n: int, variant: int, c64: bool = False) -> int:
+    """Resident workgroups per CU of a variant's kernel on the current device
+    (the upper end of its workgroups-per-CU range)."""
+    return int(lib().fftoc_variant_occupancy(int(n), DATA_C64 if c64 else 0, int(variant)))
+
+
+def workgroups_per_cu(n: int, batch: int, c64: bool = False) -> int:
+    """Workgroups-per-CU cap fftoc_fft uses at (n, batch bucket); 0 = the
+    kernel's occupancy."""
+    return int(lib().fftoc_get_workgroups_per_cu(int(n), DATA_C64 if c64 else 0, int(batch)))
+
+
+def set_workgroups_per_cu(n: int, batch: int, per_cu: int, c64: bool = False) -> None:
+    """Overrides that cap (batch < 0: every batch bucket)."""
+    rc = lib().fftoc_set_workgroups_per_cu(int(n), DATA_C64 if c64 else 0, int(batch), int(per_cu))
+    if rc != 0:
+        raise FftocError(rc, "fftoc_set_workgroups_per_cu")
+
+
 _PATHS = {"tensor": tensor_fft, "legacy8": legacy_fft, "legacy16": legacy16_fft, "best": fft}
 
 
@@ -219,6 +240,17 @@ class Plan:
                                           name.encode())
         if rc != 0:
             raise FftocError(rc, "fftoc_plan_set_routine")
+
+    @property
+    def workgroups_per_cu(self) -> int:
+        """Workgroups per CU of the plan's launch (0 = the default)."""
+        return int(lib().fftoc_plan_workgroups_per_cu(self.handle))
+
+    @workgroups_per_cu.setter
+    def workgroups_per_cu(self, k: int) -> None:
+        rc = lib().fftoc_plan_set_workgroups_per_cu(self.handle, int(k))
+        if rc != 0:
+            raise FftocError(rc, "fftoc_plan_set_workgroups_per_cu")
 
     def close(self) -> None:
         if getattr(self, "handle", None):

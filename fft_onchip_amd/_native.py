@@ -38,6 +38,10 @@ SIGNATURES.update({
     "fftoc_autotune": ([_int, _u32, _i64, _int, _vp], _int),
     "fftoc_variant_time_us": ([_int, _u32, _int], ctypes.c_double),
     "fftoc_variant_tile": ([_int, _u32, _int], _int),
+    "fftoc_variant_per_cu": ([_int, _u32, _int], _int),
+    "fftoc_variant_occupancy": ([_int, _u32, _int], _int),
+    "fftoc_get_workgroups_per_cu": ([_int, _u32, _i64], _int),
+    "fftoc_set_workgroups_per_cu": ([_int, _u32, _i64, _int], _int),
     "fftoc_tuning_save": ([ctypes.c_char_p], _int),
     "fftoc_tuning_load": ([ctypes.c_char_p], _int),
 })
@@ -55,6 +59,8 @@ SIGNATURES.update({
     "fftoc_plan_autotune": ([_vp, _int, _vp], _int),
     "fftoc_plan_routine": ([_vp, _int, _u32, ctypes.POINTER(ctypes.c_char_p)], _int),
     "fftoc_plan_set_routine": ([_vp, _int, _u32, ctypes.c_char_p], _int),
+    "fftoc_plan_workgroups_per_cu": ([_vp], _int),
+    "fftoc_plan_set_workgroups_per_cu": ([_vp, _int], _int),
     "fftoc_batch_routine_name": ([_int, _int, ctypes.POINTER(ctypes.c_char_p)], _int),
     "fftoc_plan_destroy": ([_vp], _int),
     "fftoc_batch_fft_noalloc": ([ctypes.POINTER(Job), _int, _vp], _int),

@@ -133,30 +133,45 @@ int batch_bucket(int64_t batch) {
 struct SizeRegistry {
   std::vector<Variant> v;
   std::atomic<int> choice[4][kBuckets];  // [Family][batch bucket] -> index into v
+  std::atomic<int> per_cu[4][kBuckets];  // workgroups per CU of that choice (0: its occupancy)
   std::vector<double> tuned_us;  // last autotune time per variant (<0: not run),
-                                 // guarded by g_tune_mutex
+                                 // best over the workgroups-per-CU caps tried
+  std::vector<int> tuned_per_cu;  // the cap of that time (0: the occupancy);
+                                  // both guarded by g_tune_mutex
 };
 
 SizeRegistry g_reg[13][2];  // [log2 n][complex64 data]
 std::mutex g_tune_mutex;
 std::once_flag g_reg_once;
 
-// Default choice of fftoc_fft per length, from MI355X measurements of the
-// variants at 256 MiB - 1 GiB batches (replaced by fftoc_autotune /
-// fftoc_tuning_load).
-const char *default_best(int n, int bucket) {
+// Default choice of fftoc_fft per length and batch bucket -- variant and
+// workgroups-per-CU cap (0: the kernel's occupancy) -- from MI355X
+// measurements of the variants at 2 MiB - 1 GiB batches, buffers rotating
+// past the 256 MB last-level cache (replaced by fftoc_autotune /
+// fftoc_tuning_load).  The caps pay only on large batches: fewer resident
+// workgroups per CU (or one persistent workgroup looping over tiles) gave
+// +1.5-5 % from 128 MiB up, and lost 10-20 % at 32 MiB.
+struct Default {
+  const char *name;
+  int per_cu;
+};
+Default default_best(int n, int bucket) {
   switch (n) {
     case 64:  // up to 2^16 signals the persistent form hides the launch tail
-      return bucket <= 16 ? "radix:8x8/persistent/S32" : "radix:8x8/tile/S16";
-    case 8: return "radix:8/lanes/S128";
-    case 16: return "radix:16/lanes/S64";
-    case 32: return "radix:32/lanes/S32";
-    case 128: return "radix:16x8/tile/S8";
-    case 256: return "radix:16x16/tile/S4";
-    case 512: return "radix:8x8x8/tile/S1";
-    case 1024: return "radix:8x8x16/tile/S2";
-    case 2048: return "radix:8x16x16/tile/S1";
-    default: return "radix:8x8x8x8/pipelined/S1";
+      return {bucket <= 16 ? "radix:8x8/persistent/S32" : "radix:8x8/tile/S16", 0};
+    case 8: return {"radix:8/lanes/S128", 0};
+    case 16: return {"radix:16/lanes/S64", 0};
+    case 32: return {"radix:32/lanes/S32", 0};
+    case 128: return {"radix:16x8/tile/S8", bucket >= 16 ? 6 : 0};
+    case 256: return {"radix:16x16/tile/S4", bucket >= 15 ? 6 : 0};
+    case 512:
+      return bucket >= 14 ? Default{"radix:8x8x8/pipelined/S4", 1}
+                          : Default{"radix:8x8x8/tile/S1", 0};
+    case 1024:
+      return bucket >= 13 ? Default{"radix:8x8x8x2/pipelined/S1", 3}
+                          : Default{"radix:8x8x16/tile/S2", 0};
+    case 2048: return {"radix:8x16x16/tile/S1", 0};
+    default: return {"radix:8x8x8x8/pipelined/S1", 0};
   }
 }
 
@@ -176,6 +191,7 @@ void build_registry() {
     for (int i = 0; i < nt; ++i) r.v.push_back(t[i]);
     for (int i = 0; i < nx; ++i) r.v.push_back(x[i]);
     r.tuned_us.assign(r.v.size(), -1.0);
+    r.tuned_per_cu.assign(r.v.size(), 0);
     const Family fams[4] = {Family::Tensor, Family::Radix8, Family::Radix16, Family::Best};
     for (Family f : fams) {
       int first = -1;
@@ -186,16 +202,21 @@ void build_registry() {
         }
       }
       for (int b = 0; b < kBuckets; ++b) {
-        int pick = first;
-        const char *want = c64 ? nullptr
-                           : f == Family::Best   ? default_best(n, b)
-                           : f == Family::Tensor ? default_tensor(n, b)
-                                                 : nullptr;
-        if (want) {
+        int pick = first, cap = 0;
+        // fftoc_fft's default; the radix families take it too where it is
+        // one of their variants
+        Default want{nullptr, 0};
+        if (!c64 && f != Family::Tensor) want = default_best(n, b);
+        if (!c64 && f == Family::Tensor) want.name = default_tensor(n, b);
+        if (want.name) {
           for (size_t i = 0; i < r.v.size(); ++i)
-            if (std::strcmp(r.v[i].name, want) == 0) pick = (int)i;
+            if (std::strcmp(r.v[i].name, want.name) == 0 && (r.v[i].families & family_bit(f))) {
+              pick = (int)i;
+              cap = want.per_cu;
+            }
         }
         r.choice[family_index(f)][b].store(pick);
+        r.per_cu[family_index(f)][b].store(cap);
       }
     }
   }
@@ -209,8 +230,8 @@ SizeRegistry *registry(int n, bool c64 = false) {
 
 int check_and_fill(const void *in, void *out, int64_t batch, int n, uint32_t flags,
                    void *stream, LaunchArgs &a) {
-  const uint32_t known =
-      FFTOC_INVERSE | FFTOC_NORMALIZE | FFTOC_PREC_MASK | FFTOC_DATA_C64 | FFTOC_TILE_MASK;
+  const uint32_t known = FFTOC_INVERSE | FFTOC_NORMALIZE | FFTOC_PREC_MASK | FFTOC_DATA_C64 |
+                         FFTOC_TILE_MASK | FFTOC_WGS_PER_CU_MASK;
   if (flags & ~known) return FFTOC_ERROR_INVALID_FLAGS;
   const uint32_t prec = flags & FFTOC_PREC_MASK;
   if (prec != FFTOC_PREC_FP64 && prec != FFTOC_PREC_FP32 && prec != FFTOC_PREC_BF16X3)
@@ -237,6 +258,8 @@ int check_and_fill(const void *in, void *out, int64_t batch, int n, uint32_t fla
   a.precision = prec;
   a.twiddles = nullptr;
   a.c64 = (flags & FFTOC_DATA_C64) != 0;
+  a.per_cu = (int)((flags & FFTOC_WGS_PER_CU_MASK) >> FFTOC_WGS_PER_CU_SHIFT);
+  a.occupancy = nullptr;
   return FFTOC_SUCCESS;
 }
 
@@ -269,6 +292,9 @@ int run(Family fam, const void *in, void *out, int64_t batch, int n, uint32_t fl
   int pick = variant >= 0 ? variant : reg->choice[family_index(fam)][batch_bucket(batch)].load();
   if (pick < 0 || pick >= (int)reg->v.size())
     return a.c64 ? FFTOC_ERROR_UNSUPPORTED : FFTOC_ERROR_INVALID_SIZE;
+  // the tuned workgroups-per-CU cap belongs to the tuned choice
+  if (variant < 0 && a.per_cu == 0 && a.tile == 0)
+    a.per_cu = reg->per_cu[family_index(fam)][batch_bucket(batch)].load();
   if (a.tile > 0 && reg->v[pick].tile != a.tile) {
     // FFTOC_TILE(s): the family's variant with s signals per workgroup (the
     // tuned choice when it has that tile, else the first listed)
@@ -292,8 +318,11 @@ int run(Family fam, const void *in, void *out, int64_t batch, int n, uint32_t fl
 }
 
 // Times every variant of length n and data type on zero-filled scratch
-// buffers (FFT timing does not depend on the data) and makes the fastest
-// variant of each family the choice of that family's entry point.
+// buffers (FFT timing does not depend on the data), at its occupancy and at
+// each smaller workgroups-per-CU cap, and makes the fastest (variant, cap)
+// of each family the choice of that family's entry point.
+constexpr int kMaxSweptPerCu = 16;
+constexpr double kCapMargin = 0.01;
 int autotune(int n, bool c64, int64_t batch, int reps, void *stream_p) {
   SizeRegistry *reg = registry(n, c64);
   if (!reg) return FFTOC_ERROR_INVALID_SIZE;
@@ -315,24 +344,43 @@ int autotune(int n, bool c64, int64_t batch, int reps, void *stream_p) {
       hipEventCreate(&e1) != hipSuccess)
     rc = FFTOC_ERROR_DEVICE;
   std::vector<double> us(reg->v.size(), -1.0);
+  std::vector<int> caps(reg->v.size(), 0);
   for (size_t i = 0; i < reg->v.size() && rc == FFTOC_SUCCESS; ++i) {
     LaunchArgs a{in,     out,   batch, c64 ? g_dev[dev].twiddles32 : g_dev[dev].twiddles,
-                 g_dev[dev].twiddles32, stream, false, 1.0, 0, FFTOC_PREC_FP64, c64};
-    if (reg->v[i].launch(a) != FFTOC_SUCCESS) continue;  // warm-up
-    std::vector<float> t;
-    for (int r = 0; r < reps; ++r) {
-      float ms = 0.f;
-      if (hipEventRecord(e0, stream) != hipSuccess || reg->v[i].launch(a) != FFTOC_SUCCESS ||
-          hipEventRecord(e1, stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
-          hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
-        rc = FFTOC_ERROR_DEVICE;
-        break;
+                 g_dev[dev].twiddles32, stream, false, 1.0, 0, FFTOC_PREC_FP64, c64, 0, nullptr};
+    // workgroups per CU: the kernel's occupancy (0), then every smaller cap
+    // up to kMaxSweptPerCu (the reference's blocks-per-SM sweep)
+    int occ = 0;
+    a.occupancy = &occ;
+    if (reg->v[i].launch(a) != FFTOC_SUCCESS) occ = 0;
+    a.occupancy = nullptr;
+    const int last = std::min(occ - 1, kMaxSweptPerCu);
+    for (int cap = 0; cap <= last || cap == 0; ++cap) {
+      a.per_cu = cap;
+      if (reg->v[i].launch(a) != FFTOC_SUCCESS) {  // warm-up
+        if (cap == 0) break;
+        continue;
       }
-      t.push_back(ms);
+      std::vector<float> t;
+      for (int r = 0; r < reps; ++r) {
+        float ms = 0.f;
+        if (hipEventRecord(e0, stream) != hipSuccess || reg->v[i].launch(a) != FFTOC_SUCCESS ||
+            hipEventRecord(e1, stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+            hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+          rc = FFTOC_ERROR_DEVICE;
+          break;
+        }
+        t.push_back(ms);
+      }
+      if (rc != FFTOC_SUCCESS || t.empty()) break;
+      std::sort(t.begin(), t.end());
+      const double m = 1000.0 * t[t.size() / 2];
+      // a cap must win by more than the timing noise to replace the occupancy
+      if (us[i] < 0 || m < us[i] * (1.0 - kCapMargin)) {
+        us[i] = m;
+        caps[i] = cap;
+      }
     }
-    if (rc != FFTOC_SUCCESS || t.empty()) break;
-    std::sort(t.begin(), t.end());
-    us[i] = 1000.0 * t[t.size() / 2];
   }
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);
@@ -343,6 +391,7 @@ int autotune(int n, bool c64, int64_t batch, int reps, void *stream_p) {
   {
     std::lock_guard<std::mutex> lock(g_tune_mutex);
     reg->tuned_us = us;
+    reg->tuned_per_cu = caps;
   }
   const Family fams[4] = {Family::Tensor, Family::Radix8, Family::Radix16, Family::Best};
   for (Family f : fams) {
@@ -350,7 +399,12 @@ int autotune(int n, bool c64, int64_t batch, int reps, void *stream_p) {
     for (size_t i = 0; i < us.size(); ++i)
       if ((reg->v[i].families & family_bit(f)) && us[i] >= 0 && (best < 0 || us[i] < us[best]))
         best = (int)i;
-    if (best >= 0) reg->choice[family_index(f)][batch_bucket(batch)].store(best);
+    if (best >= 0) {
+      // a call running concurrently may pair the old choice with the new
+      // cap (or the reverse): that changes its speed, never its result
+      reg->per_cu[family_index(f)][batch_bucket(batch)].store(caps[best]);
+      reg->choice[family_index(f)][batch_bucket(batch)].store(best);
+    }
   }
   return FFTOC_SUCCESS;
 }
@@ -493,9 +547,47 @@ int fftoc_set_variant(int n, uint32_t flags, int64_t batch, int variant) {
   if (!r) return FFTOC_ERROR_INVALID_SIZE;
   if (variant < 0 || variant >= (int)r->v.size()) return FFTOC_ERROR_INVALID_HANDLE;
   for (int b = 0; b < fftoc::kBuckets; ++b)
-    if (batch < 0 || b == fftoc::batch_bucket(batch))
+    if (batch < 0 || b == fftoc::batch_bucket(batch)) {
       r->choice[fftoc::family_index(Family::Best)][b].store(variant);
+      r->per_cu[fftoc::family_index(Family::Best)][b].store(0);
+    }
   return FFTOC_SUCCESS;
+}
+
+int fftoc_get_workgroups_per_cu(int n, uint32_t flags, int64_t batch) {
+  fftoc::SizeRegistry *r = fftoc::registry(n, dtype_c64(flags));
+  return r ? r->per_cu[fftoc::family_index(Family::Best)][fftoc::batch_bucket(batch)].load() : -1;
+}
+
+int fftoc_set_workgroups_per_cu(int n, uint32_t flags, int64_t batch, int per_cu) {
+  fftoc::SizeRegistry *r = fftoc::registry(n, dtype_c64(flags));
+  if (!r) return FFTOC_ERROR_INVALID_SIZE;
+  if (per_cu < 0 || per_cu > fftoc::kMaxPerCu) return FFTOC_ERROR_INVALID_FLAGS;
+  for (int b = 0; b < fftoc::kBuckets; ++b)
+    if (batch < 0 || b == fftoc::batch_bucket(batch))
+      r->per_cu[fftoc::family_index(Family::Best)][b].store(per_cu);
+  return FFTOC_SUCCESS;
+}
+
+int fftoc_variant_occupancy(int n, uint32_t flags, int variant) {
+  fftoc::SizeRegistry *r = fftoc::registry(n, dtype_c64(flags));
+  if (!r || variant < 0 || variant >= (int)r->v.size()) return -1;
+  int dev = 0;
+  if (fftoc::init_current_device(&dev) != FFTOC_SUCCESS) return -1;
+  int occ = 0;
+  fftoc::LaunchArgs a{};
+  a.precision = FFTOC_PREC_FP64;
+  a.c64 = dtype_c64(flags);
+  a.scale = 1.0;
+  a.occupancy = &occ;
+  return r->v[variant].launch(a) == FFTOC_SUCCESS && occ > 0 ? occ : -1;
+}
+
+int fftoc_variant_per_cu(int n, uint32_t flags, int variant) {
+  fftoc::SizeRegistry *r = fftoc::registry(n, dtype_c64(flags));
+  if (!r || variant < 0 || variant >= (int)r->v.size()) return -1;
+  std::lock_guard<std::mutex> lock(fftoc::g_tune_mutex);
+  return r->tuned_us[variant] < 0 ? -1 : r->tuned_per_cu[variant];
 }
 
 int fftoc_autotune(int n, uint32_t flags, int64_t batch, int reps, void *stream) {
@@ -516,19 +608,25 @@ int fftoc_variant_tile(int n, uint32_t flags, int variant) {
 }
 
 // Text format, one line per (entry point, length, data type, batch bucket):
-//   <family> <n> <c128|c64> <floor(log2 batch)> <variant name up to end of line>
+//   <family> <n> <c128|c64> <floor(log2 batch)> [wgs=<k>] <variant name up to end of line>
+// wgs=<k>: at most k workgroups per CU (absent: the kernel's occupancy).
 int fftoc_tuning_save(const char *path) {
   if (!path) return FFTOC_ERROR_NULL_POINTER;
   FILE *f = std::fopen(path, "w");
   if (!f) return FFTOC_ERROR_INVALID_HANDLE;
-  std::fprintf(f, "# fftoc tuning table: family n dtype log2(batch) variant\n");
+  std::fprintf(f, "# fftoc tuning table: family n dtype log2(batch) [wgs=per-CU cap] variant\n");
   for (int k = 3; k <= 12; ++k) {
     for (int c64 = 0; c64 < 2; ++c64) {
       fftoc::SizeRegistry *r = fftoc::registry(1 << k, c64 != 0);
       for (int fi = 0; fi < 4; ++fi) {
         for (int b = 0; b < fftoc::kBuckets; ++b) {
           const int c = r->choice[fi][b].load();
-          if (c >= 0)
+          const int cap = r->per_cu[fi][b].load();
+          if (c < 0) continue;
+          if (cap > 0)
+            std::fprintf(f, "%s %d %s %d wgs=%d %s\n", fftoc::kFamilyNames[fi], 1 << k,
+                         kDtypeNames[c64], b, cap, r->v[c].name);
+          else
             std::fprintf(f, "%s %d %s %d %s\n", fftoc::kFamilyNames[fi], 1 << k, kDtypeNames[c64],
                          b, r->v[c].name);
         }
@@ -561,6 +659,15 @@ int fftoc_tuning_load(const char *path) {
     int c64 = -1;
     for (int i = 0; i < 2; ++i)
       if (std::strcmp(dt, kDtypeNames[i]) == 0) c64 = i;
+    int cap = 0, cap_used = 0;
+    if (std::strncmp(line + used, "wgs=", 4) == 0) {
+      if (std::sscanf(line + used, "wgs=%d %n", &cap, &cap_used) != 1 || cap_used == 0 ||
+          cap < 0 || cap > fftoc::kMaxPerCu) {
+        rc = FFTOC_ERROR_INVALID_HANDLE;
+        continue;
+      }
+      used += cap_used;
+    }
     std::string name(line + used);
     while (!name.empty() && (name.back() == '\n' || name.back() == '\r')) name.pop_back();
     fftoc::SizeRegistry *r = c64 >= 0 ? fftoc::registry(n, c64 != 0) : nullptr;
@@ -582,6 +689,7 @@ int fftoc_tuning_load(const char *path) {
       rc = FFTOC_ERROR_INVALID_HANDLE;
       continue;
     }
+    r->per_cu[fi][bucket].store(cap);
     r->choice[fi][bucket].store(match);
   }
   std::fclose(f);

@@ -101,13 +101,19 @@ int batch_routine_threads(int id);
 int batch_routine_n(int id);
 const char *batch_routine_name(int id);
 int batch_routine_signals(int id);
-int batch_resident_blocks();
+// workgroups of the mixed-size kernel resident on the device, at most
+// per_cu per CU when per_cu > 0; batch_occupancy: its workgroups per CU
+int batch_resident_blocks(int per_cu = 0);
+int batch_occupancy();
 int launch_batch(const JobDev *d_jobs, int njobs, int64_t total_items, unsigned long long *d_ctr,
                  const void *tw64, const void *tw32, int grid, hipStream_t stream);
 // batch_one*.hip: the kernel specialized to one routine and data type (every
 // job of the launch runs that routine on that data type) and direction (dir
 // 0: every job forward, 1: every job inverse, 2: each job its own)
-int64_t batch_one_resident(int routine, bool c64, int dir);
+// batch_one_resident: the grid, per_cu workgroups per CU (0: two) or the
+// occupancy if lower; batch_one_occupancy: workgroups per CU and the CUs
+int64_t batch_one_resident(int routine, bool c64, int dir, int per_cu = 0);
+int batch_one_occupancy(int routine, bool c64, int dir, int *cus);
 int launch_batch_one(int routine, bool c64, int dir, const JobDev *d_jobs, int njobs,
                      int64_t total_items, unsigned long long *d_ctr, const void *tw, int grid,
                      hipStream_t stream);

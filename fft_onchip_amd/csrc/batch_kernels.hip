@@ -453,9 +453,10 @@ int batch_routine_signals(int id) {
   return (id >= 0 && id < kNumRoutines) ? kRoutines[id].signals_per_tile : 0;
 }
 
-int batch_resident_blocks() {
-  return (int)resident_workgroups<batch_kernel, kBatchThreads>();
+int batch_resident_blocks(int per_cu) {
+  return (int)resident_workgroups<batch_kernel, kBatchThreads>(per_cu);
 }
+int batch_occupancy() { return occupancy<batch_kernel, kBatchThreads>().per_cu; }
 
 // ---- allocation-free batches --------------------------------------------------
 // Scratch of fftoc_batch_fft_noalloc: the job descriptors (JobDev[32]) and the

@@ -4,6 +4,8 @@
 #define FFTOC_ONE_DIR 2
 #include "batch_one.hpp"
 
+#include <algorithm>
+
 namespace fftoc {
 namespace {
 const void *const kOneKernels[][2] = {FFTOC_ROUTINES(FFTOC_ONE)};
@@ -22,7 +24,7 @@ const void *one_kernel(int routine, bool c64, int dir) {
   return kOneKernels[routine][c64 ? 1 : 0];
 }
 
-int64_t batch_one_resident(int routine, bool c64, int dir) {
+int batch_one_occupancy(int routine, bool c64, int dir, int *cus) {
   const void *k = one_kernel(routine, c64, dir);
   if (!k || dir < 0 || dir > 2) return 0;
   static std::atomic<int64_t> cache[kNumRoutines][2][3][kMaxDevices];
@@ -31,24 +33,35 @@ int64_t batch_one_resident(int routine, bool c64, int dir) {
   const int t = c64 ? 1 : 0;
   if (dev >= 0 && dev < kMaxDevices) {
     const int64_t v = cache[routine][t][dir][dev].load(std::memory_order_relaxed);
-    if (v > 0) return v;
+    if (v > 0) {
+      *cus = (int)(v & 0xffffffff);
+      return (int)(v >> 32);
+    }
   }
-  int per_cu = 0, cus = 0;
+  int per_cu = 0, n = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, kRoutines[routine].threads, 0) !=
           hipSuccess ||
       per_cu <= 0)
     per_cu = 1;
-  // at most two workgroups per CU, as the mixed-size kernel: more (the light
-  // routines allow up to eight) measured slower on every length, the claims
-  // and the launch's tail spread over more workgroups
-  if (per_cu > kOnePerCu) per_cu = kOnePerCu;
   if (dev < 0 ||
-      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-      cus <= 0)
-    cus = 256;
-  const int64_t n = (int64_t)per_cu * cus;
-  if (dev >= 0 && dev < kMaxDevices) cache[routine][t][dir][dev].store(n, std::memory_order_relaxed);
-  return n;
+      hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      n <= 0)
+    n = 256;
+  if (dev >= 0 && dev < kMaxDevices)
+    cache[routine][t][dir][dev].store(((int64_t)per_cu << 32) | n, std::memory_order_relaxed);
+  *cus = n;
+  return per_cu;
+}
+
+int64_t batch_one_resident(int routine, bool c64, int dir, int per_cu) {
+  int cus = 0;
+  const int occ = batch_one_occupancy(routine, c64, dir, &cus);
+  if (occ <= 0) return 0;
+  // default: at most two workgroups per CU, as the mixed-size kernel: more
+  // (the light routines allow up to eight) measured slower on every length,
+  // the claims and the launch's tail spread over more workgroups
+  const int want = per_cu > 0 ? per_cu : kOnePerCu;
+  return (int64_t)std::min(want, occ) * cus;
 }
 
 int launch_batch_one(int routine, bool c64, int dir, const JobDev *d_jobs, int njobs,

@@ -350,11 +350,15 @@ __global__ __launch_bounds__(64 * kWaves) void tensor_persistent_kernel(
 template <class U, int G>
 int launch_tensor_persistent(const LaunchArgs &a) {
   if (a.precision != FFTOC_PREC_FP64) return FFTOC_ERROR_UNSUPPORTED;
+  if (a.occupancy) {
+    *a.occupancy = occupancy<tensor_persistent_kernel<U, G, false, false>, 64 * kWaves>().per_cu;
+    return FFTOC_SUCCESS;
+  }
   if (a.batch <= 0) return FFTOC_SUCCESS;
   constexpr int SPG = U::kSignals * G;
   const int64_t groups = (a.batch + SPG - 1) / SPG;
   const int64_t resident =
-      resident_workgroups<tensor_persistent_kernel<U, G, false, false>, 64 * kWaves>();
+      resident_workgroups<tensor_persistent_kernel<U, G, false, false>, 64 * kWaves>(a.per_cu);
   const int64_t need = (groups + kWaves - 1) / kWaves;
   const dim3 gd((unsigned)(need < resident ? need : resident)), bd(64 * kWaves);
   const c2 *in = static_cast<const c2 *>(a.in);
@@ -384,7 +388,7 @@ __global__ __launch_bounds__(64 * kWaves) void tensor256_kernel(const c2 *in, c2
                                                                 const c2 *__restrict__ tw,
                                                                 double scale) {
   __shared__ c2 lds[kWaves * 256];
-  const int lane = lane_id(), g = lane >> 4, j = lane & 15;
+  const int lane = lane_id();
   const int64_t sw = wave_global() * G;
   if (sw >= batch) return;
   c2 *buf = lds + (threadIdx.x >> 6) * 256;
@@ -725,7 +729,7 @@ __global__ __launch_bounds__(64 * WAVES) void tensor4096h_kernel(const c2 *in, c
   constexpr int kRowsPerWave = 16 / WAVES;
   __shared__ c2 lds[4096];
   const int t = threadIdx.x;
-  const int lane = lane_id(), g = lane >> 4, j = lane & 15;
+  const int lane = lane_id();
   const int w = t >> 6;
   const int64_t s = blockIdx.x;
   if (s >= batch) return;  // uniform over the workgroup
@@ -795,8 +799,12 @@ int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 #define FFTOC_PERSISTENT_LAUNCHER(FN, KERNEL, UNITS_PER_WG)                                   \
   int FN(const LaunchArgs &a) {                                                               \
     if (a.precision != FFTOC_PREC_FP64) return FFTOC_ERROR_UNSUPPORTED;                       \
+    if (a.occupancy) {                                                                        \
+      *a.occupancy = occupancy<KERNEL<false, false>, 64 * kWaves>().per_cu;                   \
+      return FFTOC_SUCCESS;                                                                   \
+    }                                                                                         \
     if (a.batch <= 0) return FFTOC_SUCCESS;                                                   \
-    const int64_t resident = resident_workgroups<KERNEL<false, false>, 64 * kWaves>();        \
+    const int64_t resident = resident_workgroups<KERNEL<false, false>, 64 * kWaves>(a.per_cu); \
     const int64_t need = ceil_div(a.batch, UNITS_PER_WG);                                     \
     const dim3 gd((unsigned)(need < resident ? need : resident)), bd(64 * kWaves);            \
     const c2 *in = static_cast<const c2 *>(a.in);                                             \
@@ -823,24 +831,30 @@ FFTOC_PERSISTENT_LAUNCHER(launch_t512p, tensor512p_kernel, kWaves)
 #define FFTOC_TENSOR_LAUNCHER(FN, KERNEL, UNITS_PER_WG, BLOCK, ...)                              \
   int FN(const LaunchArgs &a) {                                                                \
     if (a.precision != FFTOC_PREC_FP64) return FFTOC_ERROR_UNSUPPORTED;                        \
+    constexpr auto K0 = KERNEL<__VA_ARGS__ __VA_OPT__(, ) false, false>;                       \
+    if (a.occupancy) {                                                                         \
+      *a.occupancy = occupancy<K0, (BLOCK)>().per_cu;                                          \
+      return FFTOC_SUCCESS;                                                                    \
+    }                                                                                          \
     const c2 *in = static_cast<const c2 *>(a.in);                                              \
     c2 *out = static_cast<c2 *>(a.out);                                                        \
     const c2 *tw = static_cast<const c2 *>(a.twiddles);                                        \
     const int64_t grid = ceil_div(a.batch, (UNITS_PER_WG));                                    \
     if (grid > 0x7fffffffLL) return FFTOC_ERROR_INVALID_BATCH;                                 \
     const dim3 gd((unsigned)grid), bd(BLOCK);                                                \
+    const size_t lds = occupancy_lds<K0, (BLOCK)>(a.per_cu);                                   \
     const bool sc = a.scale != 1.0;                                                            \
     if (a.inverse && sc)                                                                       \
-      hipLaunchKernelGGL((KERNEL<__VA_ARGS__ __VA_OPT__(, ) true, true>), gd, bd, 0, a.stream, \
-                         in, out, a.batch, tw, a.scale);                                       \
+      hipLaunchKernelGGL((KERNEL<__VA_ARGS__ __VA_OPT__(, ) true, true>), gd, bd, lds,         \
+                         a.stream, in, out, a.batch, tw, a.scale);                             \
     else if (a.inverse)                                                                        \
-      hipLaunchKernelGGL((KERNEL<__VA_ARGS__ __VA_OPT__(, ) true, false>), gd, bd, 0,          \
+      hipLaunchKernelGGL((KERNEL<__VA_ARGS__ __VA_OPT__(, ) true, false>), gd, bd, lds,        \
                          a.stream, in, out, a.batch, tw, a.scale);                             \
     else if (sc)                                                                               \
-      hipLaunchKernelGGL((KERNEL<__VA_ARGS__ __VA_OPT__(, ) false, true>), gd, bd, 0,          \
+      hipLaunchKernelGGL((KERNEL<__VA_ARGS__ __VA_OPT__(, ) false, true>), gd, bd, lds,        \
                          a.stream, in, out, a.batch, tw, a.scale);                             \
     else                                                                                       \
-      hipLaunchKernelGGL((KERNEL<__VA_ARGS__ __VA_OPT__(, ) false, false>), gd, bd, 0,         \
+      hipLaunchKernelGGL((KERNEL<__VA_ARGS__ __VA_OPT__(, ) false, false>), gd, bd, lds,       \
                          a.stream, in, out, a.batch, tw, a.scale);                             \
     return hipGetLastError() == hipSuccess ? FFTOC_SUCCESS : FFTOC_ERROR_LAUNCH;               \
   }

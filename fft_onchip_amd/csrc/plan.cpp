@@ -24,6 +24,7 @@ struct fftoc_plan_t {
   int one_routine = -1;                 // the routine of every job, if they share
   bool one_c64 = false;                 // one (routine, data type); else -1
   int one_dir = 2;                      // 0 / 1: every job forward / inverse
+  int per_cu = 0;                       // workgroups per CU (0: the default)
   int routine[13][2];                   // [log2 n][complex64] -> routine id
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -33,6 +34,8 @@ struct fftoc_plan_t {
 
 namespace fftoc {
 namespace {
+
+constexpr int kMaxPlanPerCu = 16;  // largest workgroups-per-CU cap the autotuner tries
 
 bool pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
 int lg(int n) { return 31 - __builtin_clz((unsigned)n); }
@@ -165,9 +168,9 @@ int layout(fftoc_plan_t *p) {
     if (hipMemcpy(p->d_jobs, p->host.data(), bytes, hipMemcpyHostToDevice) != hipSuccess)
       return FFTOC_ERROR_DEVICE;
   }
-  const int64_t resident = p->one_routine >= 0
-                               ? batch_one_resident(p->one_routine, p->one_c64, p->one_dir)
-                               : batch_resident_blocks();
+  const int64_t resident =
+      p->one_routine >= 0 ? batch_one_resident(p->one_routine, p->one_c64, p->one_dir, p->per_cu)
+                          : batch_resident_blocks(p->per_cu);
   p->grid = (int)std::max<int64_t>(1, std::min<int64_t>(resident, p->total_items));
   if (p->exec) {
     (void)hipGraphExecDestroy(p->exec);
@@ -337,10 +340,46 @@ int fftoc_plan_autotune(fftoc_plan p, int reps, void *stream_p) {
     p->routine[k][c] = best;
     if (rc != FFTOC_SUCCESS) break;
   }
+  // then the workgroups per CU of the chosen kernel: the default, then every
+  // cap up to its occupancy; a cap must win by 1 % (timing noise)
+  if (rc == FFTOC_SUCCESS) {
+    const int keep = p->per_cu;
+    p->per_cu = 0;
+    rc = layout(p);
+    int occ = 0, cus = 0;
+    if (rc == FFTOC_SUCCESS)
+      occ = p->one_routine >= 0 ? batch_one_occupancy(p->one_routine, p->one_c64, p->one_dir, &cus)
+                                : batch_occupancy();
+    double best_us = 1e300;
+    int best = 0;
+    for (int cap = 0; cap <= std::min(occ, kMaxPlanPerCu) && rc == FFTOC_SUCCESS; ++cap) {
+      p->per_cu = cap;
+      if ((rc = layout(p)) != FFTOC_SUCCESS) break;
+      double us = 0;
+      if ((rc = time_plan(&us)) != FFTOC_SUCCESS) break;
+      if (cap == 0 || us < best_us * 0.99) {
+        best_us = us;
+        best = cap;
+      }
+    }
+    p->per_cu = rc == FFTOC_SUCCESS ? best : keep;
+  }
   const int lrc = layout(p);
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   return rc != FFTOC_SUCCESS ? rc : lrc;
+}
+
+int fftoc_plan_workgroups_per_cu(fftoc_plan p) { return p ? p->per_cu : -1; }
+
+int fftoc_plan_set_workgroups_per_cu(fftoc_plan p, int per_cu) {
+  using namespace fftoc;
+  if (!p) return FFTOC_ERROR_INVALID_HANDLE;
+  if (per_cu < 0 || per_cu > kMaxPerCu) return FFTOC_ERROR_INVALID_FLAGS;
+  const int rc = check_device(p);
+  if (rc != FFTOC_SUCCESS) return rc;
+  p->per_cu = per_cu;
+  return layout(p);
 }
 
 int fftoc_plan_routine(fftoc_plan p, int n, uint32_t flags, const char **name) {

@@ -74,6 +74,11 @@ __global__ __launch_bounds__(kLaneThreads) void lane_fft_kernel(
 
 template <int N, int C, class CT>
 int launch_lane(const LaunchArgs &a) {
+  constexpr auto K = lane_fft_kernel<N, C, false, CT>;
+  if (a.occupancy) {
+    *a.occupancy = occupancy<K, kLaneThreads>().per_cu;
+    return FFTOC_SUCCESS;
+  }
   constexpr int64_t per_block = (int64_t)kLaneThreads * C;  // elements
   const int64_t blocks = (a.batch * N + per_block - 1) / per_block;
   if (blocks > 0x7fffffffLL) return FFTOC_ERROR_INVALID_BATCH;
@@ -81,12 +86,13 @@ int launch_lane(const LaunchArgs &a) {
   CT *out = static_cast<CT *>(a.out);
   const CT *tw = static_cast<const CT *>(a.twiddles);
   const real_t<CT> scale = (real_t<CT>)a.scale;
+  const size_t lds = occupancy_lds<K, kLaneThreads>(a.per_cu);
   if (a.inverse)
     hipLaunchKernelGGL((lane_fft_kernel<N, C, true, CT>), dim3((unsigned)blocks),
-                       dim3(kLaneThreads), 0, a.stream, in, out, a.batch, tw, scale);
+                       dim3(kLaneThreads), lds, a.stream, in, out, a.batch, tw, scale);
   else
     hipLaunchKernelGGL((lane_fft_kernel<N, C, false, CT>), dim3((unsigned)blocks),
-                       dim3(kLaneThreads), 0, a.stream, in, out, a.batch, tw, scale);
+                       dim3(kLaneThreads), lds, a.stream, in, out, a.batch, tw, scale);
   return hipGetLastError() == hipSuccess ? FFTOC_SUCCESS : FFTOC_ERROR_LAUNCH;
 }
 
@@ -94,6 +100,18 @@ int launch_lane(const LaunchArgs &a) {
   Variant { NAME, FAM, &launch_lane<N, C, CT>, kLaneThreads * (C) / (N) }
 
 enum class Form { OneTile, Staged, Persistent, Pipelined };
+
+// The forward kernel of a split form (occupancy queries; the inverse
+// instantiation uses the same resources).
+template <class SP, int S, Form F, class CT>
+constexpr auto split_kernel() {
+  if constexpr (F == Form::Persistent)
+    return stockham_persistent_kernel<SP, S, false>;
+  else if constexpr (F == Form::Pipelined)
+    return stockham_pipelined_kernel<SP, S, false>;
+  else
+    return stockham_kernel<SP, S, false, F == Form::Staged, CT>;
+}
 
 template <class SP, int S, Form F, class CT = c2>
 int launch_split(const LaunchArgs &a) {
@@ -104,9 +122,14 @@ int launch_split(const LaunchArgs &a) {
   CT *out = static_cast<CT *>(a.out);
   const CT *tw = static_cast<const CT *>(a.twiddles);
   const real_t<CT> scale = (real_t<CT>)a.scale;
+  if (a.occupancy) {
+    *a.occupancy = occupancy<split_kernel<SP, S, F, CT>(), threads>().per_cu;
+    return FFTOC_SUCCESS;
+  }
   if constexpr (F == Form::Persistent) {
     static_assert(sizeof(CT) == 16, "persistent form is FP64 only");
-    const int64_t resident = resident_workgroups<stockham_persistent_kernel<SP, S, false>, threads>();
+    const int64_t resident =
+        resident_workgroups<stockham_persistent_kernel<SP, S, false>, threads>(a.per_cu);
     const unsigned grid = (unsigned)(ntiles < resident ? ntiles : resident);
     if (a.inverse)
       hipLaunchKernelGGL((stockham_persistent_kernel<SP, S, true>), dim3(grid), dim3(threads), 0,
@@ -116,7 +139,8 @@ int launch_split(const LaunchArgs &a) {
                          a.stream, in, out, a.batch, tw, a.scale);
   } else if constexpr (F == Form::Pipelined) {
     static_assert(sizeof(CT) == 16, "pipelined form is FP64 only");
-    const int64_t resident = resident_workgroups<stockham_pipelined_kernel<SP, S, false>, threads>();
+    const int64_t resident =
+        resident_workgroups<stockham_pipelined_kernel<SP, S, false>, threads>(a.per_cu);
     const unsigned grid = (unsigned)(ntiles < resident ? ntiles : resident);
     if (a.inverse)
       hipLaunchKernelGGL((stockham_pipelined_kernel<SP, S, true>), dim3(grid), dim3(threads), 0,
@@ -126,12 +150,13 @@ int launch_split(const LaunchArgs &a) {
                          a.stream, in, out, a.batch, tw, a.scale);
   } else {
     constexpr bool staged = (F == Form::Staged);
+    const size_t lds = occupancy_lds<split_kernel<SP, S, F, CT>(), threads>(a.per_cu);
     if (a.inverse)
       hipLaunchKernelGGL((stockham_kernel<SP, S, true, staged, CT>), dim3((unsigned)ntiles),
-                         dim3(threads), 0, a.stream, in, out, a.batch, tw, scale);
+                         dim3(threads), lds, a.stream, in, out, a.batch, tw, scale);
     else
       hipLaunchKernelGGL((stockham_kernel<SP, S, false, staged, CT>), dim3((unsigned)ntiles),
-                         dim3(threads), 0, a.stream, in, out, a.batch, tw, scale);
+                         dim3(threads), lds, a.stream, in, out, a.batch, tw, scale);
   }
   return hipGetLastError() == hipSuccess ? FFTOC_SUCCESS : FFTOC_ERROR_LAUNCH;
 }

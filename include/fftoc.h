@@ -80,6 +80,15 @@ typedef enum fftoc_status {
 #define FFTOC_TILE_SHIFT 16
 #define FFTOC_TILE_MASK (0xFFu << FFTOC_TILE_SHIFT)
 #define FFTOC_TILE(s) (((uint32_t)(s) & 0xFFu) << FFTOC_TILE_SHIFT)
+/* at most k workgroups resident per CU (the reference's blocks-per-SM sweep,
+ * src/saturation.cu:15-62), k = 1..63: a persistent kernel launches k per CU;
+ * a kernel with one workgroup per tile requests enough (unused) LDS that no
+ * more than k fit on a CU.  0 = the tuned choice (the defaults cap a few
+ * lengths at large batches; fftoc_autotune sweeps the caps; otherwise the
+ * kernel's occupancy).  Never changes results. */
+#define FFTOC_WGS_PER_CU_SHIFT 24
+#define FFTOC_WGS_PER_CU_MASK (0x3Fu << FFTOC_WGS_PER_CU_SHIFT)
+#define FFTOC_WGS_PER_CU(k) (((uint32_t)(k) & 0x3Fu) << FFTOC_WGS_PER_CU_SHIFT)
 
 /* ---- library ------------------------------------------------------------ */
 /* Builds the FP64 twiddle / DFT-matrix tables on the current device.
@@ -156,19 +165,31 @@ int fftoc_fft_variant(const void *in, void *out, int64_t batch, int n,
  * every bucket). */
 int fftoc_get_variant(int n, uint32_t flags, int64_t batch);
 int fftoc_set_variant(int n, uint32_t flags, int64_t batch, int variant);
-/* Times every variant of length n and data type over `batch` signals
- * (allocates and frees two device scratch buffers, synchronises `stream`) and
- * makes the fastest one of each entry-point family its default for that
- * batch bucket. */
+/* Times every variant of length n and data type over `batch` signals, at
+ * its occupancy and at each smaller workgroups-per-CU cap up to 16
+ * (allocates and frees two device scratch buffers, synchronises `stream`),
+ * and makes the fastest (variant, cap) of each entry-point family its
+ * default for that batch bucket; a cap must beat the occupancy by 1 %. */
 int fftoc_autotune(int n, uint32_t flags, int64_t batch, int reps, void *stream);
 /* Median time in microseconds of a variant in the last autotune of its
- * length and data type (<0: none). */
+ * length and data type, at its best cap (<0: none). */
 double fftoc_variant_time_us(int n, uint32_t flags, int variant);
+/* That best cap (0: the occupancy; -1: not tuned). */
+int fftoc_variant_per_cu(int n, uint32_t flags, int variant);
+/* Resident workgroups per CU of a variant's kernel on the current device:
+ * the upper end of its FFTOC_WGS_PER_CU range (-1 for an unknown variant). */
+int fftoc_variant_occupancy(int n, uint32_t flags, int variant);
+/* Workgroups-per-CU cap fftoc_fft uses with its chosen variant for length n
+ * and data type at this batch size (0: the occupancy), and its override
+ * (batch < 0: every bucket).  fftoc_set_variant resets it to 0. */
+int fftoc_get_workgroups_per_cu(int n, uint32_t flags, int64_t batch);
+int fftoc_set_workgroups_per_cu(int n, uint32_t flags, int64_t batch, int per_cu);
 /* Signals per workgroup of a variant (the FFTOC_TILE value that selects it),
  * or -1 for an unknown variant. */
 int fftoc_variant_tile(int n, uint32_t flags, int variant);
 /* Persisted tuning table, one line per (entry point, length, data type,
- * batch bucket): "<family> <n> <c128|c64> <log2 batch> <variant name>". */
+ * batch bucket): "<family> <n> <c128|c64> <log2 batch> [wgs=<k>] <variant
+ * name>", wgs=<k> for a workgroups-per-CU cap. */
 int fftoc_tuning_save(const char *path);
 int fftoc_tuning_load(const char *path);
 
@@ -198,10 +219,17 @@ int fftoc_plan_execute(fftoc_plan plan, void *stream);
 int fftoc_plan_execute_graph(fftoc_plan plan, void *stream);
 /* Times the candidate tile routines of every (length, data type) in the plan
  * on its own buffers -- `reps` back-to-back launches each, after one warm-up
- * launch -- and keeps the fastest (overwrites the outputs).  A plan whose
- * jobs share one routine, data type and direction runs a kernel specialized
- * to them, and its candidates include the 512-thread routines. */
+ * launch -- and keeps the fastest (overwrites the outputs); then does the
+ * same for the workgroups per CU of the launch (1 up to the kernel's
+ * occupancy).  A plan whose jobs share one routine, data type and direction
+ * runs a kernel specialized to them, and its candidates include the
+ * 512-thread routines. */
 int fftoc_plan_autotune(fftoc_plan plan, int reps, void *stream);
+/* Workgroups per CU of the plan's launch (0 = the default: two, or the
+ * kernel's occupancy if lower), and its override, k = 0..63 (FFTOC_ERROR_
+ * INVALID_FLAGS outside; values above the occupancy act as the occupancy). */
+int fftoc_plan_workgroups_per_cu(fftoc_plan plan);
+int fftoc_plan_set_workgroups_per_cu(fftoc_plan plan, int per_cu);
 /* Name of the tile routine the plan uses for (n, data type of flags). */
 int fftoc_plan_routine(fftoc_plan plan, int n, uint32_t flags, const char **name);
 /* Pins the plan's tile routine for (n, data type of flags) to the named one

@@ -195,3 +195,41 @@ def test_variant_lists_per_data_type_need_no_device(tmp_path):
     bad = tmp_path / "bad.txt"
     bad.write_text("fft 64 c32 3 radix:8x8/tile/S16\nfft 64 3 radix:8x8/tile/S16\n")
     assert lib.fftoc_tuning_load(str(bad).encode()) == 13
+
+
+def test_workgroups_per_cu_setting_and_tuning_lines_need_no_device(tmp_path):
+    """The workgroups-per-CU cap of fftoc_fft's choice (the reference's
+    blocks-per-SM sweep, src/saturation.cu:15-62) is kept per batch bucket,
+    reset by fftoc_set_variant, range-checked, and persisted as a `wgs=<k>`
+    token that fftoc_tuning_load reads back; a malformed token is rejected."""
+    lib = fo.lib()
+    n, dt = 256, fo.DATA_C64
+    assert lib.fftoc_get_workgroups_per_cu(n, dt, 1 << 10) == 0
+    assert lib.fftoc_set_workgroups_per_cu(n, dt, 1 << 10, 64) == 4       # above 63
+    assert lib.fftoc_set_workgroups_per_cu(n, dt, 1 << 10, -1) == 4
+    assert lib.fftoc_set_workgroups_per_cu(100, dt, 1 << 10, 2) == 1      # bad length
+    assert lib.fftoc_set_workgroups_per_cu(n, dt, 1 << 10, 3) == 0
+    assert lib.fftoc_get_workgroups_per_cu(n, dt, 1 << 10) == 3
+    assert lib.fftoc_get_workgroups_per_cu(n, dt, 1 << 11) == 0           # other bucket
+    assert lib.fftoc_get_workgroups_per_cu(n, 0, 1 << 10) == 0            # other data type
+    path = str(tmp_path / "tune.txt").encode()
+    assert lib.fftoc_tuning_save(path) == 0
+    want = f"fft {n} c64 10 wgs=3 "
+    assert any(ln.startswith(want) for ln in open(path)), "cap not persisted"
+    v = lib.fftoc_get_variant(n, dt, 1 << 10)
+    assert lib.fftoc_set_variant(n, dt, 1 << 10, v) == 0
+    assert lib.fftoc_get_workgroups_per_cu(n, dt, 1 << 10) == 0            # reset with the variant
+    assert lib.fftoc_tuning_load(path) == 0
+    assert lib.fftoc_get_workgroups_per_cu(n, dt, 1 << 10) == 3            # restored
+    assert lib.fftoc_get_variant(n, dt, 1 << 10) == v
+    name = fo.variants(n, c64=True)[v]
+    bad = tmp_path / "bad.txt"
+    bad.write_text(f"fft {n} c64 10 wgs=99 {name}\nfft {n} c64 10 wgs= {name}\n")
+    assert lib.fftoc_tuning_load(str(bad).encode()) == 13
+    assert lib.fftoc_get_workgroups_per_cu(n, dt, 1 << 10) == 3            # bad lines skipped
+    assert lib.fftoc_set_workgroups_per_cu(n, dt, -1, 0) == 0
+    assert lib.fftoc_plan_workgroups_per_cu(None) == -1
+    assert lib.fftoc_plan_set_workgroups_per_cu(None, 2) == 13
+    assert fo.make_flags(per_cu=5) == 5 << 24
+    fake = 1 << 20  # aligned, never dereferenced: rejected before any launch
+    assert lib.fftoc_fft(fake, fake, 4, 100, fo.make_flags(per_cu=5), None) == 1

@@ -156,6 +156,49 @@ def test_plan_autotune_keeps_results_correct():
         check_job(j, got)
 
 
+def test_plan_workgroups_per_cu_keeps_results_bit_for_bit():
+    """The plan's workgroups per CU (set by hand, or by the autotuner's sweep
+    after its routine choice) changes the grid only: mixed and one-routine
+    plans give the same bits at every setting."""
+    mixed = make_jobs(6)
+    one = [dict(n=4096, x=signals(4096, 300, seed=61), flags=0, c64=False, inv=False, norm=False),
+           dict(n=4096, x=signals(4096, 200, seed=62), flags=0, c64=False, inv=False, norm=False)]
+    for jobs in (mixed, one):
+        bufs = [(fo.DeviceBuffer.from_array(j["x"]), fo.DeviceBuffer(j["x"].nbytes)) for j in jobs]
+        plan = fo.Plan([(i, o, j["x"].shape[0], j["n"], j["flags"]) for (i, o), j in zip(bufs, jobs)])
+        assert plan.workgroups_per_cu == 0
+        results = []
+        for k in (0, 1, 3, 63):
+            print(f"[plan per-cu] jobs={len(jobs)} k={k}", flush=True)
+            plan.workgroups_per_cu = k
+            assert plan.workgroups_per_cu == k
+            for _, o in bufs:
+                o.upload(np.zeros(o.nbytes // 8, np.float64))
+            plan.execute()
+            fo.synchronize()
+            results.append([o.download(j["x"].shape, dtype=j["x"].dtype) for (_, o), j in zip(bufs, jobs)])
+        for r in results[1:]:
+            for a, b in zip(results[0], r):
+                assert np.array_equal(a, b)
+        with pytest.raises(fo.FftocError):
+            plan.workgroups_per_cu = 64
+        for j, got in zip(jobs, results[0]):
+            check_job(j, got)
+        plan.autotune(reps=2)  # may change the routines, hence the bits
+        tuned = plan.workgroups_per_cu
+        assert 0 <= tuned <= 16
+        outs = []
+        for k in (tuned, 0):
+            plan.workgroups_per_cu = k
+            plan.execute()
+            fo.synchronize()
+            outs.append([o.download(j["x"].shape, dtype=j["x"].dtype) for (_, o), j in zip(bufs, jobs)])
+        for j, a, b in zip(jobs, outs[0], outs[1]):
+            assert np.array_equal(a, b)
+            check_job(j, a)
+        plan.close()
+
+
 @pytest.mark.parametrize("n", fo.ALL_SIZES)
 def test_every_plan_routine_matches_oracle(n):
     """Each tile routine of length n, pinned with fftoc_plan_set_routine, for

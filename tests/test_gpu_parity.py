@@ -209,7 +209,11 @@ def test_autotune_and_tuning_roundtrip(tmp_path, c64):
     path = str(tmp_path / "tune.txt").encode()
     assert lib.fftoc_tuning_save(path) == 0
     text = open(path).read()
-    assert f"fft {n} {'c64' if c64 else 'c128'} 12 {fo.variants(n, c64=c64)[best]}\n" in text
+    cap = lib.fftoc_get_workgroups_per_cu(n, dt, 4096)
+    assert cap == lib.fftoc_variant_per_cu(n, dt, best)
+    assert 0 <= cap < max(2, lib.fftoc_variant_occupancy(n, dt, best))
+    wgs = f"wgs={cap} " if cap else ""
+    assert f"fft {n} {'c64' if c64 else 'c128'} 12 {wgs}{fo.variants(n, c64=c64)[best]}\n" in text
     other = (best + 1) % count
     assert lib.fftoc_set_variant(n, dt, 4096, other) == 0
     assert lib.fftoc_get_variant(n, dt, 4096) == other
@@ -221,6 +225,41 @@ def test_autotune_and_tuning_roundtrip(tmp_path, c64):
     tol = 2e-6 if c64 else TOL
     got = fo.fft_host(x)
     assert np.max(oracle.rel_l2(got, oracle.reference(x.astype(np.complex128)), axis=-1)) <= tol
+
+
+@pytest.mark.parametrize("n,c64", [(8, False), (64, False), (256, False), (4096, False),
+                                   (64, True), (512, True)])
+def test_workgroups_per_cu_caps_keep_results_bit_for_bit(n, c64):
+    """FFTOC_WGS_PER_CU(k) holds a variant to k workgroups per CU -- a
+    persistent grid of k per CU, or (one workgroup per tile) an LDS request
+    that lets only k fit -- and never changes a result: every variant at cap
+    1 and at one below its occupancy reproduces its uncapped output bit for
+    bit, on a batch several times one grid."""
+    lib = fo.lib()
+    dt = fo.DATA_C64 if c64 else 0
+    x = signals(n, (1 << 21) // n + 7, seed=1300 + n)
+    if c64:
+        x = x.astype(np.complex64)
+    src = fo.DeviceBuffer.from_array(x)
+    dst = fo.DeviceBuffer(x.nbytes)
+    names = fo.variants(n, c64=c64)
+    for v, name in enumerate(names):
+        occ = lib.fftoc_variant_occupancy(n, dt, v)
+        assert 1 <= occ <= 63, (name, occ)
+        assert lib.fftoc_fft_variant(src.ptr, dst.ptr, x.shape[0], n, dt, v, None) == 0
+        fo.synchronize()
+        plain = dst.download(x.shape, x.dtype)
+        for cap in sorted({1, max(1, occ - 1)}):
+            print(f"[per-cu] n={n} c64={c64} v={v} {name} occupancy={occ} cap={cap}", flush=True)
+            dst.upload(np.zeros_like(x))
+            rc = lib.fftoc_fft_variant(src.ptr, dst.ptr, x.shape[0], n,
+                                       dt | fo.make_flags(per_cu=cap), v, None)
+            assert rc == 0, (name, cap, rc)
+            fo.synchronize()
+            assert np.array_equal(dst.download(x.shape, x.dtype), plain), (name, cap)
+    tol = 2e-6 if c64 else TOL
+    err = np.max(oracle.rel_l2(plain[:64], oracle.reference(x[:64].astype(np.complex128)), axis=-1))
+    assert err <= tol
 
 
 @pytest.mark.parametrize("n", fo.TENSOR_SIZES)
