@@ -704,6 +704,27 @@ COPY_KERNEL = "__amd_rocclr_copyBuffer"
 HEADLINE = "fp64_c2c_fwd_n4096_b16384"
 
 
+def run_power_evidence(fo, steps: int, warmup: int, x: np.ndarray) -> dict:
+    """N = 4096 x 16384 tensor kernels on N(0,1) data and on zeros: the same
+    instructions and HBM traffic, different switching activity.  A kernel that
+    runs faster on zeros is bound by the power its matrix products draw (the
+    clock drops), not by its memory system or issue."""
+    zeros = np.zeros((16384, 4096), np.complex128)
+    rows = {}
+    for key, variant in (("1 matrix stage (default)", "tensor:16x16x16/VALU stages A and C, 4 waves"),
+                         ("2 matrix stages", "tensor:16x16x16/VALU stage A, 4 waves"),
+                         ("2 matrix stages, 2-product", "tensor:16x16x16/VALU stage A, 4 waves/2-product"),
+                         ("3 matrix stages", "tensor:16x16x16/8 waves")):
+        row = {"variant": variant}
+        for data, hx in (("normal", x), ("zeros", zeros)):
+            wl = Workload(f"power_{data}", 4096, 16384, 7, "normal", "variant:" + variant)
+            r, _, _ = run_gpu(fo, wl, steps, warmup, host_x=hx)
+            row[f"{data}_GBps"] = round(r["GBps"], 1)
+        row["zeros_over_normal"] = round(row["zeros_GBps"] / row["normal_GBps"], 3)
+        rows[key] = row
+    return rows
+
+
 def headline_variant(fo, wl) -> str:
     """Name of the kernel variant fftoc_fft runs for the workload."""
     lib = fo.lib()
@@ -835,6 +856,9 @@ def main():
     extra = {}
     if not args.quick:
         others = [Workload("fp64_n4096_b16384_tensor", 4096, 16384, 7, "normal", "tensor"),
+                  # Karatsuba with two of the three DFT16 stages on the matrix cores
+                  Workload("fp64_n4096_b16384_tensor_2mfma", 4096, 16384, 7, "normal",
+                           "variant:tensor:16x16x16/VALU stage A, 4 waves"),
                   # the tensor path with all three DFT16 stages on the matrix cores
                   Workload("fp64_n4096_b16384_tensor_3mfma", 4096, 16384, 7, "normal",
                            "variant:tensor:16x16x16/8 waves"),
@@ -872,6 +896,9 @@ def main():
             r, _, _ = run_gpu(fo, wl, args.steps, args.warmup, host_x=cache[key])
             extra[wl.name] = {k: (round(v, 6) if isinstance(v, float) else v) for k, v in r.items()}
             print(f"[bench] {wl.name}: {r['ffts_per_s']:.4g} FFT/s, {r['GBps']:.0f} GB/s", file=sys.stderr, flush=True)
+    if not args.quick:
+        extra["tensor_power_n4096"] = run_power_evidence(fo, args.steps, args.warmup, x)
+        print(f"[bench] tensor power: {extra['tensor_power_n4096']}", file=sys.stderr, flush=True)
     extra["hip_d2d_copy_GBps"] = round(copy_bandwidth(), 1)
     rocfft_samples = {}
     if not args.quick:

@@ -880,12 +880,15 @@ FFTOC_TENSOR_LAUNCHER(launch_t256, tensor256_kernel, kWaves, 64 * kWaves, 1, Kar
 FFTOC_TENSOR_LAUNCHER(launch_t256g2, tensor256_kernel, 2 * kWaves, 64 * kWaves, 2, KaratsubaCore)
 FFTOC_TENSOR_LAUNCHER(launch_t256g2h, tensor256_kernel, 2 * kWaves, 64 * kWaves, 2, TwoProductCore)
 FFTOC_TENSOR_LAUNCHER(launch_t256g4, tensor256_kernel, 4 * kWaves, 64 * kWaves, 4, KaratsubaCore)
+FFTOC_TENSOR_LAUNCHER(launch_t256g2q, tensor256_kernel, 2 * kWaves, 64 * kWaves, 2, KaratsubaHalfCore)
+FFTOC_TENSOR_LAUNCHER(launch_t256g4q, tensor256_kernel, 4 * kWaves, 64 * kWaves, 4, KaratsubaHalfCore)
 FFTOC_TENSOR_LAUNCHER(launch_t512, tensor512_kernel, kWaves, 64 * kWaves, 1)
 FFTOC_TENSOR_LAUNCHER(launch_t512g2, tensor512_kernel, 2 * kWaves, 64 * kWaves, 2)
 FFTOC_TENSOR_LAUNCHER(launch_t4096, tensor4096_kernel, 1, 64 * k4096Waves)
 FFTOC_TENSOR_LAUNCHER(launch_t4096t, tensor4096t_kernel, 1, 64 * k4096Waves)
 FFTOC_TENSOR_LAUNCHER(launch_t4096h, tensor4096h_kernel, 1, 256, 4, KaratsubaCore)
 FFTOC_TENSOR_LAUNCHER(launch_t4096hh, tensor4096h_kernel, 1, 256, 4, TwoProductCore)
+FFTOC_TENSOR_LAUNCHER(launch_t4096h1, tensor4096h_kernel, 1, 256, 4, KaratsubaHalfCore)
 #undef FFTOC_TENSOR_LAUNCHER
 
 constexpr unsigned TF = FAM_TENSOR | FAM_ANY;
@@ -912,13 +915,19 @@ const Variant kT128[] = {{"tensor:16x8/4 per wave", TF, &launch_t128g2, 16},
 const Variant kT256[] = {{"tensor:16x16/2 per wave", TF, &launch_t256g2, 8},
                          {"tensor:16x16/2 per wave/2-product", TF, &launch_t256g2h, 8},
                          {"tensor:16x16/4 per wave", TF, &launch_t256g4, 16},
-                         {"tensor:16x16/1 per wave", TF, &launch_t256, 4}};
+                         {"tensor:16x16/1 per wave", TF, &launch_t256, 4},
+                         {"tensor:16x16/2 per wave/VALU stage 2", TF, &launch_t256g2q, 8},
+                         {"tensor:16x16/4 per wave/VALU stage 2", TF, &launch_t256g4q, 16}};
 const Variant kT512[] = {{"tensor:16x2x16/1 per wave/persistent", TF, &launch_t512p, 4},
                          {"tensor:16x2x16/1 per wave", TF, &launch_t512, 4},
                          {"tensor:16x2x16/2 per wave", TF, &launch_t512g2, 8}};
-// The hybrid form leads: 0.64 of HBM against 0.51 for three MFMA stages
-// (bound by the matrix cores plus the vector work between them).
-const Variant kT4096[] = {{"tensor:16x16x16/VALU stage A, 4 waves", TF, &launch_t4096h, 1},
+// The first variant is the default of fftoc_tensor_fft at N = 4096: one
+// matrix-core DFT16 stage (Karatsuba) between two vector-ALU radix-16 stages,
+// 0.73-0.75 of HBM on random data, against 0.68 for two matrix-core stages
+// and 0.52 for three (FP64 matrix products on random data are power-bound:
+// the two-stage form runs at 0.785 on zeros).
+const Variant kT4096[] = {{"tensor:16x16x16/VALU stages A and C, 4 waves", TF, &launch_t4096h1, 1},
+                          {"tensor:16x16x16/VALU stage A, 4 waves", TF, &launch_t4096h, 1},
                           {"tensor:16x16x16/VALU stage A, 4 waves/2-product", TF, &launch_t4096hh, 1},
                           {"tensor:16x16x16/8 waves", TF, &launch_t4096, 1},
                           {"tensor:16x16x16/8 waves/2-product", TF, &launch_t4096t, 1}};

@@ -274,6 +274,8 @@ struct KaratsubaCore {  // three real products per DFT16 (24 MFMAs per core)
   }
   __device__ __forceinline__ void run(c2 (&a)[4], c2 (&z)[4]) const { core256(a, f, w, z); }
   __device__ __forceinline__ int out_index(int q) const { return 4 * q + g + 16 * j; }
+  // out_index(q) of the lane with group g and column j
+  __device__ static __forceinline__ int out_k2(int g, int q, int j) { return 4 * q + g + 16 * j; }
 };
 struct TwoProductCore {  // two real products per DFT16 (16 MFMAs per core)
   HFrag f;
@@ -289,6 +291,99 @@ struct TwoProductCore {  // two real products per DFT16 (16 MFMAs per core)
   }
   __device__ __forceinline__ void run(c2 (&a)[4], c2 (&z)[4]) const { core256h(a, f, k, w, g, z); }
   __device__ __forceinline__ int out_index(int q) const { return hphi(j) + 16 * hk2(g, q); }
+  __device__ static __forceinline__ int out_k2(int g, int q, int j) { return hphi(j) + 16 * hk2(g, q); }
+};
+
+// Exchanges complex values between the lane pairs (l, l + D), D = 32 (wave
+// halves) or 16 (rows 2m, 2m + 1), with one v_permlane{32,16}_swap per dword
+// (p's upper lanes swapped with q's lower lanes): afterwards both lanes of a
+// pair hold in p the lower lane's value and in q the upper lane's value, of
+// register p in the lower lane and of register q in the upper lane.
+template <int D>
+__device__ __forceinline__ void swap_halves(double &p, double &q) {
+  static_assert(D == 16 || D == 32, "lane distance 16 or 32");
+  const unsigned long long pu = __builtin_bit_cast(unsigned long long, p);
+  const unsigned long long qu = __builtin_bit_cast(unsigned long long, q);
+  unsigned plo = (unsigned)pu, phi = (unsigned)(pu >> 32);
+  unsigned qlo = (unsigned)qu, qhi = (unsigned)(qu >> 32);
+  if constexpr (D == 32) {
+    const auto rl = __builtin_amdgcn_permlane32_swap(plo, qlo, false, false);
+    const auto rh = __builtin_amdgcn_permlane32_swap(phi, qhi, false, false);
+    plo = rl[0], qlo = rl[1], phi = rh[0], qhi = rh[1];
+  } else {
+    const auto rl = __builtin_amdgcn_permlane16_swap(plo, qlo, false, false);
+    const auto rh = __builtin_amdgcn_permlane16_swap(phi, qhi, false, false);
+    plo = rl[0], qlo = rl[1], phi = rh[0], qhi = rh[1];
+  }
+  p = __builtin_bit_cast(double, ((unsigned long long)phi << 32) | plo);
+  q = __builtin_bit_cast(double, ((unsigned long long)qhi << 32) | qlo);
+}
+template <int D>
+__device__ __forceinline__ void swap_halves(c2 &p, c2 &q) {
+  swap_halves<D>(p.x, q.x);
+  swap_halves<D>(p.y, q.y);
+}
+
+// 256-point core with one matrix-core stage: stage 1 (DFT16 over n1) on the
+// matrix cores with Karatsuba's three real products, as in core256; stage 2
+// (DFT16 over n2 = 4 r + g for each k1 = j) on the vector ALUs as 4 x 4: a
+// DFT4 over r in the lane, the twiddle W_16^{g ka}, and a DFT4 over g across
+// the lanes j, j + 16, j + 32, j + 48 (radix-2 over g1 = g / 2 between wave
+// halves, then the twiddle W_4^{g0 kb0} and radix-2 over g0 = g % 2 between
+// rows, each exchange one permlane swap per dword).  12 MFMAs per core
+// instead of 24: FP64 matrix products on random data draw enough power to
+// lower the clock, and the FFT factorization of stage 2 needs a quarter of
+// the dense product's flops.  Output z[q] = X[j + 16 k2], k2 = g1 + 2 (q / 2)
+// + 4 g0 + 8 (q % 2).
+struct KaratsubaHalfCore {
+  Frag f;
+  c2 w[4];    // W_256^{(4r + g) j}
+  c2 t16[3];  // W_16^{g ka}, ka = 1..3
+  int g, j;
+  __device__ __forceinline__ void load(const c2 *__restrict__ tw, int lane) {
+    f = dft16_frag(tw, lane);
+    core256_twiddles(tw, lane, w);
+    g = lane >> 4;
+    j = lane & 15;
+#pragma unroll
+    for (int ka = 1; ka < 4; ++ka) t16[ka - 1] = tw[((g * ka) & 15) << 8];
+  }
+  __device__ __forceinline__ void run(c2 (&a)[4], c2 (&z)[4]) const {
+    v4d yr, yi;
+    kgemm(a, f, yr, yi);
+    c2 y[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) y[r] = cmul(make_double2(yr[r], yi[r]), w[r]);
+    dft4(y[0], y[1], y[2], y[3]);  // over r: y[ka]
+#pragma unroll
+    for (int ka = 1; ka < 4; ++ka) y[ka] = cmul(y[ka], t16[ka - 1]);
+    // radix-2 over g1: lanes l and l + 32 (registers paired (ka, ka + 1); the
+    // lower lane keeps ka, the upper one ka + 1)
+    swap_halves<32>(y[0], y[1]);
+    swap_halves<32>(y[2], y[3]);
+    dft2(y[0], y[1]);
+    dft2(y[2], y[3]);
+    // radix-2 over g0: rows 2m and 2m + 1; the odd row's kb0 = 1 value times -i
+    const bool odd = g & 1;
+    swap_halves<16>(y[0], y[1]);
+    swap_halves<16>(y[2], y[3]);
+#pragma unroll
+    for (int p = 1; p < 4; p += 2) {
+      // selects per double (a select of two c2 values compiled to a select
+      // of their stack addresses)
+      const double re = y[p].x, im = y[p].y;
+      y[p].x = odd ? im : re;
+      y[p].y = odd ? -re : im;
+    }
+    dft2(y[0], y[1]);
+    dft2(y[2], y[3]);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) z[q] = y[q];
+  }
+  __device__ static __forceinline__ int out_k2(int g, int q, int j) {
+    return j + 16 * ((g >> 1) + 2 * (q >> 1) + 4 * (g & 1) + 8 * (q & 1));
+  }
+  __device__ __forceinline__ int out_index(int q) const { return out_k2(g, q, j); }
 };
 
 // LDS element swizzle for the output transposes: XOR the low 4 bits of the
