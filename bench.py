@@ -145,7 +145,8 @@ ONCHIP_REF = {8: (19, "MMA FPU=8 UPB=2", 0.2355),
               4096: (274, "cuFFTDx", 253.27)}
 ONCHIP_REF_FFTS = 32 * 108
 ONCHIP_FAMILIES = {0: "radix (fftoc::LdsFft)", 1: "tensor Karatsuba (fftoc::LdsTensorFft)",
-                   2: "tensor two-product (fftoc::LdsTensorFft<.., true>)"}
+                   2: "tensor two-product (fftoc::LdsTensorFft<.., kTwoProduct>)",
+                   3: "tensor Karatsuba, one matrix stage (fftoc::LdsTensorFft<.., kKaratsubaOneStage>)"}
 
 
 def onchip_library(fo):
@@ -318,6 +319,8 @@ def run_onchip(fo, reps: int = 3):
         for fam, fam_name in ONCHIP_FAMILIES.items():
             if fam == 2 and n == 8:
                 continue  # the two-product contraction starts at N = 16
+            if fam == 3 and n not in (256, 4096):
+                continue  # one matrix stage of a 256-point core
             res = {"functor": fam_name}
             best = 0.0
             for shape, fb in shapes.items():

@@ -292,6 +292,7 @@ int run(Family fam, const void *in, void *out, int64_t batch, int n, uint32_t fl
   int pick = variant >= 0 ? variant : reg->choice[family_index(fam)][batch_bucket(batch)].load();
   if (pick < 0 || pick >= (int)reg->v.size())
     return a.c64 ? FFTOC_ERROR_UNSUPPORTED : FFTOC_ERROR_INVALID_SIZE;
+  const int requested_per_cu = a.per_cu;  // FFTOC_WGS_PER_CU of the call, 0 if none
   // the tuned workgroups-per-CU cap belongs to the tuned choice
   if (variant < 0 && a.per_cu == 0 && a.tile == 0)
     a.per_cu = reg->per_cu[family_index(fam)][batch_bucket(batch)].load();
@@ -311,6 +312,9 @@ int run(Family fam, const void *in, void *out, int64_t batch, int n, uint32_t fl
   a.twiddles = a.c64 ? g_dev[dev].twiddles32 : g_dev[dev].twiddles;
   a.twiddles32 = g_dev[dev].twiddles32;
   if (fam == Family::Tensor && a.precision != FFTOC_PREC_FP64) {
+    // the split-precision kernels are not FP64 variants: only the call's own
+    // cap applies (their defaults are in launch_tensor_precision)
+    a.per_cu = requested_per_cu;
     a.twiddles = g_dev[dev].twiddles;
     return launch_tensor_precision(n, a);
   }

@@ -65,18 +65,22 @@ int launch(const void *in, void *out, int64_t batch, const void *tw, int inverse
 
 // Tensor functor per length: units of one wave (N <= 256) with several
 // signals each, or one four-wave unit (N = 4096); 256 threads per workgroup.
-// N = 8 has the Karatsuba contraction only.
-template <int N, bool TP> struct TensorCfg;
-template <bool TP> struct TensorCfg<8, TP> { using F = fftoc::LdsTensorFft<8, 4, 64, false>; };
-template <bool TP> struct TensorCfg<16, TP> { using F = fftoc::LdsTensorFft<16, 4, 16, TP>; };
-template <bool TP> struct TensorCfg<64, TP> { using F = fftoc::LdsTensorFft<64, 4, 8, TP>; };
-template <bool TP> struct TensorCfg<128, TP> { using F = fftoc::LdsTensorFft<128, 4, 4, TP>; };
-template <bool TP> struct TensorCfg<256, TP> { using F = fftoc::LdsTensorFft<256, 4, 2, TP>; };
-template <bool TP> struct TensorCfg<512, TP> { using F = fftoc::LdsTensorFft<512, 4, 1, TP>; };
-template <bool TP> struct TensorCfg<4096, TP> { using F = fftoc::LdsTensorFft<4096, 1, 1, TP>; };
-template <int N, bool TP>
-constexpr bool kHasTensor = (N == 8 && !TP) || N == 16 || N == 64 || N == 128 || N == 256 ||
-                            N == 512 || N == 4096;
+// C: the contraction (fftoc::LdsContraction).  N = 8 has the Karatsuba
+// contraction only; the one-matrix-stage form exists where a 256-point core
+// runs (N = 256, 4096).
+template <int N, int C> struct TensorCfg;
+template <int C> struct TensorCfg<8, C> { using F = fftoc::LdsTensorFft<8, 4, 64, fftoc::kKaratsuba>; };
+template <int C> struct TensorCfg<16, C> { using F = fftoc::LdsTensorFft<16, 4, 16, C>; };
+template <int C> struct TensorCfg<64, C> { using F = fftoc::LdsTensorFft<64, 4, 8, C>; };
+template <int C> struct TensorCfg<128, C> { using F = fftoc::LdsTensorFft<128, 4, 4, C>; };
+template <int C> struct TensorCfg<256, C> { using F = fftoc::LdsTensorFft<256, 4, 2, C>; };
+template <int C> struct TensorCfg<512, C> { using F = fftoc::LdsTensorFft<512, 4, 1, C>; };
+template <int C> struct TensorCfg<4096, C> { using F = fftoc::LdsTensorFft<4096, 1, 1, C>; };
+template <int N, int C>
+constexpr bool kHasTensor =
+    (N == 16 || N == 64 || N == 128 || N == 256 || N == 512 || N == 4096 || N == 8) &&
+    (C == fftoc::kKaratsuba || (C == fftoc::kTwoProduct && N != 8) ||
+     (C == fftoc::kKaratsubaOneStage && (N == 256 || N == 4096)));
 
 // Threads and signals per workgroup of either functor family.
 template <class F> struct Shape {
@@ -125,10 +129,10 @@ __global__ __launch_bounds__(Shape<F>::threads) void windowed_tensor_kernel(
   store_block<N, F::kElems, Shape<F>::threads>(out, buf, s0, batch);
 }
 
-template <int N, bool TP>
+template <int N, int C>
 int launch_tensor(const void *in, void *out, int64_t batch, const void *tw, int inverse,
                   double scale, int window, hipStream_t stream) {
-  using F = typename TensorCfg<N, TP>::F;
+  using F = typename TensorCfg<N, C>::F;
   constexpr int S = F::kElems / N;
   const int64_t grid = (batch + S - 1) / S;
   if (grid > 0x7fffffffLL) return FFTOC_ERROR_INVALID_BATCH;
@@ -139,12 +143,19 @@ int launch_tensor(const void *in, void *out, int64_t batch, const void *tw, int 
 }
 
 template <int N>
-int tensor(bool tp, const void *in, void *out, int64_t batch, const void *tw, int inverse,
+int tensor(int contraction, const void *in, void *out, int64_t batch, const void *tw, int inverse,
            double scale, int window, hipStream_t s) {
-  if constexpr (kHasTensor<N, true>)
-    if (tp) return launch_tensor<N, true>(in, out, batch, tw, inverse, scale, window, s);
-  if (tp) return FFTOC_ERROR_UNSUPPORTED;
-  return launch_tensor<N, false>(in, out, batch, tw, inverse, scale, window, s);
+  if constexpr (kHasTensor<N, fftoc::kKaratsuba>)
+    if (contraction == fftoc::kKaratsuba)
+      return launch_tensor<N, fftoc::kKaratsuba>(in, out, batch, tw, inverse, scale, window, s);
+  if constexpr (kHasTensor<N, fftoc::kTwoProduct>)
+    if (contraction == fftoc::kTwoProduct)
+      return launch_tensor<N, fftoc::kTwoProduct>(in, out, batch, tw, inverse, scale, window, s);
+  if constexpr (kHasTensor<N, fftoc::kKaratsubaOneStage>)
+    if (contraction == fftoc::kKaratsubaOneStage)
+      return launch_tensor<N, fftoc::kKaratsubaOneStage>(in, out, batch, tw, inverse, scale,
+                                                         window, s);
+  return FFTOC_ERROR_UNSUPPORTED;
 }
 
 // The reference's timing harness with either functor family: one load, R
@@ -181,12 +192,18 @@ template <int N>
 int onchip(int family, void *data, int64_t batch, const void *tw, int repeats, double scale,
            hipStream_t s) {
   if (family == 0) return launch_onchip<N, Fft<N>>(data, batch, tw, repeats, scale, s);
-  if constexpr (kHasTensor<N, false>)
+  if constexpr (kHasTensor<N, fftoc::kKaratsuba>)
     if (family == 1)
-      return launch_onchip<N, typename TensorCfg<N, false>::F>(data, batch, tw, repeats, scale, s);
-  if constexpr (kHasTensor<N, true>)
+      return launch_onchip<N, typename TensorCfg<N, fftoc::kKaratsuba>::F>(data, batch, tw,
+                                                                           repeats, scale, s);
+  if constexpr (kHasTensor<N, fftoc::kTwoProduct>)
     if (family == 2)
-      return launch_onchip<N, typename TensorCfg<N, true>::F>(data, batch, tw, repeats, scale, s);
+      return launch_onchip<N, typename TensorCfg<N, fftoc::kTwoProduct>::F>(data, batch, tw,
+                                                                            repeats, scale, s);
+  if constexpr (kHasTensor<N, fftoc::kKaratsubaOneStage>)
+    if (family == 3)
+      return launch_onchip<N, typename TensorCfg<N, fftoc::kKaratsubaOneStage>::F>(
+          data, batch, tw, repeats, scale, s);
   return FFTOC_ERROR_UNSUPPORTED;
 }
 
@@ -220,25 +237,26 @@ int fftoc_example_windowed_fft(const void *in, void *out, int64_t batch, int n, 
 }
 
 // The matrix-core functors (fftoc::LdsTensorFft) in the same example: n in
-// {8, 16, 64, 128, 256, 512, 4096}; two_product selects the two-real-product
-// contraction (not at n = 8).
+// {8, 16, 64, 128, 256, 512, 4096}; contraction 0 = Karatsuba, 1 = the
+// two-real-product form (not at n = 8), 2 = Karatsuba on one matrix stage of
+// the 256-point core (n = 256, 4096).
 int fftoc_example_windowed_tensor_fft(const void *in, void *out, int64_t batch, int n,
                                       const void *tw, int inverse, double scale, int window,
-                                      int two_product, void *stream) {
+                                      int contraction, void *stream) {
   if (batch < 0) return FFTOC_ERROR_INVALID_BATCH;
   if (batch == 0) return FFTOC_SUCCESS;
   if (!in || !out || !tw) return FFTOC_ERROR_NULL_POINTER;
   if (in == out) return FFTOC_ERROR_OVERLAP;
+  if (contraction < 0 || contraction > 2) return FFTOC_ERROR_INVALID_FLAGS;
   hipStream_t s = static_cast<hipStream_t>(stream);
-  const bool tp = two_product != 0;
   switch (n) {
-    case 8: return tensor<8>(tp, in, out, batch, tw, inverse, scale, window, s);
-    case 16: return tensor<16>(tp, in, out, batch, tw, inverse, scale, window, s);
-    case 64: return tensor<64>(tp, in, out, batch, tw, inverse, scale, window, s);
-    case 128: return tensor<128>(tp, in, out, batch, tw, inverse, scale, window, s);
-    case 256: return tensor<256>(tp, in, out, batch, tw, inverse, scale, window, s);
-    case 512: return tensor<512>(tp, in, out, batch, tw, inverse, scale, window, s);
-    case 4096: return tensor<4096>(tp, in, out, batch, tw, inverse, scale, window, s);
+    case 8: return tensor<8>(contraction, in, out, batch, tw, inverse, scale, window, s);
+    case 16: return tensor<16>(contraction, in, out, batch, tw, inverse, scale, window, s);
+    case 64: return tensor<64>(contraction, in, out, batch, tw, inverse, scale, window, s);
+    case 128: return tensor<128>(contraction, in, out, batch, tw, inverse, scale, window, s);
+    case 256: return tensor<256>(contraction, in, out, batch, tw, inverse, scale, window, s);
+    case 512: return tensor<512>(contraction, in, out, batch, tw, inverse, scale, window, s);
+    case 4096: return tensor<4096>(contraction, in, out, batch, tw, inverse, scale, window, s);
     default: return FFTOC_ERROR_INVALID_SIZE;
   }
 }
@@ -246,14 +264,15 @@ int fftoc_example_windowed_tensor_fft(const void *in, void *out, int64_t batch, 
 // On-chip timing harness: `repeats` forward transforms (outputs times `scale`)
 // of the batch held in LDS between one load and one store, in place on data.
 // family 0 = fftoc::LdsFft (any n), 1 = fftoc::LdsTensorFft with Karatsuba, 2 =
-// with the two-product contraction (tensor n in {8, 16, 64, 128, 256, 512,
-// 4096}, two-product not at 8).
+// with the two-product contraction, 3 = Karatsuba on one matrix stage (tensor
+// n in {8, 16, 64, 128, 256, 512, 4096}, two-product not at 8, family 3 at
+// n = 256 and 4096).
 int fftoc_example_onchip_fft(void *data, int64_t batch, int n, const void *tw, int family,
                              int repeats, double scale, void *stream) {
   if (batch < 0 || repeats < 0) return FFTOC_ERROR_INVALID_BATCH;
   if (batch == 0) return FFTOC_SUCCESS;
   if (!data || !tw) return FFTOC_ERROR_NULL_POINTER;
-  if (family < 0 || family > 2) return FFTOC_ERROR_INVALID_FLAGS;
+  if (family < 0 || family > 3) return FFTOC_ERROR_INVALID_FLAGS;
   hipStream_t s = static_cast<hipStream_t>(stream);
   switch (n) {
     case 8: return onchip<8>(family, data, batch, tw, repeats, scale, s);

@@ -252,24 +252,34 @@ __global__ __launch_bounds__(64 * kWaves) void tensor64p_kernel(const c2 *in, c2
 
 // 256-point core on one wave in precision P: a[c] = x[64g + 16c + j]
 // (n = 16 n1 + n2, n1 = 4g + c, n2 = j); z[r] = X[4r + g + 16 j]... with the
-// D rows 4g + r: z[r] = X[4g + r + 16 j].
+// D rows 4g + r: z[r] = X[4g + r + 16 j].  w[r] = W_256^{(4g + r) j} rounded
+// to FP32 (core256p_twiddles).
+__device__ __forceinline__ void core256p_twiddles(const f2 *__restrict__ tw32, int lane,
+                                                  f2 (&w)[4]) {
+  const int g = lane >> 4, j = lane & 15;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) w[r] = tw32[((4 * g + r) * j) << 4];
+}
 template <int P>
 __device__ __forceinline__ void core256p(f2 (&a)[4], const typename Stage<P>::Frag &f,
-                                         const f2 *__restrict__ tw32, int lane, f2 (&z)[4]) {
-  const int g = lane >> 4, j = lane & 15;
+                                         const f2 (&w)[4], f2 (&z)[4]) {
   v4f yr, yi;
   Stage<P>::gemm(a, f, yr, yi);
   // D register r = Y[n2 = 4g + r][k1 = j]; twiddle W_256^{n2 k1}
   f2 b[4];
 #pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    const int n2 = 4 * g + r;
-    b[r] = cmul(make_float2(yr[r], yi[r]), tw32[(n2 * j) << 4]);
-  }
+  for (int r = 0; r < 4; ++r) b[r] = cmul(make_float2(yr[r], yi[r]), w[r]);
   // stage 2: A = b (row k1 = j, K = n2 = 4g + r), same fragment
   Stage<P>::gemm(b, f, yr, yi);
 #pragma unroll
   for (int r = 0; r < 4; ++r) z[r] = make_float2(yr[r], yi[r]);
+}
+template <int P>
+__device__ __forceinline__ void core256p(f2 (&a)[4], const typename Stage<P>::Frag &f,
+                                         const f2 *__restrict__ tw32, int lane, f2 (&z)[4]) {
+  f2 w[4];
+  core256p_twiddles(tw32, lane, w);
+  core256p<P>(a, f, w, z);
 }
 
 __device__ __forceinline__ int swz16(int e) { return e ^ ((e >> 4) & 15); }
@@ -330,11 +340,18 @@ __global__ __launch_bounds__(256) void tensor4096ph_kernel(const c2 *in, c2 *out
                                                            double scale) {
   __shared__ f2 lds[4096];
   const int t = threadIdx.x;
-  const int lane = lane_id(), g = lane >> 4, j = lane & 15;
+  const int lane = lane_id();
   const int w = t >> 6;
   const int64_t s = blockIdx.x;
   if (s >= batch) return;  // uniform over the workgroup
   const c2 *x = in + s * 4096;
+  // The column's twiddle bases before the column, whose loads go out at
+  // raised priority (as in the FP64 kernel: loaded after the data, each
+  // table wait would also wait for it; the other resident workgroups are
+  // usually in their matrix stage).
+  TwBase<16, c2> ta;
+  ta.fetch(tw, t);
+  __builtin_amdgcn_s_setprio(3);
   {
     c2 v[16];
 #pragma unroll
@@ -343,31 +360,43 @@ __global__ __launch_bounds__(256) void tensor4096ph_kernel(const c2 *in, c2 *out
       if constexpr (INV) e = cswap(e);
       v[n1] = e;
     }
+    __builtin_amdgcn_s_setprio(0);
     dft<16>(v);
-    apply_twiddles<16>(v, tw, t);  // v[k1] *= W_4096^{t k1}
+    apply_twiddle_base<16>(v, ta);  // v[k1] *= W_4096^{t k1}
 #pragma unroll
     for (int k1 = 0; k1 < 16; ++k1) lds[swz_z(k1, t)] = make_float2((float)v[k1].x, (float)v[k1].y);
   }
-  __syncthreads();
+  // the core's table values (L2-resident) load while the workgroup meets at
+  // the barrier
   const auto f = Stage<P>::frag(tw, lane);
+  f2 cw[4];
+  core256p_twiddles(tw32, lane, cw);
+  const int g = lane >> 4, j = lane & 15;
+  __syncthreads();
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
     const int k1 = 4 * w + q;
     f2 a[4], z[4];
 #pragma unroll
     for (int c = 0; c < 4; ++c) a[c] = lds[swz_z(k1, 64 * g + 16 * c + j)];
-    core256p<P>(a, f, tw32, lane, z);
+    core256p<P>(a, f, cw, z);
     // z[r] = X[k1 + 16 k2], k2 = 4g + r + 16j
 #pragma unroll
     for (int r = 0; r < 4; ++r) lds[swz_row(k1, 4 * g + r + 16 * j)] = z[r];
   }
   __syncthreads();
-  c2 *y = out + s * 4096;
+  // every output read from LDS before the first store (the compiler would
+  // otherwise pair each read with its store and wait for that read alone)
+  f2 o[16];
 #pragma unroll
   for (int c = 0; c < 16; ++c) {
     const int e = 256 * c + t;
-    st_out<INV>(y + e, lds[swz_row(e & 15, e >> 4)], scale);
+    o[c] = lds[swz_row(e & 15, e >> 4)];
   }
+  __builtin_amdgcn_sched_barrier(0);
+  c2 *y = out + s * 4096;
+#pragma unroll
+  for (int c = 0; c < 16; ++c) st_out<INV>(y + 256 * c + t, o[c], scale);
 }
 
 int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
@@ -392,13 +421,25 @@ int launch_p(int n, const LaunchArgs &a) {
     default: return FFTOC_ERROR_UNSUPPORTED;
   }
   if (grid > 0x7fffffffLL) return FFTOC_ERROR_INVALID_BATCH;
+  // FFTOC_WGS_PER_CU on the one-tile kernels: an unused dynamic LDS request
+  // that leaves room for at most that many workgroups per CU.  Default at
+  // N = 4096 from 8192 signals: two per CU for FP32 stages, three for split
+  // BF16 (measured at 16384 signals: 0.727 / 0.707 of HBM against 0.683 /
+  // 0.660 at the occupancy, four per CU).
+  int per_cu = a.per_cu;
+  if (per_cu == 0 && n == 4096 && a.batch >= 8192) per_cu = P == kFP32 ? 2 : 3;
+  size_t lds = 0;
+  if (n == 256) lds = occupancy_lds<tensor256p_kernel<P, false>, 64 * kWaves>(per_cu);
+  if (n == 4096) lds = occupancy_lds<tensor4096ph_kernel<P, false>, 256>(per_cu);
   const dim3 gd((unsigned)grid), bd(block);
 #define FFTOC_P_LAUNCH(K)                                                                      \
   do {                                                                                        \
     if (a.inverse)                                                                            \
-      hipLaunchKernelGGL((K<P, true>), gd, bd, 0, a.stream, in, out, a.batch, tw, tw32, a.scale); \
+      hipLaunchKernelGGL((K<P, true>), gd, bd, lds, a.stream, in, out, a.batch, tw, tw32,      \
+                         a.scale);                                                            \
     else                                                                                      \
-      hipLaunchKernelGGL((K<P, false>), gd, bd, 0, a.stream, in, out, a.batch, tw, tw32, a.scale); \
+      hipLaunchKernelGGL((K<P, false>), gd, bd, lds, a.stream, in, out, a.batch, tw, tw32,     \
+                         a.scale);                                                            \
   } while (0)
   if (n == 64) FFTOC_P_LAUNCH(tensor64p_kernel);
   else if (n == 256) FFTOC_P_LAUNCH(tensor256p_kernel);

@@ -84,11 +84,16 @@ struct LdsFft {
 // 256) or four waves (N = 4096, 64 KiB of LDS per signal).  Every thread of
 // the block calls operator(); units of one wave need no barrier, the N = 4096
 // unit synchronises the block.  `tw` is the library's FP64 table
-// (fftoc_twiddle_table).  TWO_PRODUCT selects the two-real-product DFT16
-// contraction instead of Karatsuba's three (the library's "2-product"
-// variants).  operator()(inverse, scale): forward or inverse DFT, each output
-// multiplied by scale.
-template <int N, int UNITS = 1, int FPU = 1, bool TWO_PRODUCT = false>
+// (fftoc_twiddle_table).  CONTRACTION selects the DFT16 products:
+// kKaratsuba (0, Karatsuba's three real products), kTwoProduct (1, or true:
+// the two-real-product form, the library's "2-product" variants) or
+// kKaratsubaOneStage (2: where a 256-point core runs, N = 256 and 4096, only
+// its first DFT16 on the matrix cores with Karatsuba and its second as 4 x 4
+// on the vector ALUs, the library's N = 4096 default; elsewhere the same as
+// kKaratsuba).  operator()(inverse, scale): forward or inverse DFT, each
+// output multiplied by scale.
+enum LdsContraction : int { kKaratsuba = 0, kTwoProduct = 1, kKaratsubaOneStage = 2 };
+template <int N, int UNITS = 1, int FPU = 1, int CONTRACTION = kKaratsuba>
 struct LdsTensorFft;
 
 namespace lds_detail {
@@ -100,21 +105,23 @@ __device__ __forceinline__ void st_dir(double2 *p, double2 v, bool inverse, doub
   v = make_double2(v.x * scale, v.y * scale);
   *p = inverse ? make_double2(v.y, v.x) : v;
 }
-template <bool TWO_PRODUCT>
-using Dft16 = std::conditional_t<TWO_PRODUCT, TwoProductDft16, KaratsubaDft16>;
-template <bool TWO_PRODUCT>
-using Core256 = std::conditional_t<TWO_PRODUCT, TwoProductCore, KaratsubaCore>;
+template <int C>
+using Dft16 = std::conditional_t<C == kTwoProduct, TwoProductDft16, KaratsubaDft16>;
+template <int C>
+using Core256 = std::conditional_t<C == kTwoProduct, TwoProductCore,
+                                   std::conditional_t<C == kKaratsubaOneStage, KaratsubaHalfCore,
+                                                      KaratsubaCore>>;
 }  // namespace lds_detail
 
 // N = 16: one DFT16 per matrix row, 16 signals per matrix product.
-template <int UNITS, int FPU, bool TWO_PRODUCT>
-struct LdsTensorFft<16, UNITS, FPU, TWO_PRODUCT> {
+template <int UNITS, int FPU, int CONTRACTION>
+struct LdsTensorFft<16, UNITS, FPU, CONTRACTION> {
   static_assert(FPU % 16 == 0, "N = 16: ffts_per_unit must be a multiple of 16");
   static constexpr int threads = 64, units_per_block = UNITS, ffts_per_unit = FPU;
   static constexpr int max_threads_per_block = UNITS * threads;
   static constexpr int kElems = 16 * FPU * UNITS;
   double2 *buf;
-  lds_detail::Dft16<TWO_PRODUCT> d;
+  lds_detail::Dft16<CONTRACTION> d;
   __device__ LdsTensorFft(double2 *smem, const double2 *tw)
       : buf(smem + 16 * FPU * (threadIdx.x / threads)) {
     d.load(tw, threadIdx.x & 63);
@@ -138,14 +145,14 @@ struct LdsTensorFft<16, UNITS, FPU, TWO_PRODUCT> {
 // N = 64 = 16 x 4: four signals per matrix product (rows = (signal, n2)),
 // the DFT16 over n1 on the matrix cores, twiddle W_64^{n2 k1} and DFT4 over n2
 // in registers.
-template <int UNITS, int FPU, bool TWO_PRODUCT>
-struct LdsTensorFft<64, UNITS, FPU, TWO_PRODUCT> {
+template <int UNITS, int FPU, int CONTRACTION>
+struct LdsTensorFft<64, UNITS, FPU, CONTRACTION> {
   static_assert(FPU % 4 == 0, "N = 64: ffts_per_unit must be a multiple of 4");
   static constexpr int threads = 64, units_per_block = UNITS, ffts_per_unit = FPU;
   static constexpr int max_threads_per_block = UNITS * threads;
   static constexpr int kElems = 64 * FPU * UNITS;
   double2 *buf;
-  lds_detail::Dft16<TWO_PRODUCT> d;
+  lds_detail::Dft16<CONTRACTION> d;
   c2 w[4];  // W_64^{n2 k1} of register r (n2 = r, k1 = the lane's frequency)
   __device__ LdsTensorFft(double2 *smem, const double2 *tw)
       : buf(smem + 64 * FPU * (threadIdx.x / threads)) {
@@ -177,9 +184,9 @@ struct LdsTensorFft<64, UNITS, FPU, TWO_PRODUCT> {
 
 // N = 8: 32 signals per matrix product (the A operand holds two signals per
 // row, the DFT matrix is diag(F8, F8)); Karatsuba contraction only.
-template <int UNITS, int FPU, bool TWO_PRODUCT>
-struct LdsTensorFft<8, UNITS, FPU, TWO_PRODUCT> {
-  static_assert(!TWO_PRODUCT, "N = 8: the Karatsuba contraction only");
+template <int UNITS, int FPU, int CONTRACTION>
+struct LdsTensorFft<8, UNITS, FPU, CONTRACTION> {
+  static_assert(CONTRACTION != kTwoProduct, "N = 8: the Karatsuba contraction only");
   static_assert(FPU % 32 == 0, "N = 8: ffts_per_unit must be a multiple of 32");
   static constexpr int threads = 64, units_per_block = UNITS, ffts_per_unit = FPU;
   static constexpr int max_threads_per_block = UNITS * threads;
@@ -214,14 +221,14 @@ struct LdsTensorFft<8, UNITS, FPU, TWO_PRODUCT> {
 // N = 128 = 16 x 8: two signals per matrix product (rows = (signal, n2)),
 // the DFT16 over n1 on the matrix cores, twiddle W_128^{n2 k1}, DFT8 over n2
 // as a DFT4 in registers and one radix-2 step with the lane 32 apart.
-template <int UNITS, int FPU, bool TWO_PRODUCT>
-struct LdsTensorFft<128, UNITS, FPU, TWO_PRODUCT> {
+template <int UNITS, int FPU, int CONTRACTION>
+struct LdsTensorFft<128, UNITS, FPU, CONTRACTION> {
   static_assert(FPU % 2 == 0, "N = 128: ffts_per_unit must be even");
   static constexpr int threads = 64, units_per_block = UNITS, ffts_per_unit = FPU;
   static constexpr int max_threads_per_block = UNITS * threads;
   static constexpr int kElems = 128 * FPU * UNITS;
   double2 *buf;
-  lds_detail::Dft16<TWO_PRODUCT> d;
+  lds_detail::Dft16<CONTRACTION> d;
   c2 w[4];  // W_128^{n2 k1}, n2 = 2r + g / 2
   __device__ LdsTensorFft(double2 *smem, const double2 *tw)
       : buf(smem + 128 * FPU * (threadIdx.x / threads)) {
@@ -270,12 +277,12 @@ struct LdsTensorFft<128, UNITS, FPU, TWO_PRODUCT> {
 // N = 512 = 16 x 2 x 16: one signal per pass of the wave.  n = 32 n1 + 16 h +
 // n2'; a DFT16 over n1 for h = 0, 1, twiddle W_512^{n2 k1}, a radix-2 over h
 // with W_32^{n2' k2a}, a DFT16 over n2' for k2a = 0, 1; X[k1 + 16 k2a + 32 k2b].
-template <int UNITS, int FPU, bool TWO_PRODUCT>
-struct LdsTensorFft<512, UNITS, FPU, TWO_PRODUCT> {
+template <int UNITS, int FPU, int CONTRACTION>
+struct LdsTensorFft<512, UNITS, FPU, CONTRACTION> {
   static constexpr int threads = 64, units_per_block = UNITS, ffts_per_unit = FPU;
   static constexpr int max_threads_per_block = UNITS * threads;
   static constexpr int kElems = 512 * FPU * UNITS;
-  using D = lds_detail::Dft16<TWO_PRODUCT>;
+  using D = lds_detail::Dft16<CONTRACTION>;
   double2 *buf;
   D d;
   c2 w1[2][4];  // W_512^{n2 k1}, n2 = 16 h + 4 r + g, k1 = d.col
@@ -326,13 +333,13 @@ struct LdsTensorFft<512, UNITS, FPU, TWO_PRODUCT> {
 };
 
 // N = 256 = 16 x 16: the two-stage core, one signal per pass of the wave.
-template <int UNITS, int FPU, bool TWO_PRODUCT>
-struct LdsTensorFft<256, UNITS, FPU, TWO_PRODUCT> {
+template <int UNITS, int FPU, int CONTRACTION>
+struct LdsTensorFft<256, UNITS, FPU, CONTRACTION> {
   static constexpr int threads = 64, units_per_block = UNITS, ffts_per_unit = FPU;
   static constexpr int max_threads_per_block = UNITS * threads;
   static constexpr int kElems = 256 * FPU * UNITS;
   double2 *buf;
-  lds_detail::Core256<TWO_PRODUCT> core;
+  lds_detail::Core256<CONTRACTION> core;
   __device__ LdsTensorFft(double2 *smem, const double2 *tw)
       : buf(smem + 256 * FPU * (threadIdx.x / threads)) {
     core.load(tw, threadIdx.x & 63);
@@ -357,13 +364,13 @@ struct LdsTensorFft<256, UNITS, FPU, TWO_PRODUCT> {
 // registers, twiddle W_4096^{m k1}, back into the column), stage B (MFMA: the
 // 256-point core of each row k1, four rows per wave, results back into the
 // row in core order), then a gather into natural order.
-template <int UNITS, int FPU, bool TWO_PRODUCT>
-struct LdsTensorFft<4096, UNITS, FPU, TWO_PRODUCT> {
+template <int UNITS, int FPU, int CONTRACTION>
+struct LdsTensorFft<4096, UNITS, FPU, CONTRACTION> {
   static constexpr int threads = 256, units_per_block = UNITS, ffts_per_unit = FPU;
   static constexpr int max_threads_per_block = UNITS * threads;
   static constexpr int kElems = 4096 * FPU * UNITS;
   double2 *buf;
-  lds_detail::Core256<TWO_PRODUCT> core;
+  lds_detail::Core256<CONTRACTION> core;
   TwBase<16, double2> ta;
   __device__ LdsTensorFft(double2 *smem, const double2 *tw)
       : buf(smem + 4096 * FPU * (threadIdx.x / threads)) {

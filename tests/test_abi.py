@@ -102,13 +102,16 @@ def test_fused_example_library_loads_and_checks_arguments():
     assert t(16, 16, 4, 64, 32, 0, 1.0, 0, 1, None) == 5
     assert t(16, 4096, 4, 32, 32, 0, 1.0, 0, 0, None) == 1      # no tensor functor at 32
     assert t(16, 4096, 4, 8, 32, 0, 1.0, 0, 1, None) == 6       # n = 8: Karatsuba only
+    assert t(16, 4096, 4, 64, 32, 0, 1.0, 0, 3, None) == 4      # unknown contraction
+    assert t(16, 4096, 4, 64, 32, 0, 1.0, 0, 2, None) == 6      # one matrix stage: n = 256, 4096
     o = lib.fftoc_example_onchip_fft
     o.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
                   ctypes.c_int, ctypes.c_double, ctypes.c_void_p]
     o.restype = ctypes.c_int
     assert o(None, 0, 64, None, 0, 1, 1.0, None) == 0
     assert o(None, 4, 64, None, 0, 1, 1.0, None) == 3
-    assert o(16, 4, 64, 32, 3, 1, 1.0, None) == 4                # unknown family
+    assert o(16, 4, 64, 32, 4, 1, 1.0, None) == 4                # unknown family
+    assert o(16, 4, 64, 32, 3, 1, 1.0, None) == 6                # one matrix stage: n = 256, 4096
     assert o(16, 4, 64, 32, 0, -1, 1.0, None) == 2               # negative repeat count
     assert o(16, 4, 100, 32, 0, 1, 1.0, None) == 1
     assert o(16, 4, 32, 32, 1, 1, 1.0, None) == 6                # no tensor functor at 32

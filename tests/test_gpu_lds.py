@@ -83,14 +83,16 @@ def test_twiddle_table_entry_point():
 
 # signals per workgroup of the example's tensor functors (fused_example.hip)
 TENSOR_PER_GROUP = {8: 256, 16: 64, 64: 32, 128: 16, 256: 8, 512: 4, 4096: 1}
-TENSOR_CASES = [(n, tp) for n in sorted(TENSOR_PER_GROUP) for tp in (False, True)
-                if not (n == 8 and tp)]  # n = 8: Karatsuba only
+# contraction: 0 Karatsuba, 1 two-product (not at n = 8), 2 Karatsuba on one
+# matrix stage of the 256-point core (n = 256, 4096)
+TENSOR_CASES = [(n, c) for n in sorted(TENSOR_PER_GROUP) for c in (0, 1, 2)
+                if not (n == 8 and c == 1) and not (c == 2 and n not in (256, 4096))]
 
 
 @pytest.mark.parametrize("n,two_product", TENSOR_CASES)
 @pytest.mark.parametrize("window,inverse", [(False, False), (True, True)])
 def test_in_lds_tensor_functor_matches_oracle(example, n, two_product, window, inverse):
-    """fftoc::LdsTensorFft<N, UNITS, FPU, TWO_PRODUCT> (the reference's
+    """fftoc::LdsTensorFft<N, UNITS, FPU, CONTRACTION> (the reference's
     tensor_fft_N functor form on the matrix cores), several units per
     workgroup, a partial last workgroup."""
     f, tw = example
@@ -114,7 +116,8 @@ def test_in_lds_tensor_functor_matches_oracle(example, n, two_product, window, i
 
 @pytest.mark.parametrize("n,family", [(8, 0), (64, 0), (128, 0), (256, 0), (512, 0), (4096, 0),
                                       (8, 1), (16, 1), (64, 1), (128, 1), (256, 1), (512, 1),
-                                      (4096, 1), (64, 2), (128, 2), (512, 2), (4096, 2)])
+                                      (4096, 1), (64, 2), (128, 2), (512, 2), (4096, 2),
+                                      (256, 3), (4096, 3)])
 def test_onchip_harness_repeats_in_lds(example, n, family):
     """The on-chip timing harness: R = 1 and 2 transforms of the LDS-resident
     signals, each output times 1/sqrt(N), equal the oracle applied R times."""
