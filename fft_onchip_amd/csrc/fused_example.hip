@@ -76,6 +76,20 @@ template <int C> struct TensorCfg<128, C> { using F = fftoc::LdsTensorFft<128, 4
 template <int C> struct TensorCfg<256, C> { using F = fftoc::LdsTensorFft<256, 4, 2, C>; };
 template <int C> struct TensorCfg<512, C> { using F = fftoc::LdsTensorFft<512, 4, 1, C>; };
 template <int C> struct TensorCfg<4096, C> { using F = fftoc::LdsTensorFft<4096, 1, 1, C>; };
+// One-wave shapes for batches of a few signals per CU (the reference's
+// saturation sweep at small k): the fewest signals one unit takes, so that
+// k FFTs per CU still spread over every CU.
+template <int N, int C> struct TensorCfgSmall;
+template <int C> struct TensorCfgSmall<8, C> { using F = fftoc::LdsTensorFft<8, 1, 32, fftoc::kKaratsuba>; };
+template <int C> struct TensorCfgSmall<16, C> { using F = fftoc::LdsTensorFft<16, 1, 16, C>; };
+template <int C> struct TensorCfgSmall<64, C> { using F = fftoc::LdsTensorFft<64, 1, 4, C>; };
+template <int C> struct TensorCfgSmall<128, C> { using F = fftoc::LdsTensorFft<128, 1, 2, C>; };
+template <int C> struct TensorCfgSmall<256, C> { using F = fftoc::LdsTensorFft<256, 1, 1, C>; };
+template <int C> struct TensorCfgSmall<512, C> { using F = fftoc::LdsTensorFft<512, 1, 1, C>; };
+template <int C> struct TensorCfgSmall<4096, C> { using F = fftoc::LdsTensorFft<4096, 1, 1, C>; };
+template <int N>
+using FftSmall = LdsFft<N, (kSignals<N> < 4 ? kSignals<N> : 4)>;
+
 template <int N, int C>
 constexpr bool kHasTensor =
     (N == 16 || N == 64 || N == 128 || N == 256 || N == 512 || N == 4096 || N == 8) &&
@@ -188,28 +202,53 @@ int launch_onchip(void *data, int64_t batch, const void *tw, int repeats, double
   return hipGetLastError() == hipSuccess ? FFTOC_SUCCESS : FFTOC_ERROR_LAUNCH;
 }
 
-template <int N>
+template <int N, bool SMALL>
 int onchip(int family, void *data, int64_t batch, const void *tw, int repeats, double scale,
            hipStream_t s) {
-  if (family == 0) return launch_onchip<N, Fft<N>>(data, batch, tw, repeats, scale, s);
-  if constexpr (kHasTensor<N, fftoc::kKaratsuba>)
-    if (family == 1)
-      return launch_onchip<N, typename TensorCfg<N, fftoc::kKaratsuba>::F>(data, batch, tw,
-                                                                           repeats, scale, s);
-  if constexpr (kHasTensor<N, fftoc::kTwoProduct>)
-    if (family == 2)
-      return launch_onchip<N, typename TensorCfg<N, fftoc::kTwoProduct>::F>(data, batch, tw,
-                                                                            repeats, scale, s);
-  if constexpr (kHasTensor<N, fftoc::kKaratsubaOneStage>)
-    if (family == 3)
-      return launch_onchip<N, typename TensorCfg<N, fftoc::kKaratsubaOneStage>::F>(
-          data, batch, tw, repeats, scale, s);
+  using R = std::conditional_t<SMALL, FftSmall<N>, Fft<N>>;
+  if (family == 0) return launch_onchip<N, R>(data, batch, tw, repeats, scale, s);
+  if constexpr (kHasTensor<N, fftoc::kKaratsuba>) {
+    using T = std::conditional_t<SMALL, TensorCfgSmall<N, fftoc::kKaratsuba>,
+                                 TensorCfg<N, fftoc::kKaratsuba>>;
+    if (family == 1) return launch_onchip<N, typename T::F>(data, batch, tw, repeats, scale, s);
+  }
+  if constexpr (kHasTensor<N, fftoc::kTwoProduct>) {
+    using T = std::conditional_t<SMALL, TensorCfgSmall<N, fftoc::kTwoProduct>,
+                                 TensorCfg<N, fftoc::kTwoProduct>>;
+    if (family == 2) return launch_onchip<N, typename T::F>(data, batch, tw, repeats, scale, s);
+  }
+  if constexpr (kHasTensor<N, fftoc::kKaratsubaOneStage>) {
+    using T = std::conditional_t<SMALL, TensorCfgSmall<N, fftoc::kKaratsubaOneStage>,
+                                 TensorCfg<N, fftoc::kKaratsubaOneStage>>;
+    if (family == 3) return launch_onchip<N, typename T::F>(data, batch, tw, repeats, scale, s);
+  }
   return FFTOC_ERROR_UNSUPPORTED;
+}
+
+template <bool SMALL>
+int onchip_n(int n, int family, void *data, int64_t batch, const void *tw, int repeats,
+             double scale, hipStream_t s) {
+  switch (n) {
+    case 8: return onchip<8, SMALL>(family, data, batch, tw, repeats, scale, s);
+    case 16: return onchip<16, SMALL>(family, data, batch, tw, repeats, scale, s);
+    case 32: return onchip<32, SMALL>(family, data, batch, tw, repeats, scale, s);
+    case 64: return onchip<64, SMALL>(family, data, batch, tw, repeats, scale, s);
+    case 128: return onchip<128, SMALL>(family, data, batch, tw, repeats, scale, s);
+    case 256: return onchip<256, SMALL>(family, data, batch, tw, repeats, scale, s);
+    case 512: return onchip<512, SMALL>(family, data, batch, tw, repeats, scale, s);
+    case 1024: return onchip<1024, SMALL>(family, data, batch, tw, repeats, scale, s);
+    case 2048: return onchip<2048, SMALL>(family, data, batch, tw, repeats, scale, s);
+    case 4096: return onchip<4096, SMALL>(family, data, batch, tw, repeats, scale, s);
+    default: return FFTOC_ERROR_INVALID_SIZE;
+  }
 }
 
 }  // namespace
 
 extern "C" {
+
+int fftoc_example_onchip_fft_shape(void *data, int64_t batch, int n, const void *tw, int family,
+                                   int shape, int repeats, double scale, void *stream);
 
 // Windowed (window != 0: Hann, w[e] = 0.5 - 0.5 cos(2 pi e / n)) forward or
 // inverse FP64 FFT of `batch` signals of n points, each output times `scale`.
@@ -269,24 +308,23 @@ int fftoc_example_windowed_tensor_fft(const void *in, void *out, int64_t batch, 
 // n = 256 and 4096).
 int fftoc_example_onchip_fft(void *data, int64_t batch, int n, const void *tw, int family,
                              int repeats, double scale, void *stream) {
+  return fftoc_example_onchip_fft_shape(data, batch, n, tw, family, 0, repeats, scale, stream);
+}
+
+// The same harness with a workgroup shape: shape 0 = the shapes above (up to
+// 256 threads and 64 KiB of signals per workgroup), 1 = one-wave workgroups
+// holding the fewest signals a functor unit takes (radix: up to 4; tensor: 32
+// at n = 8, 16 at 16, 4 at 64, 2 at 128, 1 from 256), for batches of a few
+// signals per CU.
+int fftoc_example_onchip_fft_shape(void *data, int64_t batch, int n, const void *tw, int family,
+                                   int shape, int repeats, double scale, void *stream) {
   if (batch < 0 || repeats < 0) return FFTOC_ERROR_INVALID_BATCH;
   if (batch == 0) return FFTOC_SUCCESS;
   if (!data || !tw) return FFTOC_ERROR_NULL_POINTER;
-  if (family < 0 || family > 3) return FFTOC_ERROR_INVALID_FLAGS;
+  if (family < 0 || family > 3 || shape < 0 || shape > 1) return FFTOC_ERROR_INVALID_FLAGS;
   hipStream_t s = static_cast<hipStream_t>(stream);
-  switch (n) {
-    case 8: return onchip<8>(family, data, batch, tw, repeats, scale, s);
-    case 16: return onchip<16>(family, data, batch, tw, repeats, scale, s);
-    case 32: return onchip<32>(family, data, batch, tw, repeats, scale, s);
-    case 64: return onchip<64>(family, data, batch, tw, repeats, scale, s);
-    case 128: return onchip<128>(family, data, batch, tw, repeats, scale, s);
-    case 256: return onchip<256>(family, data, batch, tw, repeats, scale, s);
-    case 512: return onchip<512>(family, data, batch, tw, repeats, scale, s);
-    case 1024: return onchip<1024>(family, data, batch, tw, repeats, scale, s);
-    case 2048: return onchip<2048>(family, data, batch, tw, repeats, scale, s);
-    case 4096: return onchip<4096>(family, data, batch, tw, repeats, scale, s);
-    default: return FFTOC_ERROR_INVALID_SIZE;
-  }
+  return shape ? onchip_n<true>(n, family, data, batch, tw, repeats, scale, s)
+               : onchip_n<false>(n, family, data, batch, tw, repeats, scale, s);
 }
 
 }  // extern "C"

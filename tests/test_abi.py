@@ -116,6 +116,14 @@ def test_fused_example_library_loads_and_checks_arguments():
     assert o(16, 4, 100, 32, 0, 1, 1.0, None) == 1
     assert o(16, 4, 32, 32, 1, 1, 1.0, None) == 6                # no tensor functor at 32
     assert o(16, 4, 8, 32, 2, 1, 1.0, None) == 6                 # n = 8: Karatsuba only
+    sh = lib.fftoc_example_onchip_fft_shape
+    sh.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
+                   ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_void_p]
+    sh.restype = ctypes.c_int
+    assert sh(None, 0, 64, None, 0, 1, 1, 1.0, None) == 0
+    assert sh(16, 4, 64, 32, 0, 2, 1, 1.0, None) == 4            # unknown shape
+    assert sh(16, 4, 64, 32, 4, 1, 1, 1.0, None) == 4            # unknown family
+    assert sh(16, 4, 32, 32, 1, 1, 1, 1.0, None) == 6            # no tensor functor at 32
 
 
 def test_plan_rejects_jobs_writing_into_each_other():

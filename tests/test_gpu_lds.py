@@ -41,7 +41,11 @@ def example():
     o.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
                   ctypes.c_int, ctypes.c_double, ctypes.c_void_p]
     o.restype = ctypes.c_int
-    f.tensor, f.onchip = t, o
+    sh = lib.fftoc_example_onchip_fft_shape
+    sh.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
+                   ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_void_p]
+    sh.restype = ctypes.c_int
+    f.tensor, f.onchip, f.onchip_shape = t, o, sh
     return f, tw
 
 
@@ -136,3 +140,23 @@ def test_onchip_harness_repeats_in_lds(example, n, family):
         ref = r.as_longdouble().astype(np.complex128) * scale  # input of the next repeat
         err = np.max(oracle.rel_l2(got, oracle.DD(r.hi * scale, r.lo * scale), axis=-1))
         assert err <= TOL, (n, family, repeats, err)
+
+
+@pytest.mark.parametrize("n,family", [(8, 0), (64, 0), (256, 0), (4096, 0), (8, 1), (16, 1),
+                                      (64, 1), (128, 1), (256, 1), (512, 1), (4096, 1), (16, 2),
+                                      (64, 2), (128, 2), (256, 2), (512, 2), (256, 3)])
+def test_onchip_harness_one_wave_shape(example, n, family):
+    """The harness's one-wave workgroup shape (the fewest signals a functor
+    unit takes; used at a few FFTs per CU) transforms like the oracle, over a
+    batch that leaves a partial last workgroup."""
+    f, tw = example
+    batch = 37
+    x = signals(n, batch, seed=950 + n)
+    scale = 1.0 / np.sqrt(n)
+    print(f"[onchip shape 1] n={n} family={family}", flush=True)
+    buf = fo.DeviceBuffer.from_array(x)
+    assert f.onchip_shape(buf.ptr, batch, n, tw, family, 1, 1, scale, None) == 0
+    fo.synchronize()
+    r = oracle.reference(x)
+    err = np.max(oracle.rel_l2(buf.download(x.shape), oracle.DD(r.hi * scale, r.lo * scale), axis=-1))
+    assert err <= TOL, (n, family, err)
