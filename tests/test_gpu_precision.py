@@ -79,3 +79,28 @@ def test_split_precision_rejected_where_not_built():
     with pytest.raises(fo.FftocError) as e:
         fo.fft(a, 1, 64, flags=fo.PREC_BF16X3)
     assert e.value.name == "UNSUPPORTED"
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
+def test_default_cap_keeps_results_bit_for_bit(prec):
+    """From 8192 signals the N = 4096 split-precision kernels run capped at
+    two (FP32) or three (split BF16) workgroups per CU by default; the cap
+    changes residency only: the output equals an uncapped launch bit for
+    bit, and the sampled signals stay within tolerance."""
+    n, batch = 4096, 8192
+    x = np.zeros((batch, n), np.complex128)
+    x[:32] = stress_signals(n, 32, seed=21)
+    x[-32:] = stress_signals(n, 32, seed=22)
+    src = fo.DeviceBuffer.from_array(x)
+    capped, uncapped = fo.DeviceBuffer(x.nbytes), fo.DeviceBuffer(x.nbytes)
+    print(f"[precision cap] {prec} n={n} batch={batch}", flush=True)
+    fo.tensor_fft(src, batch, n, out=capped, flags=fo.make_flags(precision=prec))
+    fo.tensor_fft(src, batch, n, out=uncapped, flags=fo.make_flags(precision=prec, per_cu=63))
+    fo.synchronize()
+    a = capped.download(x.shape)
+    assert np.array_equal(a, uncapped.download(x.shape))
+    for rows in (slice(0, 32), slice(batch - 32, batch)):
+        err = oracle.rel_l2(a[rows], oracle.reference(x[rows]), axis=-1)
+        assert err.max() <= TOL_SPLIT[prec]
+    for b in (src, capped, uncapped):
+        b.free()
