@@ -17,7 +17,7 @@ namespace {
 // LDS elements (of the data type) a routine's tile uses.
 constexpr int routine_lds(TForm f, int n, int s, bool c128) {
   if (f == TForm::Lanes) return 1;
-  if ((f == TForm::TensorK || f == TForm::TensorH) && c128) return 4096;
+  if ((f == TForm::TensorK || f == TForm::TensorH || f == TForm::TensorK1) && c128) return 4096;
   return lds_elems(s * n);
 }
 

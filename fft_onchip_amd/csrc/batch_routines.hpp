@@ -19,9 +19,10 @@ constexpr int kLdsElems = lds_elems(4096);
 // over the routine's tiles (Stream: the next tile's loads in flight during
 // this tile's passes); the lane-shuffle tile, pipelined (Lanes, N = 8 ... 64);
 // the N = 4096 matrix-core tile, pipelined, with the Karatsuba (TensorK) or
-// two-product (TensorH) DFT16 contraction (complex128; complex64 jobs run
-// the Stockham stream of the same split).
-enum class TForm { Tile, Staged, Stream, Lanes, TensorK, TensorH };
+// two-product (TensorH) DFT16 contraction, or Karatsuba on one of the
+// 256-point core's two DFT16 stages (TensorK1, the library's tensor default)
+// (complex128; complex64 jobs run the Stockham stream of the same split).
+enum class TForm { Tile, Staged, Stream, Lanes, TensorK, TensorH, TensorK1 };
 
 // N = 4096 = 16 x 256 on one 256-thread workgroup, as the library's tensor
 // kernel: stage A is the DFT16 over n1 of column m = thread (VALU) times
@@ -159,6 +160,10 @@ template <class SP, int S, int DIR>
 struct RoutineKernel<SP, S, TForm::TensorH, c2, DIR> {
   using type = TensorTile4096<TwoProductCore, DIR>;
 };
+template <class SP, int S, int DIR>
+struct RoutineKernel<SP, S, TForm::TensorK1, c2, DIR> {
+  using type = TensorTile4096<KaratsubaHalfCore, DIR>;
+};
 // which kernels a routine streams with run_stream (the others run run_tile
 // per tile)
 constexpr bool streams(TForm f) { return f != TForm::Tile && f != TForm::Staged; }
@@ -211,7 +216,8 @@ struct RoutineKernel<SP, S, TForm::Lanes, CT, DIR> {
   X(37, 4096, "tensor:16x16x16/stream/2-product/S1", (Split<16, 16, 16>), 1, TForm::TensorH) \
   X(38, 1024, "8x8x8x2/stream/S2", (Split<8, 8, 8, 2>), 2, TForm::Stream) \
   X(39, 2048, "8x8x8x4/stream/S1", (Split<8, 8, 8, 4>), 1, TForm::Stream) \
-  X(40, 4096, "8x8x8x8/stream/S1 (512 threads)", (Split<8, 8, 8, 8>), 1, TForm::Stream)
+  X(40, 4096, "8x8x8x8/stream/S1 (512 threads)", (Split<8, 8, 8, 8>), 1, TForm::Stream) \
+  X(41, 4096, "tensor:16x16x16/stream/one matrix stage/S1", (Split<16, 16, 16>), 1, TForm::TensorK1)
 
 #define FFTOC_UNPAREN(...) __VA_ARGS__
 
