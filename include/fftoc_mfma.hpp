@@ -324,6 +324,48 @@ __device__ __forceinline__ void swap_halves(c2 &p, c2 &q) {
   swap_halves<D>(p.y, q.y);
 }
 
+// DFT16 over n = 4r + g held as register r of the lanes j, j + 16, j + 32,
+// j + 48 (g = 0..3), on the vector ALUs as 4 x 4: a DFT4 over r in the lane,
+// the twiddle W_16^{g ka} (t16[ka - 1]), and a DFT4 over g across the lanes
+// (radix-2 over g1 = g / 2 between wave halves, then the twiddle
+// W_4^{g0 kb0} and radix-2 over g0 = g % 2 between rows, each exchange one
+// permlane swap per dword).  Register q of lane (g, j) then holds
+// X[lanes_k(g, q)].
+__device__ __forceinline__ int lanes_k(int g, int q) {
+  return (g >> 1) + 2 * (q >> 1) + 4 * (g & 1) + 8 * (q & 1);
+}
+__device__ __forceinline__ void dft16_lanes(c2 (&y)[4], const c2 (&t16)[3], int g) {
+  dft4(y[0], y[1], y[2], y[3]);  // over r: y[ka]
+#pragma unroll
+  for (int ka = 1; ka < 4; ++ka) y[ka] = cmul(y[ka], t16[ka - 1]);
+  // radix-2 over g1: lanes l and l + 32 (registers paired (ka, ka + 1); the
+  // lower lane keeps ka, the upper one ka + 1)
+  swap_halves<32>(y[0], y[1]);
+  swap_halves<32>(y[2], y[3]);
+  dft2(y[0], y[1]);
+  dft2(y[2], y[3]);
+  // radix-2 over g0: rows 2m and 2m + 1; the odd row's kb0 = 1 value times -i
+  const bool odd = g & 1;
+  swap_halves<16>(y[0], y[1]);
+  swap_halves<16>(y[2], y[3]);
+#pragma unroll
+  for (int p = 1; p < 4; p += 2) {
+    // selects per double (a select of two c2 values compiled to a select
+    // of their stack addresses)
+    const double re = y[p].x, im = y[p].y;
+    y[p].x = odd ? im : re;
+    y[p].y = odd ? -re : im;
+  }
+  dft2(y[0], y[1]);
+  dft2(y[2], y[3]);
+}
+// The lane's twiddles of dft16_lanes: t16[ka - 1] = W_16^{g ka}.
+__device__ __forceinline__ void dft16_lanes_twiddles(const c2 *__restrict__ tw, int g,
+                                                     c2 (&t16)[3]) {
+#pragma unroll
+  for (int ka = 1; ka < 4; ++ka) t16[ka - 1] = tw[((g * ka) & 15) << 8];
+}
+
 // 256-point core with one matrix-core stage: stage 1 (DFT16 over n1) on the
 // matrix cores with Karatsuba's three real products, as in core256; stage 2
 // (DFT16 over n2 = 4 r + g for each k1 = j) on the vector ALUs as 4 x 4: a
@@ -345,44 +387,16 @@ struct KaratsubaHalfCore {
     core256_twiddles(tw, lane, w);
     g = lane >> 4;
     j = lane & 15;
-#pragma unroll
-    for (int ka = 1; ka < 4; ++ka) t16[ka - 1] = tw[((g * ka) & 15) << 8];
+    dft16_lanes_twiddles(tw, g, t16);
   }
   __device__ __forceinline__ void run(c2 (&a)[4], c2 (&z)[4]) const {
     v4d yr, yi;
     kgemm(a, f, yr, yi);
-    c2 y[4];
 #pragma unroll
-    for (int r = 0; r < 4; ++r) y[r] = cmul(make_double2(yr[r], yi[r]), w[r]);
-    dft4(y[0], y[1], y[2], y[3]);  // over r: y[ka]
-#pragma unroll
-    for (int ka = 1; ka < 4; ++ka) y[ka] = cmul(y[ka], t16[ka - 1]);
-    // radix-2 over g1: lanes l and l + 32 (registers paired (ka, ka + 1); the
-    // lower lane keeps ka, the upper one ka + 1)
-    swap_halves<32>(y[0], y[1]);
-    swap_halves<32>(y[2], y[3]);
-    dft2(y[0], y[1]);
-    dft2(y[2], y[3]);
-    // radix-2 over g0: rows 2m and 2m + 1; the odd row's kb0 = 1 value times -i
-    const bool odd = g & 1;
-    swap_halves<16>(y[0], y[1]);
-    swap_halves<16>(y[2], y[3]);
-#pragma unroll
-    for (int p = 1; p < 4; p += 2) {
-      // selects per double (a select of two c2 values compiled to a select
-      // of their stack addresses)
-      const double re = y[p].x, im = y[p].y;
-      y[p].x = odd ? im : re;
-      y[p].y = odd ? -re : im;
-    }
-    dft2(y[0], y[1]);
-    dft2(y[2], y[3]);
-#pragma unroll
-    for (int q = 0; q < 4; ++q) z[q] = y[q];
+    for (int r = 0; r < 4; ++r) z[r] = cmul(make_double2(yr[r], yi[r]), w[r]);
+    dft16_lanes(z, t16, g);
   }
-  __device__ static __forceinline__ int out_k2(int g, int q, int j) {
-    return j + 16 * ((g >> 1) + 2 * (q >> 1) + 4 * (g & 1) + 8 * (q & 1));
-  }
+  __device__ static __forceinline__ int out_k2(int g, int q, int j) { return j + 16 * lanes_k(g, q); }
   __device__ __forceinline__ int out_index(int q) const { return out_k2(g, q, j); }
 };
 
