@@ -889,6 +889,7 @@ FFTOC_TENSOR_LAUNCHER(launch_t4096t, tensor4096t_kernel, 1, 64 * k4096Waves)
 FFTOC_TENSOR_LAUNCHER(launch_t4096h, tensor4096h_kernel, 1, 256, 4, KaratsubaCore)
 FFTOC_TENSOR_LAUNCHER(launch_t4096hh, tensor4096h_kernel, 1, 256, 4, TwoProductCore)
 FFTOC_TENSOR_LAUNCHER(launch_t4096h1, tensor4096h_kernel, 1, 256, 4, KaratsubaHalfCore)
+FFTOC_TENSOR_LAUNCHER(launch_t4096h1h, tensor4096h_kernel, 1, 256, 4, TwoProductHalfCore)
 #undef FFTOC_TENSOR_LAUNCHER
 
 constexpr unsigned TF = FAM_TENSOR | FAM_ANY;
@@ -927,6 +928,7 @@ const Variant kT512[] = {{"tensor:16x2x16/1 per wave/persistent", TF, &launch_t5
 // and 0.52 for three (FP64 matrix products on random data are power-bound:
 // the two-stage form runs at 0.785 on zeros).
 const Variant kT4096[] = {{"tensor:16x16x16/VALU stages A and C, 4 waves", TF, &launch_t4096h1, 1},
+                          {"tensor:16x16x16/VALU stages A and C, 4 waves/2-product", TF, &launch_t4096h1h, 1},
                           {"tensor:16x16x16/VALU stage A, 4 waves", TF, &launch_t4096h, 1},
                           {"tensor:16x16x16/VALU stage A, 4 waves/2-product", TF, &launch_t4096hh, 1},
                           {"tensor:16x16x16/8 waves", TF, &launch_t4096, 1},

@@ -400,6 +400,35 @@ struct KaratsubaHalfCore {
   __device__ __forceinline__ int out_index(int q) const { return out_k2(g, q, j); }
 };
 
+// The one-matrix-stage core with the two-product contraction on its matrix
+// stage: stage 1 gives Y[n2 = 4r + g][k1 = hphi(j)] (hgemm_a), twiddled by
+// W_256^{n2 k1}, then the same VALU stage 2.  8 MFMAs per core.
+struct TwoProductHalfCore {
+  HFrag f;
+  HCoef k;
+  c2 w[4];    // W_256^{(4r + g) hphi(j)}
+  c2 t16[3];  // W_16^{g ka}
+  int g, j;
+  __device__ __forceinline__ void load(const c2 *__restrict__ tw, int lane) {
+    f = hdft16_frag(tw, lane);
+    g = lane >> 4;
+    j = lane & 15;
+    k = hcoef(j);
+    core256h_twiddles(tw, lane, w);
+    dft16_lanes_twiddles(tw, g, t16);
+  }
+  __device__ __forceinline__ void run(c2 (&a)[4], c2 (&z)[4]) const {
+    hgemm_a(a, f, k, z);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) z[r] = cmul(z[r], w[r]);
+    dft16_lanes(z, t16, g);
+  }
+  __device__ static __forceinline__ int out_k2(int g, int q, int j) {
+    return hphi(j) + 16 * lanes_k(g, q);
+  }
+  __device__ __forceinline__ int out_index(int q) const { return out_k2(g, q, j); }
+};
+
 // LDS element swizzle for the output transposes: XOR the low 4 bits of the
 // element index with bits 4..7 (16 x 16-byte slots per 256 B bank window).
 __device__ __forceinline__ int swz16(int e) { return e ^ ((e >> 4) & 15); }
