@@ -61,15 +61,19 @@ def run_gpu(fo, wl: Workload, steps: int, warmup: int, host_x: np.ndarray | None
     from fft_onchip_amd import hip
 
     n, batch = wl.n, wl.batch
-    nbytes = n * batch * 16
+    c64 = (wl.flags & fo.DATA_C64) != 0  # complex64 data (8 B per point)
+    es = 8 if c64 else 16
+    nbytes = n * batch * es
     x = gen_signals(n, batch, wl.seed, wl.kind) if host_x is None else host_x
+    if c64:
+        x = x.astype(np.complex64)
     # enough in/out pairs that the rotated footprint is >= 4x the 256 MB MALL
     pairs = max(1, min(8, math.ceil(4 * MALL_BYTES / (2 * nbytes))))
     ins = [hip.DeviceBuffer.from_array(x) for _ in range(pairs)]
     outs = [hip.DeviceBuffer(nbytes) for _ in range(pairs)]
     stream = hip.Stream()
     if wl.path.startswith("variant:"):  # one named kernel variant
-        vidx = fo.variant_index(n, wl.path[len("variant:"):])
+        vidx = fo.variant_index(n, wl.path[len("variant:"):], c64=c64)
 
         def fn(x_, b_, n_, out=None, flags=0, stream=None):
             fo.fft_variant(x_, b_, n_, vidx, out=out, flags=flags, stream=stream)
@@ -112,14 +116,14 @@ def run_gpu(fo, wl: Workload, steps: int, warmup: int, host_x: np.ndarray | None
     stream.synchronize()
     launch_ms = [a.elapsed_ms(b) for a, b in ev]
     result = outs[(steps - 1) % pairs]
-    sample = result.download((min(batch, 256), n))
+    sample = result.download((min(batch, 256), n), dtype=np.complex64 if c64 else np.complex128)
     for b in ins + outs:
         b.free()
     # average launch duration = HIP events around the timed region / K (the
     # per-launch event pairs add ~2 us of their own to a 25 us launch)
     mean_launch_ms = total_ms / steps
     ffts_per_s = batch * steps / (total_ms * 1e-3)
-    alg_bytes = 32.0 * n * batch
+    alg_bytes = 2.0 * es * n * batch
     achieved = alg_bytes / (mean_launch_ms * 1e-3) / 1e9
     flops = 5.0 * n * math.log2(n) * batch
     return {
@@ -898,6 +902,11 @@ def main():
             if n in (16, 256):
                 others.append(Workload(f"fp64_n{n}_b{(1 << 24) // n}_legacy16", n, (1 << 24) // n,
                                        1 + i, "uniform", "legacy16"))
+        # complex64 data (FFTOC_DATA_C64, FP32 arithmetic), the input bytes of
+        # C3 and C1: 8 B per point in and out
+        for n, b, seed in ((4096, 32768, 7), (64, 131072, 0)):
+            others.append(Workload(f"c64_n{n}_b{b}_best", n, b, seed,
+                                   "normal" if n == 4096 else "uniform", "best", fo.DATA_C64))
         # the lengths outside C2, at the same 2^24 points
         for n, seed in ((32, 21), (1024, 22), (2048, 23)):
             for path in ("best", "tensor") if n == 32 else ("best",):
