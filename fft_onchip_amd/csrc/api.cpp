@@ -175,6 +175,17 @@ Default default_best(int n, int bucket) {
   }
 }
 
+// complex64 data: the pipelined form (with a cap of workgroups per CU) where
+// it measured ahead of the list's first variant; elsewhere that first one.
+Default default_c64(int n, int bucket) {
+  switch (n) {
+    case 64: return {"radix:4x4x4/pipelined/S16", 3};
+    case 128: return bucket <= 18 ? Default{"radix:8x4x4/pipelined/S16", 4} : Default{nullptr, 0};
+    case 512: return {"radix:8x8x8/pipelined/S4", 2};
+    default: return {nullptr, 0};
+  }
+}
+
 // Default choice of fftoc_tensor_fft where it depends on the batch size (else
 // the first tensor variant of the length).
 const char *default_tensor(int n, int bucket) {
@@ -207,6 +218,7 @@ void build_registry() {
         // one of their variants
         Default want{nullptr, 0};
         if (!c64 && f != Family::Tensor) want = default_best(n, b);
+        if (c64 && f != Family::Tensor) want = default_c64(n, b);
         if (!c64 && f == Family::Tensor) want.name = default_tensor(n, b);
         if (want.name) {
           for (size_t i = 0; i < r.v.size(); ++i)

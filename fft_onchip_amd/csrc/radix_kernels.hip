@@ -47,18 +47,18 @@ __global__ __launch_bounds__((kThreads<SP, S>)) void stockham_kernel(
   StockhamKernel<SP, S, INV, STAGED, CT>::run(in, out, batch, tw, scale);
 }
 
-template <class SP, int S, bool INV>
+template <class SP, int S, bool INV, class CT>
 __global__ __launch_bounds__((kThreads<SP, S>)) __attribute__((amdgpu_waves_per_eu(2)))
 void stockham_persistent_kernel(
-    const c2 *in, c2 *out, int64_t batch, const c2 *__restrict__ tw, double scale) {
-  StockhamKernel<SP, S, INV, false>::run_persistent(in, out, batch, tw, scale);
+    const CT *in, CT *out, int64_t batch, const CT *__restrict__ tw, real_t<CT> scale) {
+  StockhamKernel<SP, S, INV, false, CT>::run_persistent(in, out, batch, tw, scale);
 }
 
-template <class SP, int S, bool INV>
+template <class SP, int S, bool INV, class CT>
 __global__ __launch_bounds__((kThreads<SP, S>)) __attribute__((amdgpu_waves_per_eu(2)))
 void stockham_pipelined_kernel(
-    const c2 *in, c2 *out, int64_t batch, const c2 *__restrict__ tw, double scale) {
-  StockhamKernel<SP, S, INV, false>::run_pipelined_persistent(in, out, batch, tw, scale);
+    const CT *in, CT *out, int64_t batch, const CT *__restrict__ tw, real_t<CT> scale) {
+  StockhamKernel<SP, S, INV, false, CT>::run_pipelined_persistent(in, out, batch, tw, scale);
 }
 
 // N = 8 / 16 with the butterflies across lanes (lane_fft.hpp): each wave
@@ -106,9 +106,9 @@ enum class Form { OneTile, Staged, Persistent, Pipelined };
 template <class SP, int S, Form F, class CT>
 constexpr auto split_kernel() {
   if constexpr (F == Form::Persistent)
-    return stockham_persistent_kernel<SP, S, false>;
+    return stockham_persistent_kernel<SP, S, false, CT>;
   else if constexpr (F == Form::Pipelined)
-    return stockham_pipelined_kernel<SP, S, false>;
+    return stockham_pipelined_kernel<SP, S, false, CT>;
   else
     return stockham_kernel<SP, S, false, F == Form::Staged, CT>;
 }
@@ -127,27 +127,25 @@ int launch_split(const LaunchArgs &a) {
     return FFTOC_SUCCESS;
   }
   if constexpr (F == Form::Persistent) {
-    static_assert(sizeof(CT) == 16, "persistent form is FP64 only");
     const int64_t resident =
-        resident_workgroups<stockham_persistent_kernel<SP, S, false>, threads>(a.per_cu);
+        resident_workgroups<stockham_persistent_kernel<SP, S, false, CT>, threads>(a.per_cu);
     const unsigned grid = (unsigned)(ntiles < resident ? ntiles : resident);
     if (a.inverse)
-      hipLaunchKernelGGL((stockham_persistent_kernel<SP, S, true>), dim3(grid), dim3(threads), 0,
-                         a.stream, in, out, a.batch, tw, a.scale);
+      hipLaunchKernelGGL((stockham_persistent_kernel<SP, S, true, CT>), dim3(grid), dim3(threads), 0,
+                         a.stream, in, out, a.batch, tw, scale);
     else
-      hipLaunchKernelGGL((stockham_persistent_kernel<SP, S, false>), dim3(grid), dim3(threads), 0,
-                         a.stream, in, out, a.batch, tw, a.scale);
+      hipLaunchKernelGGL((stockham_persistent_kernel<SP, S, false, CT>), dim3(grid), dim3(threads), 0,
+                         a.stream, in, out, a.batch, tw, scale);
   } else if constexpr (F == Form::Pipelined) {
-    static_assert(sizeof(CT) == 16, "pipelined form is FP64 only");
     const int64_t resident =
-        resident_workgroups<stockham_pipelined_kernel<SP, S, false>, threads>(a.per_cu);
+        resident_workgroups<stockham_pipelined_kernel<SP, S, false, CT>, threads>(a.per_cu);
     const unsigned grid = (unsigned)(ntiles < resident ? ntiles : resident);
     if (a.inverse)
-      hipLaunchKernelGGL((stockham_pipelined_kernel<SP, S, true>), dim3(grid), dim3(threads), 0,
-                         a.stream, in, out, a.batch, tw, a.scale);
+      hipLaunchKernelGGL((stockham_pipelined_kernel<SP, S, true, CT>), dim3(grid), dim3(threads), 0,
+                         a.stream, in, out, a.batch, tw, scale);
     else
-      hipLaunchKernelGGL((stockham_pipelined_kernel<SP, S, false>), dim3(grid), dim3(threads), 0,
-                         a.stream, in, out, a.batch, tw, a.scale);
+      hipLaunchKernelGGL((stockham_pipelined_kernel<SP, S, false, CT>), dim3(grid), dim3(threads), 0,
+                         a.stream, in, out, a.batch, tw, scale);
   } else {
     constexpr bool staged = (F == Form::Staged);
     const size_t lds = occupancy_lds<split_kernel<SP, S, F, CT>(), threads>(a.per_cu);
@@ -272,21 +270,34 @@ const Variant kW32[] = {FFTOC_LANE("radix:32/lanes/S64", MIX, 32, 8, f2),
 // complex64: with 8-byte values the T lanes of a signal read T*8 bytes per
 // instruction, so splits with a smaller largest radix (more lanes per signal)
 // read longer runs: N=64 4x4x4 and N=128 8x4x4 (T = 16) stream at 5.8 TB/s
-// against 4.9 / 4.3 for 8x8 / 16x8 (T = 8).
+// against 4.9 / 4.3 for 8x8 / 16x8 (T = 8).  A complex64 signal is half the
+// bytes of a complex128 one for the same per-signal work (twiddle fetches,
+// index arithmetic, LDS exchanges), so the pipelined form -- each lane's base
+// twiddles loaded once before its tile loop, the next tile's loads in flight
+// during the current tile -- gains more here: +5-12 % at N = 64, 128, 512
+// from 2^14 to 2^18 signals.
+// Splits of radix 4 throughout (N = 256 4x4x4x4, N = 128 4x4x4x2) read
+// longer runs still but pay a fourth LDS exchange, and measured 10-30 % below.
 const Variant kW64[] = {FFTOC_V32("radix:4x4x4/tile/S16", MIX, (Split<4, 4, 4>, 16, Form::OneTile)),
                         FFTOC_V32("radix:8x8/staged/S32", R8, (Split<8, 8>, 32, Form::Staged)),
                         FFTOC_V32("radix:8x8/tile/S16", R8, (Split<8, 8>, 16, Form::OneTile)),
                         FFTOC_LANE("radix:64/lanes/S16", MIX, 64, 4, f2),
-                        FFTOC_LANE("radix:64/lanes/S32", MIX, 64, 8, f2)};
+                        FFTOC_LANE("radix:64/lanes/S32", MIX, 64, 8, f2),
+                        FFTOC_V32("radix:4x4x4/pipelined/S16", MIX, (Split<4, 4, 4>, 16, Form::Pipelined)),
+                        FFTOC_V32("radix:4x4x4/pipelined/S32", MIX, (Split<4, 4, 4>, 32, Form::Pipelined))};
 const Variant kW128[] = {FFTOC_V32("radix:8x4x4/tile/S16", MIX, (Split<8, 4, 4>, 16, Form::OneTile)),
                          FFTOC_V32("radix:16x8/tile/S8", MIX, (Split<16, 8>, 8, Form::OneTile)),
-                         FFTOC_V32("radix:8x8x2/tile/S16", MIX, (Split<8, 8, 2>, 16, Form::OneTile))};
+                         FFTOC_V32("radix:8x8x2/tile/S16", MIX, (Split<8, 8, 2>, 16, Form::OneTile)),
+                         FFTOC_V32("radix:8x4x4/pipelined/S16", MIX, (Split<8, 4, 4>, 16, Form::Pipelined))};
 const Variant kW256[] = {FFTOC_V32("radix:16x16/tile/S4", R16, (Split<16, 16>, 4, Form::OneTile))};
-const Variant kW512[] = {FFTOC_V32("radix:8x8x8/tile/S2", R8, (Split<8, 8, 8>, 2, Form::OneTile))};
-const Variant kW1024[] = {FFTOC_V32("radix:8x8x16/tile/S2", MIX, (Split<8, 8, 16>, 2, Form::OneTile))};
+const Variant kW512[] = {FFTOC_V32("radix:8x8x8/tile/S2", R8, (Split<8, 8, 8>, 2, Form::OneTile)),
+                         FFTOC_V32("radix:8x8x8/pipelined/S4", R8, (Split<8, 8, 8>, 4, Form::Pipelined))};
+const Variant kW1024[] = {FFTOC_V32("radix:8x8x16/tile/S2", MIX, (Split<8, 8, 16>, 2, Form::OneTile)),
+                          FFTOC_V32("radix:8x8x8x2/pipelined/S1", MIX, (Split<8, 8, 8, 2>, 1, Form::Pipelined))};
 const Variant kW2048[] = {FFTOC_V32("radix:8x16x16/tile/S1", MIX, (Split<8, 16, 16>, 1, Form::OneTile))};
 const Variant kW4096[] = {FFTOC_V32("radix:16x16x16/tile/S1", R16, (Split<16, 16, 16>, 1, Form::OneTile)),
-                          FFTOC_V32("radix:8x8x8x8/tile/S1", R8, (Split<8, 8, 8, 8>, 1, Form::OneTile))};
+                          FFTOC_V32("radix:8x8x8x8/tile/S1", R8, (Split<8, 8, 8, 8>, 1, Form::OneTile)),
+                          FFTOC_V32("radix:8x8x8x8/pipelined/S1", R8, (Split<8, 8, 8, 8>, 1, Form::Pipelined))};
 #undef FFTOC_V32
 #undef FFTOC_LANE
 #undef FFTOC_UNPAREN

@@ -228,7 +228,8 @@ def test_autotune_and_tuning_roundtrip(tmp_path, c64):
 
 
 @pytest.mark.parametrize("n,c64", [(8, False), (64, False), (256, False), (4096, False),
-                                   (64, True), (512, True)])
+                                   (64, True), (128, True), (256, True), (512, True),
+                                   (4096, True)])
 def test_workgroups_per_cu_caps_keep_results_bit_for_bit(n, c64):
     """FFTOC_WGS_PER_CU(k) holds a variant to k workgroups per CU -- a
     persistent grid of k per CU, or (one workgroup per tile) an LDS request
