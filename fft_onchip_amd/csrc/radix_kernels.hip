@@ -278,6 +278,9 @@ const Variant kW32[] = {FFTOC_LANE("radix:32/lanes/S64", MIX, 32, 8, f2),
 // from 2^14 to 2^18 signals.
 // Splits of radix 4 throughout (N = 256 4x4x4x4, N = 128 4x4x4x2) read
 // longer runs still but pay a fourth LDS exchange, and measured 10-30 % below.
+// At N = 4096 the one-signal radix-16 tile kernel stays ahead (0.74 of HBM at
+// 2^15 signals) of every pipelined form (8x8x8x8 S1 0.72; S2 and radix-16
+// pipelined 0.62-0.72) and of two signals per workgroup (0.72).
 const Variant kW64[] = {FFTOC_V32("radix:4x4x4/tile/S16", MIX, (Split<4, 4, 4>, 16, Form::OneTile)),
                         FFTOC_V32("radix:8x8/staged/S32", R8, (Split<8, 8>, 32, Form::Staged)),
                         FFTOC_V32("radix:8x8/tile/S16", R8, (Split<8, 8>, 16, Form::OneTile)),
