@@ -648,6 +648,50 @@ def copy_bandwidth(nbytes: int = 1 << 30, reps: int = 10) -> float:
     return 2.0 * nbytes / (ms * 1e-3) / 1e9
 
 
+def copy_ceiling(nbytes: int = 1 << 30, launches: int = 20, ramp: int = 200) -> dict:
+    """Plain copies of `nbytes` (read + write) moving 1, 4, 16 and 64 KiB per
+    workgroup (fft_onchip_amd/libfftoc_copy.so): the streaming rate a kernel
+    of that granularity can reach on this device.  A transform's workgroup
+    owns whole signals (64 KiB at N = 4096 complex128), so its rate is
+    compared with the copy at the same block size.  Median of 3 timed runs of
+    `launches` launches, after `ramp` untimed ones."""
+    import ctypes
+
+    from fft_onchip_amd import hip
+    lib = ctypes.CDLL(os.path.join(ROOT, "fft_onchip_amd", "libfftoc_copy.so"))
+    fn = lib.fftoc_copy_ceiling
+    fn.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_void_p]
+    fn.restype = ctypes.c_int
+    src, dst = hip.DeviceBuffer(nbytes), hip.DeviceBuffer(nbytes)
+    stream = hip.Stream()
+    hip.check(hip.lib().hipMemsetAsync(ctypes.c_void_p(src.ptr), 0, nbytes,
+                                       ctypes.c_void_p(stream.handle)))
+    out = {"method": "one-shot copy kernels, non-temporal 16 B per lane, 1 GiB in + 1 GiB out",
+           "GBps": {}, "frac": {}}
+    for kib in (1, 4, 16, 64):
+        def launch():
+            rc = fn(src.ptr, dst.ptr, nbytes, kib, stream.handle)
+            if rc != 0:
+                raise RuntimeError(f"fftoc_copy_ceiling({kib} KiB) returned {rc}")
+        for _ in range(ramp):
+            launch()
+        times = []
+        for _ in range(3):
+            a, b = hip.Event(), hip.Event()
+            a.record(stream)
+            for _ in range(launches):
+                launch()
+            b.record(stream)
+            stream.synchronize()
+            times.append(a.elapsed_ms(b) / launches)
+        gbs = 2.0 * nbytes / (sorted(times)[1] * 1e-3) / 1e9
+        out["GBps"][f"{kib} KiB"] = round(gbs, 1)
+        out["frac"][f"{kib} KiB"] = round(gbs / (HBM_PEAK_GBPS), 4)
+    src.free()
+    dst.free()
+    return out
+
+
 def cpu_baseline(x: np.ndarray, gpu_sample: np.ndarray, threads: int, budget_s: float = 12.0):
     """FP64 FFT of the same signals on the host cores (the oracle's Stockham
     transform, OpenMP over the batch); median of 5 timed runs."""
@@ -926,6 +970,9 @@ def main():
         extra["tensor_power_n4096"] = run_power_evidence(fo, args.steps, args.warmup, x)
         print(f"[bench] tensor power: {extra['tensor_power_n4096']}", file=sys.stderr, flush=True)
     extra["hip_d2d_copy_GBps"] = round(copy_bandwidth(), 1)
+    extra["copy_ceiling"] = copy_ceiling()
+    extra["copy_ceiling"]["headline_over_64KiB_copy"] = round(res["GBps"] / extra["copy_ceiling"]["GBps"]["64 KiB"], 3)
+    print(f"[bench] copy ceiling: {extra['copy_ceiling']}", file=sys.stderr, flush=True)
     rocfft_samples = {}
     if not args.quick:
         extra["rocfft_x4"] = {}

@@ -159,9 +159,12 @@ Default default_best(int n, int bucket) {
   switch (n) {
     case 64:  // up to 2^16 signals the persistent form hides the launch tail
       return {bucket <= 16 ? "radix:8x8/persistent/S32" : "radix:8x8/tile/S16", 0};
-    case 8: return {"radix:8/lanes/S128", 0};
-    case 16: return {"radix:16/lanes/S64", 0};
-    case 32: return {"radix:32/lanes/S32", 0};
+    // N = 8, 16, 32 from 2^18 / 2^17 / 2^16 signals: one-wave workgroups of
+    // 1-2 KiB, whose loads and stores stay in the compact, in-order address
+    // window the dispatcher gives small workgroups (+2-4 % over 256 threads)
+    case 8: return {bucket >= 18 ? "radix:8/lanes/S8/64 threads" : "radix:8/lanes/S128", 0};
+    case 16: return {bucket >= 17 ? "radix:16/lanes/S4/64 threads" : "radix:16/lanes/S64", 0};
+    case 32: return {bucket >= 16 ? "radix:32/lanes/S4/64 threads" : "radix:32/lanes/S32", 0};
     case 128: return {"radix:16x8/tile/S8", bucket >= 16 ? 6 : 0};
     case 256: return {"radix:16x16/tile/S4", bucket >= 15 ? 6 : 0};
     case 512:

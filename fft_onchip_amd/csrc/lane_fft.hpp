@@ -141,14 +141,15 @@ struct LaneFft {
   }
 };
 
-// One tile of S = 256 * C / N consecutive signals on a 256-thread workgroup:
-// wave w transforms elements [tile * S * N + w * C * 64, + C * 64), C load
-// instructions of 64 consecutive values.  INVM: 0 forward, 1 inverse, 2 the
-// `inv` argument.  Same interface as StockhamKernel::run_tile (the LDS
-// argument is unused).
-template <int N, int C, int INVM, class CT>
+// One tile of S = TH * C / N consecutive signals on a TH-thread workgroup
+// (256 unless given): wave w transforms elements [tile * S * N + w * C * 64,
+// + C * 64), C load instructions of 64 consecutive values.  INVM: 0 forward,
+// 1 inverse, 2 the `inv` argument.  Same interface as
+// StockhamKernel::run_tile (the LDS argument is unused).
+template <int N, int C, int INVM, class CT, int TH = 256>
 struct LaneTile {
-  static constexpr int THREADS = 256;
+  static constexpr int THREADS = TH;
+  static_assert(TH % 64 == 0 && TH <= 1024, "whole waves");
   static constexpr int S = THREADS * C / N;
   static_assert(S * N == THREADS * C, "whole signals per wave");
   static_assert(C * 64 <= kTwN, "prefetch dummies inside the twiddle table");

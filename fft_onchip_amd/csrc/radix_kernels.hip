@@ -61,43 +61,46 @@ void stockham_pipelined_kernel(
   StockhamKernel<SP, S, INV, false, CT>::run_pipelined_persistent(in, out, batch, tw, scale);
 }
 
-// N = 8 / 16 with the butterflies across lanes (lane_fft.hpp): each wave
-// transforms C * 64 / N signals, C load instructions of 64 consecutive values.
+// N = 8 ... 64 with the butterflies across lanes (lane_fft.hpp): each wave
+// transforms C * 64 / N signals, C load instructions of 64 consecutive
+// values; TH threads per workgroup.
 constexpr int kLaneThreads = 256;
 static_assert(LaneTile<16, 4, 0, c2>::THREADS == kLaneThreads);
 
-template <int N, int C, bool INV, class CT>
-__global__ __launch_bounds__(kLaneThreads) void lane_fft_kernel(
+template <int N, int C, bool INV, class CT, int TH>
+__global__ __launch_bounds__(TH) void lane_fft_kernel(
     const CT *in, CT *out, int64_t batch, const CT *__restrict__ tw, real_t<CT> scale) {
-  LaneTile<N, C, INV ? 1 : 0, CT>::run_tile(in, out, batch, blockIdx.x, nullptr, tw, scale);
+  LaneTile<N, C, INV ? 1 : 0, CT, TH>::run_tile(in, out, batch, blockIdx.x, nullptr, tw, scale);
 }
 
-template <int N, int C, class CT>
+template <int N, int C, class CT, int TH = kLaneThreads>
 int launch_lane(const LaunchArgs &a) {
-  constexpr auto K = lane_fft_kernel<N, C, false, CT>;
+  constexpr auto K = lane_fft_kernel<N, C, false, CT, TH>;
   if (a.occupancy) {
-    *a.occupancy = occupancy<K, kLaneThreads>().per_cu;
+    *a.occupancy = occupancy<K, TH>().per_cu;
     return FFTOC_SUCCESS;
   }
-  constexpr int64_t per_block = (int64_t)kLaneThreads * C;  // elements
+  constexpr int64_t per_block = (int64_t)TH * C;  // elements
   const int64_t blocks = (a.batch * N + per_block - 1) / per_block;
   if (blocks > 0x7fffffffLL) return FFTOC_ERROR_INVALID_BATCH;
   const CT *in = static_cast<const CT *>(a.in);
   CT *out = static_cast<CT *>(a.out);
   const CT *tw = static_cast<const CT *>(a.twiddles);
   const real_t<CT> scale = (real_t<CT>)a.scale;
-  const size_t lds = occupancy_lds<K, kLaneThreads>(a.per_cu);
+  const size_t lds = occupancy_lds<K, TH>(a.per_cu);
   if (a.inverse)
-    hipLaunchKernelGGL((lane_fft_kernel<N, C, true, CT>), dim3((unsigned)blocks),
-                       dim3(kLaneThreads), lds, a.stream, in, out, a.batch, tw, scale);
+    hipLaunchKernelGGL((lane_fft_kernel<N, C, true, CT, TH>), dim3((unsigned)blocks),
+                       dim3(TH), lds, a.stream, in, out, a.batch, tw, scale);
   else
-    hipLaunchKernelGGL((lane_fft_kernel<N, C, false, CT>), dim3((unsigned)blocks),
-                       dim3(kLaneThreads), lds, a.stream, in, out, a.batch, tw, scale);
+    hipLaunchKernelGGL((lane_fft_kernel<N, C, false, CT, TH>), dim3((unsigned)blocks),
+                       dim3(TH), lds, a.stream, in, out, a.batch, tw, scale);
   return hipGetLastError() == hipSuccess ? FFTOC_SUCCESS : FFTOC_ERROR_LAUNCH;
 }
 
 #define FFTOC_LANE(NAME, FAM, N, C, CT) \
   Variant { NAME, FAM, &launch_lane<N, C, CT>, kLaneThreads * (C) / (N) }
+#define FFTOC_LANE_T(NAME, FAM, N, C, CT, TH) \
+  Variant { NAME, FAM, &launch_lane<N, C, CT, TH>, (TH) * (C) / (N) }
 
 enum class Form { OneTile, Staged, Persistent, Pipelined };
 
@@ -177,6 +180,8 @@ const Variant kV8[] = {
     FFTOC_V("radix:8/staged/S128", R8, Split<8>, 128, Form::Staged),
     FFTOC_V("radix:8/staged/S256", R8, Split<8>, 256, Form::Staged),
     FFTOC_V("radix:4x2/staged/S128", MIX, Split<4, 2>, 128, Form::Staged),
+    FFTOC_LANE_T("radix:8/lanes/S8/64 threads", R8, 8, 1, c2, 64),
+    FFTOC_LANE_T("radix:8/lanes/S16/64 threads", R8, 8, 2, c2, 64),
 };
 const Variant kV16[] = {
     FFTOC_LANE("radix:16/lanes/S64", R16, 16, 4, c2),
@@ -184,6 +189,8 @@ const Variant kV16[] = {
     FFTOC_V("radix:16/staged/S64", R16, Split<16>, 64, Form::Staged),
     FFTOC_V("radix:16/staged/S128", R16, Split<16>, 128, Form::Staged),
     FFTOC_V("radix:4x4/staged/S64", MIX, Split<4, 4>, 64, Form::Staged),
+    FFTOC_LANE_T("radix:16/lanes/S4/64 threads", R16, 16, 1, c2, 64),
+    FFTOC_LANE_T("radix:16/lanes/S8/64 threads", R16, 16, 2, c2, 64),
 };
 const Variant kV32[] = {
     FFTOC_V("radix:8x4/staged/S32", MIX, Split<8, 4>, 32, Form::Staged),
@@ -193,6 +200,7 @@ const Variant kV32[] = {
     FFTOC_V("radix:4x4x2/staged/S64", MIX, Split<4, 4, 2>, 64, Form::Staged),
     FFTOC_LANE("radix:32/lanes/S32", MIX, 32, 4, c2),
     FFTOC_LANE("radix:32/lanes/S64", MIX, 32, 8, c2),
+    FFTOC_LANE_T("radix:32/lanes/S4/64 threads", MIX, 32, 2, c2, 64),
 };
 const Variant kV64[] = {
     FFTOC_V("radix:8x8/tile/S16", R8, Split<8, 8>, 16, Form::OneTile),
@@ -250,7 +258,10 @@ const Variant kV4096[] = {
     FFTOC_V("radix:8x8x8x8/persistent/S1", R8, Split<8, 8, 8, 8>, 1, Form::Persistent),
 };
 
-// complex64 data (FP32 arithmetic, float2 twiddles rounded from the FP64 table)
+// complex64 data (FP32 arithmetic, float2 twiddles rounded from the FP64 table).
+// The one-wave lane workgroups that lead for complex128 at N = 8, 16, 32 lose
+// here (N = 8 at 2^22 signals: 0.59 of HBM against 0.80): one load
+// instruction moves 512 B, too little work per wave.
 #define FFTOC_V32(NAME, FAM, SPLIT_S_FORM) \
   Variant { NAME, FAM, &launch_split<FFTOC_UNPAREN SPLIT_S_FORM, f2>, \
             tile_of<FFTOC_UNPAREN SPLIT_S_FORM, f2>() }
@@ -303,6 +314,7 @@ const Variant kW4096[] = {FFTOC_V32("radix:16x16x16/tile/S1", R16, (Split<16, 16
                           FFTOC_V32("radix:8x8x8x8/pipelined/S1", R8, (Split<8, 8, 8, 8>, 1, Form::Pipelined))};
 #undef FFTOC_V32
 #undef FFTOC_LANE
+#undef FFTOC_LANE_T
 #undef FFTOC_UNPAREN
 #undef FFTOC_V
 
