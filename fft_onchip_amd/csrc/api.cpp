@@ -193,6 +193,8 @@ Default default_c64(int n, int bucket) {
 // the first tensor variant of the length).
 const char *default_tensor(int n, int bucket) {
   if (n == 64 && bucket <= 16) return "tensor:16x4/8 per wave/persistent";
+  if (n == 8 && bucket >= 19) return "tensor:diag(F8,F8)/32 per wave/1 wave";
+  if (n == 32 && bucket >= 19) return "tensor:16x2/8 per wave/1 wave";
   return nullptr;
 }
 

@@ -885,6 +885,11 @@ FFTOC_TENSOR_LAUNCHER(launch_t256g4q, tensor256_kernel, 4 * kWaves, 64 * kWaves,
 FFTOC_TENSOR_LAUNCHER(launch_t512, tensor512_kernel, kWaves, 64 * kWaves, 1)
 FFTOC_TENSOR_LAUNCHER(launch_t512g2, tensor512_kernel, 2 * kWaves, 64 * kWaves, 2)
 FFTOC_TENSOR_LAUNCHER(launch_t4096, tensor4096_kernel, 1, 64 * k4096Waves)
+// One-wave workgroups: a workgroup moves 4 KiB instead of 16 KiB, which
+// streams faster at large batches (N = 8 x 2^21: 0.736 of HBM against 0.720;
+// N = 32 x 2^19: 0.777 against 0.767; N = 16, 64, 128 gained nothing).
+FFTOC_TENSOR_LAUNCHER(launch_t8w1, tensor8_kernel, 32, 64, 1)
+FFTOC_TENSOR_LAUNCHER(launch_t32w1, tensor16xm_kernel, 8, 64, 2, 1, KaratsubaDft16)
 FFTOC_TENSOR_LAUNCHER(launch_t4096t, tensor4096t_kernel, 1, 64 * k4096Waves)
 FFTOC_TENSOR_LAUNCHER(launch_t4096h, tensor4096h_kernel, 1, 256, 4, KaratsubaCore)
 FFTOC_TENSOR_LAUNCHER(launch_t4096hh, tensor4096h_kernel, 1, 256, 4, TwoProductCore)
@@ -896,13 +901,15 @@ constexpr unsigned TF = FAM_TENSOR | FAM_ANY;
 // tile: signals per workgroup (4 waves x signals per wave)
 const Variant kT8[] = {{"tensor:diag(F8,F8)/32 per wave", TF, &launch_t8, 128},
                        {"tensor:diag(F8,F8)/64 per wave", TF, &launch_t8g2, 256},
-                       {"tensor:diag(F8,F8)/128 per wave", TF, &launch_t8g4, 512}};
+                       {"tensor:diag(F8,F8)/128 per wave", TF, &launch_t8g4, 512},
+                       {"tensor:diag(F8,F8)/32 per wave/1 wave", TF, &launch_t8w1, 32}};
 const Variant kT16[] = {{"tensor:16/16 per wave", TF, &launch_t16, 64},
                         {"tensor:16/32 per wave", TF, &launch_t16g2, 128},
                         {"tensor:16/64 per wave", TF, &launch_t16g4, 256}};
 const Variant kT32[] = {{"tensor:16x2/16 per wave", TF, &launch_t32g2, 64},
                         {"tensor:16x2/8 per wave", TF, &launch_t32, 32},
-                        {"tensor:16x2/32 per wave", TF, &launch_t32g4, 128}};
+                        {"tensor:16x2/32 per wave", TF, &launch_t32g4, 128},
+                        {"tensor:16x2/8 per wave/1 wave", TF, &launch_t32w1, 8}};
 const Variant kT64[] = {{"tensor:16x4/8 per wave", TF, &launch_t64g2, 32},
                         {"tensor:16x4/8 per wave/persistent", TF, &launch_tensor_persistent<Unit16xM<4>, 2>, 32},
                         {"tensor:16x4/8 per wave/persistent/2-product", TF, &launch_tensor_persistent<Unit16xM<4, TwoProductDft16>, 2>, 32},
