@@ -190,12 +190,16 @@ Default default_c64(int n, int bucket) {
 }
 
 // Default choice of fftoc_tensor_fft where it depends on the batch size (else
-// the first tensor variant of the length).
-const char *default_tensor(int n, int bucket) {
-  if (n == 64 && bucket <= 16) return "tensor:16x4/8 per wave/persistent";
-  if (n == 8 && bucket >= 19) return "tensor:diag(F8,F8)/32 per wave/1 wave";
-  if (n == 32 && bucket >= 19) return "tensor:16x2/8 per wave/1 wave";
-  return nullptr;
+// the first tensor variant of the length).  N = 512 from 2^14 signals: the
+// persistent kernel with one-wave workgroups held to four per CU (one wave
+// per SIMD): 0.68-0.75 of HBM against 0.65-0.69 for four-wave workgroups at
+// their occupancy; three or five per CU measured 0.63-0.68 / 0.58.
+Default default_tensor(int n, int bucket) {
+  if (n == 64 && bucket <= 16) return {"tensor:16x4/8 per wave/persistent", 0};
+  if (n == 8 && bucket >= 19) return {"tensor:diag(F8,F8)/32 per wave/1 wave", 0};
+  if (n == 32 && bucket >= 19) return {"tensor:16x2/8 per wave/1 wave", 0};
+  if (n == 512 && bucket >= 14) return {"tensor:16x2x16/1 per wave/persistent/1 wave", 4};
+  return {nullptr, 0};
 }
 
 void build_registry() {
@@ -224,7 +228,7 @@ void build_registry() {
         Default want{nullptr, 0};
         if (!c64 && f != Family::Tensor) want = default_best(n, b);
         if (c64 && f != Family::Tensor) want = default_c64(n, b);
-        if (!c64 && f == Family::Tensor) want.name = default_tensor(n, b);
+        if (!c64 && f == Family::Tensor) want = default_tensor(n, b);
         if (want.name) {
           for (size_t i = 0; i < r.v.size(); ++i)
             if (std::strcmp(r.v[i].name, want.name) == 0 && (r.v[i].families & family_bit(f))) {

@@ -299,14 +299,14 @@ struct Unit16xM {
 // The twiddles come from a copy in LDS: read from HBM after those loads,
 // each twiddle's wait (the vector-memory counter retires in order) would
 // also wait for the whole prefetch.
-template <class U, int G, bool INV, bool SC>
-__global__ __launch_bounds__(64 * kWaves) void tensor_persistent_kernel(
+template <class U, int G, int WAVES, bool INV, bool SC>
+__global__ __launch_bounds__(64 * WAVES) void tensor_persistent_kernel(
     const c2 *in, c2 *out, int64_t batch, const c2 *__restrict__ tw, double scale) {
   __shared__ c2 twl[U::kTwElems];
   constexpr int SPG = U::kSignals * G;  // signals per group
   const int lane = lane_id(), g = lane >> 4, j = lane & 15;
   const int64_t groups = (batch + SPG - 1) / SPG;
-  const int64_t stride = (int64_t)gridDim.x * kWaves;
+  const int64_t stride = (int64_t)gridDim.x * WAVES;
   int64_t q = wave_global();
   const bool active = q < groups;  // uniform over the wave
   // The first group's loads go out before the twiddle table is copied into
@@ -347,35 +347,36 @@ __global__ __launch_bounds__(64 * kWaves) void tensor_persistent_kernel(
   }
 }
 
-template <class U, int G>
+template <class U, int G, int WAVES = kWaves>
 int launch_tensor_persistent(const LaunchArgs &a) {
   if (a.precision != FFTOC_PREC_FP64) return FFTOC_ERROR_UNSUPPORTED;
   if (a.occupancy) {
-    *a.occupancy = occupancy<tensor_persistent_kernel<U, G, false, false>, 64 * kWaves>().per_cu;
+    *a.occupancy =
+        occupancy<tensor_persistent_kernel<U, G, WAVES, false, false>, 64 * WAVES>().per_cu;
     return FFTOC_SUCCESS;
   }
   if (a.batch <= 0) return FFTOC_SUCCESS;
   constexpr int SPG = U::kSignals * G;
   const int64_t groups = (a.batch + SPG - 1) / SPG;
   const int64_t resident =
-      resident_workgroups<tensor_persistent_kernel<U, G, false, false>, 64 * kWaves>(a.per_cu);
-  const int64_t need = (groups + kWaves - 1) / kWaves;
-  const dim3 gd((unsigned)(need < resident ? need : resident)), bd(64 * kWaves);
+      resident_workgroups<tensor_persistent_kernel<U, G, WAVES, false, false>, 64 * WAVES>(a.per_cu);
+  const int64_t need = (groups + WAVES - 1) / WAVES;
+  const dim3 gd((unsigned)(need < resident ? need : resident)), bd(64 * WAVES);
   const c2 *in = static_cast<const c2 *>(a.in);
   c2 *out = static_cast<c2 *>(a.out);
   const c2 *tw = static_cast<const c2 *>(a.twiddles);
   const bool sc = a.scale != 1.0;
   if (a.inverse && sc)
-    hipLaunchKernelGGL((tensor_persistent_kernel<U, G, true, true>), gd, bd, 0, a.stream, in, out,
+    hipLaunchKernelGGL((tensor_persistent_kernel<U, G, WAVES, true, true>), gd, bd, 0, a.stream, in, out,
                        a.batch, tw, a.scale);
   else if (a.inverse)
-    hipLaunchKernelGGL((tensor_persistent_kernel<U, G, true, false>), gd, bd, 0, a.stream, in,
+    hipLaunchKernelGGL((tensor_persistent_kernel<U, G, WAVES, true, false>), gd, bd, 0, a.stream, in,
                        out, a.batch, tw, a.scale);
   else if (sc)
-    hipLaunchKernelGGL((tensor_persistent_kernel<U, G, false, true>), gd, bd, 0, a.stream, in,
+    hipLaunchKernelGGL((tensor_persistent_kernel<U, G, WAVES, false, true>), gd, bd, 0, a.stream, in,
                        out, a.batch, tw, a.scale);
   else
-    hipLaunchKernelGGL((tensor_persistent_kernel<U, G, false, false>), gd, bd, 0, a.stream, in,
+    hipLaunchKernelGGL((tensor_persistent_kernel<U, G, WAVES, false, false>), gd, bd, 0, a.stream, in,
                        out, a.batch, tw, a.scale);
   return hipGetLastError() == hipSuccess ? FFTOC_SUCCESS : FFTOC_ERROR_LAUNCH;
 }
@@ -516,12 +517,12 @@ __device__ __forceinline__ void unit512(const c2 (&a)[2][4], const Frag &f, cons
   }
 }
 
-template <int G, bool INV, bool SC>
-__global__ __launch_bounds__(64 * kWaves) void tensor512_kernel(const c2 *in, c2 *out,
-                                                                int64_t batch,
-                                                                const c2 *__restrict__ tw,
-                                                                double scale) {
-  __shared__ c2 lds[kWaves * 512];
+template <int G, int WAVES, bool INV, bool SC>
+__global__ __launch_bounds__(64 * WAVES) void tensor512_kernel(const c2 *in, c2 *out,
+                                                               int64_t batch,
+                                                               const c2 *__restrict__ tw,
+                                                               double scale) {
+  __shared__ c2 lds[WAVES * 512];
   const int lane = lane_id(), g = lane >> 4, j = lane & 15;
   const int64_t sw = wave_global() * G;
   if (sw >= batch) return;
@@ -541,15 +542,15 @@ __global__ __launch_bounds__(64 * kWaves) void tensor512_kernel(const c2 *in, c2
 // The twiddles are read from a copy in LDS: read from HBM after those loads,
 // each twiddle's wait (the vector-memory counter retires in order) would
 // also wait for the whole prefetch.
-template <bool INV, bool SC>
-__global__ __launch_bounds__(64 * kWaves) void tensor512p_kernel(const c2 *in, c2 *out,
-                                                                 int64_t batch,
-                                                                 const c2 *__restrict__ tw,
-                                                                 double scale) {
-  __shared__ c2 lds[kWaves * 512];
+template <int WAVES, bool INV, bool SC>
+__global__ __launch_bounds__(64 * WAVES) void tensor512p_kernel(const c2 *in, c2 *out,
+                                                                int64_t batch,
+                                                                const c2 *__restrict__ tw,
+                                                                double scale) {
+  __shared__ c2 lds[WAVES * 512];
   __shared__ c2 twl[kTw512LdsElems];
   const int lane = lane_id(), g = lane >> 4, j = lane & 15;
-  const int64_t stride = (int64_t)gridDim.x * kWaves;
+  const int64_t stride = (int64_t)gridDim.x * WAVES;
   int64_t s = wave_global();
   const bool active = s < batch;  // uniform over the wave
   // first signal's loads before the twiddle copy (see tensor_persistent_kernel)
@@ -796,36 +797,39 @@ int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 // Persistent launch of a one-signal-per-wave kernel: one wave per signal up
 // to the resident count.
 
-#define FFTOC_PERSISTENT_LAUNCHER(FN, KERNEL, UNITS_PER_WG)                                   \
+#define FFTOC_PERSISTENT_LAUNCHER(FN, KERNEL, WAVES)                                          \
   int FN(const LaunchArgs &a) {                                                               \
     if (a.precision != FFTOC_PREC_FP64) return FFTOC_ERROR_UNSUPPORTED;                       \
     if (a.occupancy) {                                                                        \
-      *a.occupancy = occupancy<KERNEL<false, false>, 64 * kWaves>().per_cu;                   \
+      *a.occupancy = occupancy<KERNEL<WAVES, false, false>, 64 * (WAVES)>().per_cu;           \
       return FFTOC_SUCCESS;                                                                   \
     }                                                                                         \
     if (a.batch <= 0) return FFTOC_SUCCESS;                                                   \
-    const int64_t resident = resident_workgroups<KERNEL<false, false>, 64 * kWaves>(a.per_cu); \
-    const int64_t need = ceil_div(a.batch, UNITS_PER_WG);                                     \
-    const dim3 gd((unsigned)(need < resident ? need : resident)), bd(64 * kWaves);            \
+    const int64_t resident =                                                                  \
+        resident_workgroups<KERNEL<WAVES, false, false>, 64 * (WAVES)>(a.per_cu);            \
+    const int64_t need = ceil_div(a.batch, (WAVES));                                          \
+    const dim3 gd((unsigned)(need < resident ? need : resident)), bd(64 * (WAVES));           \
     const c2 *in = static_cast<const c2 *>(a.in);                                             \
     c2 *out = static_cast<c2 *>(a.out);                                                       \
     const c2 *tw = static_cast<const c2 *>(a.twiddles);                                       \
     const bool sc = a.scale != 1.0;                                                           \
     if (a.inverse && sc)                                                                      \
-      hipLaunchKernelGGL((KERNEL<true, true>), gd, bd, 0, a.stream, in, out, a.batch, tw,     \
+      hipLaunchKernelGGL((KERNEL<WAVES, true, true>), gd, bd, 0, a.stream, in, out, a.batch, tw,     \
                          a.scale);                                                            \
     else if (a.inverse)                                                                       \
-      hipLaunchKernelGGL((KERNEL<true, false>), gd, bd, 0, a.stream, in, out, a.batch, tw,    \
+      hipLaunchKernelGGL((KERNEL<WAVES, true, false>), gd, bd, 0, a.stream, in, out, a.batch, tw,    \
                          a.scale);                                                            \
     else if (sc)                                                                              \
-      hipLaunchKernelGGL((KERNEL<false, true>), gd, bd, 0, a.stream, in, out, a.batch, tw,    \
+      hipLaunchKernelGGL((KERNEL<WAVES, false, true>), gd, bd, 0, a.stream, in, out, a.batch, tw,    \
                          a.scale);                                                            \
     else                                                                                      \
-      hipLaunchKernelGGL((KERNEL<false, false>), gd, bd, 0, a.stream, in, out, a.batch, tw,   \
+      hipLaunchKernelGGL((KERNEL<WAVES, false, false>), gd, bd, 0, a.stream, in, out, a.batch, tw,   \
                          a.scale);                                                            \
     return hipGetLastError() == hipSuccess ? FFTOC_SUCCESS : FFTOC_ERROR_LAUNCH;              \
   }
 FFTOC_PERSISTENT_LAUNCHER(launch_t512p, tensor512p_kernel, kWaves)
+// one wave (one 8 KiB signal) per workgroup
+FFTOC_PERSISTENT_LAUNCHER(launch_t512p1, tensor512p_kernel, 1)
 #undef FFTOC_PERSISTENT_LAUNCHER
 
 #define FFTOC_TENSOR_LAUNCHER(FN, KERNEL, UNITS_PER_WG, BLOCK, ...)                              \
@@ -882,8 +886,9 @@ FFTOC_TENSOR_LAUNCHER(launch_t256g2h, tensor256_kernel, 2 * kWaves, 64 * kWaves,
 FFTOC_TENSOR_LAUNCHER(launch_t256g4, tensor256_kernel, 4 * kWaves, 64 * kWaves, 4, KaratsubaCore)
 FFTOC_TENSOR_LAUNCHER(launch_t256g2q, tensor256_kernel, 2 * kWaves, 64 * kWaves, 2, KaratsubaHalfCore)
 FFTOC_TENSOR_LAUNCHER(launch_t256g4q, tensor256_kernel, 4 * kWaves, 64 * kWaves, 4, KaratsubaHalfCore)
-FFTOC_TENSOR_LAUNCHER(launch_t512, tensor512_kernel, kWaves, 64 * kWaves, 1)
-FFTOC_TENSOR_LAUNCHER(launch_t512g2, tensor512_kernel, 2 * kWaves, 64 * kWaves, 2)
+FFTOC_TENSOR_LAUNCHER(launch_t512, tensor512_kernel, kWaves, 64 * kWaves, 1, kWaves)
+FFTOC_TENSOR_LAUNCHER(launch_t512g2, tensor512_kernel, 2 * kWaves, 64 * kWaves, 2, kWaves)
+FFTOC_TENSOR_LAUNCHER(launch_t512w1, tensor512_kernel, 1, 64, 1, 1)
 FFTOC_TENSOR_LAUNCHER(launch_t4096, tensor4096_kernel, 1, 64 * k4096Waves)
 // One-wave workgroups: a workgroup moves 4 KiB instead of 16 KiB, which
 // streams faster at large batches (N = 8 x 2^21: 0.736 of HBM against 0.720;
@@ -928,7 +933,9 @@ const Variant kT256[] = {{"tensor:16x16/2 per wave", TF, &launch_t256g2, 8},
                          {"tensor:16x16/4 per wave/VALU stage 2", TF, &launch_t256g4q, 16}};
 const Variant kT512[] = {{"tensor:16x2x16/1 per wave/persistent", TF, &launch_t512p, 4},
                          {"tensor:16x2x16/1 per wave", TF, &launch_t512, 4},
-                         {"tensor:16x2x16/2 per wave", TF, &launch_t512g2, 8}};
+                         {"tensor:16x2x16/2 per wave", TF, &launch_t512g2, 8},
+                         {"tensor:16x2x16/1 per wave/persistent/1 wave", TF, &launch_t512p1, 1},
+                         {"tensor:16x2x16/1 per wave/1 wave", TF, &launch_t512w1, 1}};
 // The first variant is the default of fftoc_tensor_fft at N = 4096: one
 // matrix-core DFT16 stage (Karatsuba) between two vector-ALU radix-16 stages,
 // 0.73-0.75 of HBM on random data, against 0.68 for two matrix-core stages

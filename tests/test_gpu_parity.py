@@ -227,7 +227,7 @@ def test_autotune_and_tuning_roundtrip(tmp_path, c64):
     assert np.max(oracle.rel_l2(got, oracle.reference(x.astype(np.complex128)), axis=-1)) <= tol
 
 
-@pytest.mark.parametrize("n,c64", [(8, False), (64, False), (256, False), (4096, False),
+@pytest.mark.parametrize("n,c64", [(8, False), (64, False), (256, False), (512, False), (4096, False),
                                    (64, True), (128, True), (256, True), (512, True),
                                    (4096, True)])
 def test_workgroups_per_cu_caps_keep_results_bit_for_bit(n, c64):
