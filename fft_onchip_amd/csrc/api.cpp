@@ -190,14 +190,16 @@ Default default_c64(int n, int bucket) {
 }
 
 // Default choice of fftoc_tensor_fft where it depends on the batch size (else
-// the first tensor variant of the length).  N = 512 from 2^14 signals: the
-// persistent kernel with one-wave workgroups held to four per CU (one wave
-// per SIMD): 0.68-0.75 of HBM against 0.65-0.69 for four-wave workgroups at
-// their occupancy; three or five per CU measured 0.63-0.68 / 0.58.
+// the first tensor variant of the length).  N = 256 and 512 from 2^14
+// signals: persistent kernels with one-wave workgroups held to four per CU
+// (one wave per SIMD): N = 256 0.75-0.80 of HBM against 0.69-0.74, N = 512
+// 0.68-0.75 against 0.65-0.69 for four-wave workgroups at their occupancy;
+// three or five per CU measured lower (N = 512: 0.63-0.68 / 0.58).
 Default default_tensor(int n, int bucket) {
   if (n == 64 && bucket <= 16) return {"tensor:16x4/8 per wave/persistent", 0};
   if (n == 8 && bucket >= 19) return {"tensor:diag(F8,F8)/32 per wave/1 wave", 0};
   if (n == 32 && bucket >= 19) return {"tensor:16x2/8 per wave/1 wave", 0};
+  if (n == 256 && bucket >= 14) return {"tensor:16x16/1 per wave/persistent/1 wave", 4};
   if (n == 512 && bucket >= 14) return {"tensor:16x2x16/1 per wave/persistent/1 wave", 4};
   return {nullptr, 0};
 }

@@ -428,6 +428,90 @@ __global__ __launch_bounds__(64 * kWaves) void tensor256_kernel(const c2 *in, c2
   }
 }
 
+// Persistent N = 256: wave w handles signals w, w + (waves in the grid), ...;
+// the next signal's loads are issued before the current one is transformed.
+// Registered with one-wave workgroups, the fftoc_tensor_fft default from 2^14
+// signals at four workgroups per CU (one wave per SIMD): 0.75-0.80 of HBM
+// against 0.69-0.74 for the two-signal kernel above (three or five per CU:
+// 0.62-0.74; the one-matrix-stage and two-product cores within 1 % at 2^16
+// signals, 3 % lower at 2^18; the same form at N = 32 and 128 did not gain).
+// The core's DFT fragment and twiddles live in registers, loaded once before
+// the first signal's data, so no vector-memory load inside the loop waits
+// behind the prefetch.  No workgroup barrier: a wave without a signal leaves.
+template <class CORE, int WAVES, bool INV, bool SC>
+__global__ __launch_bounds__(64 * WAVES) void tensor256p_kernel(const c2 *in, c2 *out,
+                                                                int64_t batch,
+                                                                const c2 *__restrict__ tw,
+                                                                double scale) {
+  __shared__ c2 lds[WAVES * 256];
+  const int lane = lane_id();
+  const int64_t stride = (int64_t)gridDim.x * WAVES;
+  int64_t s = wave_global();
+  if (s >= batch) return;  // uniform over the wave
+  c2 *buf = lds + (threadIdx.x >> 6) * 256;
+  CORE core;
+  core.load(tw, lane);
+  c2 cur[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) cur[c] = ld_in<INV>(in + s * 256 + 64 * c + lane);
+  for (bool reuse = false;; reuse = true) {
+    const int64_t sn = s + stride;
+    c2 nxt[4];
+    if (sn < batch) {
+#pragma unroll
+      for (int c = 0; c < 4; ++c) nxt[c] = ld_in<INV>(in + sn * 256 + 64 * c + lane);
+    }
+    c2 z[4];
+    core.run(cur, z);
+    if (reuse) wave_lds_sync();  // the previous signal's LDS reads are done
+#pragma unroll
+    for (int r = 0; r < 4; ++r) buf[swz16(core.out_index(r))] = z[r];
+    wave_lds_sync();
+    c2 *y = out + s * 256;
+    c2 o[4];  // every read before the first store (see tensor4096h_kernel)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) o[c] = buf[swz16(64 * c + lane)];
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) st_out<INV, SC>(y + 64 * c + lane, o[c], scale);
+    if (sn >= batch) break;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) cur[c] = nxt[c];
+    s = sn;
+  }
+}
+
+template <class CORE, int WAVES>
+int launch_t256p(const LaunchArgs &a) {
+  if (a.precision != FFTOC_PREC_FP64) return FFTOC_ERROR_UNSUPPORTED;
+  constexpr auto K0 = tensor256p_kernel<CORE, WAVES, false, false>;
+  if (a.occupancy) {
+    *a.occupancy = occupancy<K0, 64 * WAVES>().per_cu;
+    return FFTOC_SUCCESS;
+  }
+  if (a.batch <= 0) return FFTOC_SUCCESS;
+  const int64_t resident = resident_workgroups<K0, 64 * WAVES>(a.per_cu);
+  const int64_t need = (a.batch + WAVES - 1) / WAVES;
+  const dim3 gd((unsigned)(need < resident ? need : resident)), bd(64 * WAVES);
+  const c2 *in = static_cast<const c2 *>(a.in);
+  c2 *out = static_cast<c2 *>(a.out);
+  const c2 *tw = static_cast<const c2 *>(a.twiddles);
+  const bool sc = a.scale != 1.0;
+  if (a.inverse && sc)
+    hipLaunchKernelGGL((tensor256p_kernel<CORE, WAVES, true, true>), gd, bd, 0, a.stream, in, out,
+                       a.batch, tw, a.scale);
+  else if (a.inverse)
+    hipLaunchKernelGGL((tensor256p_kernel<CORE, WAVES, true, false>), gd, bd, 0, a.stream, in,
+                       out, a.batch, tw, a.scale);
+  else if (sc)
+    hipLaunchKernelGGL((tensor256p_kernel<CORE, WAVES, false, true>), gd, bd, 0, a.stream, in,
+                       out, a.batch, tw, a.scale);
+  else
+    hipLaunchKernelGGL((tensor256p_kernel<CORE, WAVES, false, false>), gd, bd, 0, a.stream, in,
+                       out, a.batch, tw, a.scale);
+  return hipGetLastError() == hipSuccess ? FFTOC_SUCCESS : FFTOC_ERROR_LAUNCH;
+}
+
 // ---------------------------------------------------------------------------
 // N = 512 = 16 x 32, one signal per unit.  n = 32 n1 + n2, n2 = 16 h + n2'.
 // Stage 1 (two MFMA groups h = 0, 1): DFT16 over n1 -> Y_h[n2'][k1];
@@ -930,7 +1014,8 @@ const Variant kT256[] = {{"tensor:16x16/2 per wave", TF, &launch_t256g2, 8},
                          {"tensor:16x16/4 per wave", TF, &launch_t256g4, 16},
                          {"tensor:16x16/1 per wave", TF, &launch_t256, 4},
                          {"tensor:16x16/2 per wave/VALU stage 2", TF, &launch_t256g2q, 8},
-                         {"tensor:16x16/4 per wave/VALU stage 2", TF, &launch_t256g4q, 16}};
+                         {"tensor:16x16/4 per wave/VALU stage 2", TF, &launch_t256g4q, 16},
+                         {"tensor:16x16/1 per wave/persistent/1 wave", TF, &launch_t256p<KaratsubaCore, 1>, 1}};
 const Variant kT512[] = {{"tensor:16x2x16/1 per wave/persistent", TF, &launch_t512p, 4},
                          {"tensor:16x2x16/1 per wave", TF, &launch_t512, 4},
                          {"tensor:16x2x16/2 per wave", TF, &launch_t512g2, 8},
