@@ -197,8 +197,7 @@ Default default_c64(int n, int bucket) {
 // three or five per CU measured lower (N = 512: 0.63-0.68 / 0.58).
 Default default_tensor(int n, int bucket) {
   if (n == 64 && bucket <= 16) return {"tensor:16x4/8 per wave/persistent", 0};
-  if (n == 8 && bucket >= 19) return {"tensor:diag(F8,F8)/32 per wave/1 wave", 0};
-  if (n == 32 && bucket >= 19) return {"tensor:16x2/8 per wave/1 wave", 0};
+  if (n == 8 && bucket >= 18) return {"tensor:diag(F8,F8)/32 per wave/LDS-staged/1 wave", 0};
   if (n == 256 && bucket >= 14) return {"tensor:16x16/1 per wave/persistent/1 wave", 4};
   if (n == 512 && bucket >= 14) return {"tensor:16x2x16/1 per wave/persistent/1 wave", 4};
   return {nullptr, 0};

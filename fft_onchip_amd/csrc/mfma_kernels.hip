@@ -120,6 +120,68 @@ __global__ __launch_bounds__(64 * kWaves) void tensor16_kernel(const c2 *in, c2 
 }
 
 // ---------------------------------------------------------------------------
+// N = 8 and 16 staged through LDS: a wave loads its 4 KiB tile (32 signals at
+// N = 8, 16 at N = 16) with four fully contiguous 1 KiB load instructions,
+// moves it into the matrix-core layout through a wave-private LDS buffer,
+// runs the same DFT (diag(F8, F8), or F16, on the matrix cores), and stores
+// the results through the buffer again, 1 KiB contiguous per instruction.
+// The direct kernels above read and write 64-256 B pieces per instruction.
+// LDS element e lives at swz16(e).
+template <int N>
+__device__ __forceinline__ int staged_in(int c, int g, int j) {
+  if constexpr (N == 8) return ((c < 2 ? 0 : 16) + j) * 8 + 4 * (c & 1) + g;  // signal, element
+  else return j * 16 + 4 * c + g;
+}
+template <int N>
+__device__ __forceinline__ int staged_out(int r, int g, int j) {
+  if constexpr (N == 8) return ((j < 8 ? 0 : 16) + 4 * r + g) * 8 + (j & 7);
+  else return (4 * r + g) * 16 + j;
+}
+
+template <int N, int WAVES, bool INV, bool SC>
+__global__ __launch_bounds__(64 * WAVES) void tensor_staged_kernel(const c2 *in, c2 *out,
+                                                                   int64_t batch,
+                                                                   const c2 *__restrict__ tw,
+                                                                   double scale) {
+  static_assert(N == 8 || N == 16, "N = 8 or 16");
+  __shared__ c2 lds[WAVES * 256];
+  const int lane = lane_id(), g = lane >> 4, j = lane & 15;
+  const int64_t e0 = wave_global() * 256;  // first element of the wave's tile
+  const int64_t total = batch * N;
+  if (e0 >= total) return;
+  c2 *buf = lds + (threadIdx.x >> 6) * 256;
+  const Frag f = N == 8 ? dft8x2_frag(tw, lane) : dft16_frag(tw, lane);
+  c2 v[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    // past the end: read the last element (no branch) and use zero
+    const int64_t e = e0 + 64 * c + lane;
+    v[c] = ld_in<INV>(in + (e < total ? e : total - 1));
+    if (e >= total) v[c] = make_double2(0.0, 0.0);
+  }
+#pragma unroll
+  for (int c = 0; c < 4; ++c) buf[swz16(64 * c + lane)] = v[c];
+  wave_lds_sync();
+  c2 a[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) a[c] = buf[swz16(staged_in<N>(c, g, j))];
+  v4d yr, yi;
+  kgemm(a, f, yr, yi);
+  wave_lds_sync();  // every lane has read its operands before they are overwritten
+#pragma unroll
+  for (int r = 0; r < 4; ++r) buf[swz16(staged_out<N>(r, g, j))] = make_double2(yr[r], yi[r]);
+  wave_lds_sync();
+  c2 o[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) o[c] = buf[swz16(64 * c + lane)];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const int64_t e = e0 + 64 * c + lane;
+    if (e < total) st_out<INV, SC>(out + e, o[c], scale);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // N = 16 * M with M in {2, 4, 8}: n = M n1 + n2.  Rows of the MFMA are
 // (signal, n2) pairs, SPW = 16/M signals per wave, row i <-> (signal i % SPW,
 // n2 = i / SPW).  Stage 1: DFT16 over n1 on MFMA; then twiddle W_N^{n2 k1}
@@ -168,14 +230,33 @@ struct TwNRegs {
   __device__ c2 get(int r, int) const { return w[r]; }
 };
 
-// Transforms and stores the unit whose first signal is s0: the DFT16 over n1
-// on the matrix cores (contraction D), whose lane column holds frequency
-// k1 = d.col, then the twiddle W_N^{n2 k1} and the DFT_M over n2.
-template <int M, bool INV, bool SC, class TW = TwNGlobal<16 * M>, class D = KaratsubaDft16>
-__device__ __forceinline__ void unit16xm(const c2 (&a)[4], const D &d, const TW &tw, c2 *out,
-                                         int64_t s0, int64_t batch, double scale, int g) {
+// Where unit16xm puts result k of the unit's signal t (t < 16 / M): the
+// output in HBM (signal s0 + t, when it exists), or a wave's LDS tile.
+template <int N, bool INV, bool SC>
+struct HbmSink {
+  c2 *out;
+  int64_t s0, batch;
+  double scale;
+  __device__ bool has(int t) const { return s0 + t < batch; }
+  __device__ void operator()(int t, int k, c2 v) const {
+    st_out<INV, SC>(out + (s0 + t) * N + k, v, scale);
+  }
+};
+template <int N>
+struct LdsSink {
+  c2 *buf;  // the unit's 16 / M * N = 256 values, element e at swz16(e)
+  __device__ bool has(int) const { return true; }
+  __device__ void operator()(int t, int k, c2 v) const { buf[swz16(t * N + k)] = v; }
+};
+
+// Transforms the unit whose first signal is s0 and hands its results to
+// `put`: the DFT16 over n1 on the matrix cores (contraction D), whose lane
+// column holds frequency k1 = d.col, then the twiddle W_N^{n2 k1} and the
+// DFT_M over n2.
+template <int M, class TW, class D, class SINK>
+__device__ __forceinline__ void unit16xm_to(const c2 (&a)[4], const D &d, const TW &tw,
+                                            const SINK &put, int g) {
   constexpr int SPW = 16 / M;
-  constexpr int N = 16 * M;
   const int j = d.col;  // the frequency k1 of this lane's results
   c2 y[4];
   d.run(a, y);
@@ -188,20 +269,19 @@ __device__ __forceinline__ void unit16xm(const c2 (&a)[4], const D &d, const TW 
     // registers (0, 2) -> signal g, (1, 3) -> signal 4 + g; DFT2 over n2
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
-      const int64_t s = s0 + 4 * h + g;
+      const int t = 4 * h + g;
       c2 u0 = y[h], u1 = y[h + 2];
       dft2(u0, u1);
-      if (s < batch) {
-        st_out<INV, SC>(out + s * N + j, u0, scale);
-        st_out<INV, SC>(out + s * N + j + 16, u1, scale);
+      if (put.has(t)) {
+        put(t, j, u0);
+        put(t, j + 16, u1);
       }
     }
   } else if constexpr (M == 4) {
     dft4(y[0], y[1], y[2], y[3]);
-    const int64_t s = s0 + g;
-    if (s < batch) {
+    if (put.has(g)) {
 #pragma unroll
-      for (int k2 = 0; k2 < 4; ++k2) st_out<INV, SC>(out + s * N + j + 16 * k2, y[k2], scale);
+      for (int k2 = 0; k2 < 4; ++k2) put(g, j + 16 * k2, y[k2]);
     }
   } else {
     static_assert(M == 8, "M must be 2, 4 or 8");
@@ -220,15 +300,80 @@ __device__ __forceinline__ void unit16xm(const c2 (&a)[4], const D &d, const TW 
       p[r].x = __shfl_xor(y[r].x, 32);
       p[r].y = __shfl_xor(y[r].y, 32);
     }
-    const int64_t s = s0 + (g & 1);
-    if (s < batch) {
+    if (put.has(g & 1)) {
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
         const c2 v = (h == 0) ? cadd(y[k], p[k]) : csub(p[k], y[k]);
-        st_out<INV, SC>(out + s * N + j + 16 * (4 * h + k), v, scale);
+        put(g & 1, j + 16 * (4 * h + k), v);
       }
     }
   }
+}
+
+// Transforms and stores the unit whose first signal is s0 (signals past
+// `batch` are not stored).
+template <int M, bool INV, bool SC, class TW = TwNGlobal<16 * M>, class D = KaratsubaDft16>
+__device__ __forceinline__ void unit16xm(const c2 (&a)[4], const D &d, const TW &tw, c2 *out,
+                                         int64_t s0, int64_t batch, double scale, int g) {
+  unit16xm_to<M>(a, d, tw, HbmSink<16 * M, INV, SC>{out, s0, batch, scale}, g);
+}
+
+// The 16 x M kernel staged through LDS (as tensor_staged_kernel): a unit's
+// 16 / M signals are 256 consecutive values, so a wave's G units are one
+// contiguous tile of 256 G values, loaded and stored 1 KiB per instruction
+// and moved between that order and the matrix-core layout in its LDS buffer.
+template <int M, int G, int WAVES, class D, bool INV, bool SC>
+__global__ __launch_bounds__(64 * WAVES) void tensor16xm_staged_kernel(
+    const c2 *in, c2 *out, int64_t batch, const c2 *__restrict__ tw, double scale) {
+  constexpr int N = 16 * M, SPW = 16 / M, TILE = 256 * G;
+  __shared__ c2 lds[WAVES * TILE];
+  const int lane = lane_id(), g = lane >> 4, j = lane & 15;
+  const int64_t e0 = wave_global() * TILE;  // first value of the wave's tile
+  const int64_t total = batch * N;
+  if (e0 >= total) return;
+  c2 *buf = lds + (threadIdx.x >> 6) * TILE;
+  // fragment and twiddles before the data (the vector-memory counter
+  // retires in order)
+  D d;
+  d.load(tw, lane);
+  TwNRegs w;
+  w.fetch<M>(tw, g, d.col);
+  c2 v[G][4];
+#pragma unroll
+  for (int u = 0; u < G; ++u)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      // past the end: read the last value (no branch) and use zero
+      const int64_t e = e0 + 256 * u + 64 * c + lane;
+      v[u][c] = ld_in<INV>(in + (e < total ? e : total - 1));
+      if (e >= total) v[u][c] = make_double2(0.0, 0.0);
+    }
+#pragma unroll
+  for (int u = 0; u < G; ++u)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) buf[swz16(256 * u + 64 * c + lane)] = v[u][c];
+  wave_lds_sync();
+  // A operand: row j <-> (signal j % SPW, n2 = j / SPW), K index n1 = 4c + g
+#pragma unroll
+  for (int u = 0; u < G; ++u)
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+      v[u][c] = buf[swz16(256 * u + (j % SPW) * N + M * (4 * c + g) + j / SPW)];
+  wave_lds_sync();  // every lane has read its operands before they are overwritten
+#pragma unroll
+  for (int u = 0; u < G; ++u) unit16xm_to<M>(v[u], d, w, LdsSink<N>{buf + 256 * u}, g);
+  wave_lds_sync();
+#pragma unroll
+  for (int u = 0; u < G; ++u)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) v[u][c] = buf[swz16(256 * u + 64 * c + lane)];
+#pragma unroll
+  for (int u = 0; u < G; ++u)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const int64_t e = e0 + 256 * u + 64 * c + lane;
+      if (e < total) st_out<INV, SC>(out + e, v[u][c], scale);
+    }
 }
 
 template <int M, int G, class D, bool INV, bool SC>
@@ -974,11 +1119,14 @@ FFTOC_TENSOR_LAUNCHER(launch_t512, tensor512_kernel, kWaves, 64 * kWaves, 1, kWa
 FFTOC_TENSOR_LAUNCHER(launch_t512g2, tensor512_kernel, 2 * kWaves, 64 * kWaves, 2, kWaves)
 FFTOC_TENSOR_LAUNCHER(launch_t512w1, tensor512_kernel, 1, 64, 1, 1)
 FFTOC_TENSOR_LAUNCHER(launch_t4096, tensor4096_kernel, 1, 64 * k4096Waves)
-// One-wave workgroups: a workgroup moves 4 KiB instead of 16 KiB, which
-// streams faster at large batches (N = 8 x 2^21: 0.736 of HBM against 0.720;
-// N = 32 x 2^19: 0.777 against 0.767; N = 16, 64, 128 gained nothing).
-FFTOC_TENSOR_LAUNCHER(launch_t8w1, tensor8_kernel, 32, 64, 1)
-FFTOC_TENSOR_LAUNCHER(launch_t32w1, tensor16xm_kernel, 8, 64, 2, 1, KaratsubaDft16)
+// LDS-staged N = 8 / 16 (tile: 256 values per wave), one or four waves per
+// workgroup
+FFTOC_TENSOR_LAUNCHER(launch_t8s1, tensor_staged_kernel, 32, 64, 8, 1)
+FFTOC_TENSOR_LAUNCHER(launch_t8s4, tensor_staged_kernel, 128, 256, 8, 4)
+FFTOC_TENSOR_LAUNCHER(launch_t16s1, tensor_staged_kernel, 16, 64, 16, 1)
+FFTOC_TENSOR_LAUNCHER(launch_t16s4, tensor_staged_kernel, 64, 256, 16, 4)
+// LDS-staged 16 x 2 (N = 32; tile: 256 values per wave, one-wave workgroups)
+FFTOC_TENSOR_LAUNCHER(launch_t32s1, tensor16xm_staged_kernel, 8, 64, 2, 1, 1, KaratsubaDft16)
 FFTOC_TENSOR_LAUNCHER(launch_t4096t, tensor4096t_kernel, 1, 64 * k4096Waves)
 FFTOC_TENSOR_LAUNCHER(launch_t4096h, tensor4096h_kernel, 1, 256, 4, KaratsubaCore)
 FFTOC_TENSOR_LAUNCHER(launch_t4096hh, tensor4096h_kernel, 1, 256, 4, TwoProductCore)
@@ -988,17 +1136,26 @@ FFTOC_TENSOR_LAUNCHER(launch_t4096h1h, tensor4096h_kernel, 1, 256, 4, TwoProduct
 
 constexpr unsigned TF = FAM_TENSOR | FAM_ANY;
 // tile: signals per workgroup (4 waves x signals per wave)
-const Variant kT8[] = {{"tensor:diag(F8,F8)/32 per wave", TF, &launch_t8, 128},
+// N = 8, 16, 32: the LDS-staged kernels first (the defaults): 1 KiB contiguous
+// per load and store instruction instead of 64-256 B pieces, measured
+// against the direct kernels in one process: N = 8 x 2^21 0.794 of HBM
+// against 0.737, N = 16 x 2^20 0.792 against 0.721, N = 32 x 2^19 0.798
+// against 0.780 (2^17: 0.704 against 0.659).  The same staging at N = 64
+// and 128 did not beat their defaults.
+const Variant kT8[] = {{"tensor:diag(F8,F8)/32 per wave/LDS-staged", TF, &launch_t8s4, 128},
+                       {"tensor:diag(F8,F8)/32 per wave/LDS-staged/1 wave", TF, &launch_t8s1, 32},
+                       {"tensor:diag(F8,F8)/32 per wave", TF, &launch_t8, 128},
                        {"tensor:diag(F8,F8)/64 per wave", TF, &launch_t8g2, 256},
-                       {"tensor:diag(F8,F8)/128 per wave", TF, &launch_t8g4, 512},
-                       {"tensor:diag(F8,F8)/32 per wave/1 wave", TF, &launch_t8w1, 32}};
-const Variant kT16[] = {{"tensor:16/16 per wave", TF, &launch_t16, 64},
+                       {"tensor:diag(F8,F8)/128 per wave", TF, &launch_t8g4, 512}};
+const Variant kT16[] = {{"tensor:16/16 per wave/LDS-staged/1 wave", TF, &launch_t16s1, 16},
+                        {"tensor:16/16 per wave/LDS-staged", TF, &launch_t16s4, 64},
+                        {"tensor:16/16 per wave", TF, &launch_t16, 64},
                         {"tensor:16/32 per wave", TF, &launch_t16g2, 128},
                         {"tensor:16/64 per wave", TF, &launch_t16g4, 256}};
-const Variant kT32[] = {{"tensor:16x2/16 per wave", TF, &launch_t32g2, 64},
+const Variant kT32[] = {{"tensor:16x2/8 per wave/LDS-staged/1 wave", TF, &launch_t32s1, 8},
+                        {"tensor:16x2/16 per wave", TF, &launch_t32g2, 64},
                         {"tensor:16x2/8 per wave", TF, &launch_t32, 32},
-                        {"tensor:16x2/32 per wave", TF, &launch_t32g4, 128},
-                        {"tensor:16x2/8 per wave/1 wave", TF, &launch_t32w1, 8}};
+                        {"tensor:16x2/32 per wave", TF, &launch_t32g4, 128}};
 const Variant kT64[] = {{"tensor:16x4/8 per wave", TF, &launch_t64g2, 32},
                         {"tensor:16x4/8 per wave/persistent", TF, &launch_tensor_persistent<Unit16xM<4>, 2>, 32},
                         {"tensor:16x4/8 per wave/persistent/2-product", TF, &launch_tensor_persistent<Unit16xM<4, TwoProductDft16>, 2>, 32},
