@@ -45,6 +45,38 @@ def test_forward_matches_oracle(path, n):
     assert np.max(err) <= TOL, (path, n, np.max(err))
 
 
+# Defaults that take over only from a batch size on (one-wave lane
+# workgroups; the staged and the persistent one-wave matrix-core kernels with
+# their workgroup caps), each at its first batch bucket plus 3 signals (a
+# partial last tile): forward against the oracle on the first and last 32
+# signals, then the normalised inverse of that output gives every input back.
+LARGE_BATCH_DEFAULTS = [("best", 8, 1 << 18, "radix:8/lanes/S8/64 threads"),
+                        ("best", 16, 1 << 17, "radix:16/lanes/S4/64 threads"),
+                        ("best", 32, 1 << 16, "radix:32/lanes/S4/64 threads"),
+                        ("tensor", 8, 1 << 18, None), ("tensor", 256, 1 << 14, None),
+                        ("tensor", 512, 1 << 14, None)]
+
+
+@pytest.mark.parametrize("path,n,batch,variant", LARGE_BATCH_DEFAULTS)
+def test_large_batch_defaults_forward_and_normalized_inverse(path, n, batch, variant):
+    x = signals(n, batch + 3, seed=1700 + n)
+    print(f"[large-batch default] path={path} n={n} batch={x.shape[0]}", flush=True)
+    if variant is not None:
+        assert fo.variants(n)[fo.lib().fftoc_get_variant(n, 0, x.shape[0])] == variant
+    fn = fo.fft if path == "best" else fo.tensor_fft
+    src = fo.DeviceBuffer.from_array(x)
+    dst = fo.DeviceBuffer(x.nbytes)
+    fn(src, x.shape[0], n, out=dst)
+    fo.synchronize()
+    got = dst.download(x.shape)
+    idx = np.r_[0:32, x.shape[0] - 32:x.shape[0]]
+    err = np.max(oracle.rel_l2(got[idx], oracle.reference(x[idx]), axis=-1))
+    assert err <= TOL, (path, n, err)
+    fn(dst, x.shape[0], n, out=src, flags=fo.INVERSE | fo.NORMALIZE)
+    fo.synchronize()
+    assert np.max(np.abs(src.download(x.shape) - x)) < 1e-13, (path, n)
+
+
 @pytest.mark.parametrize("path,n", PATH_SIZES)
 def test_inverse_and_normalize(path, n):
     x = signals(n, 33, seed=100 + n)
