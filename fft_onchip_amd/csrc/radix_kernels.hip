@@ -261,7 +261,9 @@ const Variant kV4096[] = {
 // complex64 data (FP32 arithmetic, float2 twiddles rounded from the FP64 table).
 // The one-wave lane workgroups that lead for complex128 at N = 8, 16, 32 lose
 // here (N = 8 at 2^22 signals: 0.59 of HBM against 0.80): one load
-// instruction moves 512 B, too little work per wave.
+// instruction moves 512 B, too little work per wave.  One-wave 4x4x4 tiles
+// at N = 64 (S4 / S8, one-shot or pipelined) stayed 1-10 % below the S16
+// pipelined default.
 #define FFTOC_V32(NAME, FAM, SPLIT_S_FORM) \
   Variant { NAME, FAM, &launch_split<FFTOC_UNPAREN SPLIT_S_FORM, f2>, \
             tile_of<FFTOC_UNPAREN SPLIT_S_FORM, f2>() }
