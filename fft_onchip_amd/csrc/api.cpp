@@ -167,8 +167,8 @@ Default default_best(int n, int bucket) {
     case 32: return {bucket >= 16 ? "radix:32/lanes/S4/64 threads" : "radix:32/lanes/S32", 0};
     case 128: return {"radix:16x8/tile/S8", bucket >= 16 ? 6 : 0};
     case 256: return {"radix:16x16/tile/S4", bucket >= 15 ? 6 : 0};
-    case 512:
-      return bucket >= 14 ? Default{"radix:8x8x8/pipelined/S4", 1}
+    case 512:  // two-wave pipelined workgroups, two per CU (one wave per SIMD)
+      return bucket >= 13 ? Default{"radix:8x8x8/pipelined/S2", 2}
                           : Default{"radix:8x8x8/tile/S1", 0};
     case 1024:
       return bucket >= 13 ? Default{"radix:8x8x8x2/pipelined/S1", 3}

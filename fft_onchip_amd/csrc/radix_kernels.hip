@@ -234,6 +234,7 @@ const Variant kV512[] = {
     FFTOC_V("radix:16x16x2/persistent/S4", MIX, Split<16, 16, 2>, 4, Form::Persistent),
     FFTOC_V("radix:16x16x2/tile/S2", MIX, Split<16, 16, 2>, 2, Form::OneTile),
     FFTOC_V("radix:8x8x8/pipelined/S4", R8, Split<8, 8, 8>, 4, Form::Pipelined),
+    FFTOC_V("radix:8x8x8/pipelined/S2", R8, Split<8, 8, 8>, 2, Form::Pipelined),
 };
 const Variant kV1024[] = {
     FFTOC_V("radix:16x16x4/persistent/S2", MIX, Split<16, 16, 4>, 2, Form::Persistent),
