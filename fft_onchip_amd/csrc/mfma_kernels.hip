@@ -23,6 +23,10 @@
 //    wave keeps 12 doubles of DFT matrix (Br, Bi, Br+Bi per K-chunk).
 //  * Twiddles and DFT-matrix entries come from the FP64 table W_4096^j built
 //    on the host; only additions are done in-kernel (Br + Bi).
+//  * HBM access: N = 8, 16, 32 stage each wave's 4 KiB tile through LDS so
+//    every load and store instruction moves 1 KiB contiguously; N = 256 and
+//    512 run persistent one-wave workgroups, four per CU (one wave per SIMD)
+//    at large batches; N = 4096 loads 1 KiB-contiguous columns directly.
 #include "../../include/fftoc_device.hpp"
 #include "../../include/fftoc_mfma.hpp"
 #include "fftoc_internal.hpp"
