@@ -190,14 +190,17 @@ Default default_c64(int n, int bucket) {
 }
 
 // Default choice of fftoc_tensor_fft where it depends on the batch size (else
-// the first tensor variant of the length).  N = 256 and 512 from 2^14
-// signals: persistent kernels with one-wave workgroups held to four per CU
-// (one wave per SIMD): N = 256 0.75-0.80 of HBM against 0.69-0.74, N = 512
-// 0.68-0.75 against 0.65-0.69 for four-wave workgroups at their occupancy;
-// three or five per CU measured lower (N = 512: 0.63-0.68 / 0.58).
+// the first tensor variant of the length).  N = 128 from 2^15 signals and
+// N = 256, 512 from 2^14: persistent kernels with one-wave workgroups held
+// to four per CU (one wave per SIMD), one unit of 4-8 KiB per iteration:
+// N = 128 0.68-0.78 of HBM against 0.64-0.78, N = 256 0.75-0.80 against
+// 0.69-0.74, N = 512 0.68-0.75 against 0.65-0.69 for four-wave workgroups at
+// their occupancy; three or five per CU measured lower (N = 512: 0.63-0.68 /
+// 0.58).
 Default default_tensor(int n, int bucket) {
   if (n == 64 && bucket <= 16) return {"tensor:16x4/8 per wave/persistent", 0};
   if (n == 8 && bucket >= 18) return {"tensor:diag(F8,F8)/32 per wave/LDS-staged/1 wave", 0};
+  if (n == 128 && bucket >= 15) return {"tensor:16x8/2 per wave/persistent/1 wave", 4};
   if (n == 256 && bucket >= 14) return {"tensor:16x16/1 per wave/persistent/1 wave", 4};
   if (n == 512 && bucket >= 14) return {"tensor:16x2x16/1 per wave/persistent/1 wave", 4};
   return {nullptr, 0};

@@ -583,7 +583,9 @@ __global__ __launch_bounds__(64 * kWaves) void tensor256_kernel(const c2 *in, c2
 // signals at four workgroups per CU (one wave per SIMD): 0.75-0.80 of HBM
 // against 0.69-0.74 for the two-signal kernel above (three or five per CU:
 // 0.62-0.74; the one-matrix-stage and two-product cores within 1 % at 2^16
-// signals, 3 % lower at 2^18; the same form at N = 32 and 128 did not gain).
+// signals, 3 % lower at 2^18; at N = 128 the same shape pays through
+// tensor_persistent_kernel with one 16x8 unit per iteration; at N = 32 it did
+// not gain).
 // The core's DFT fragment and twiddles live in registers, loaded once before
 // the first signal's data, so no vector-memory load inside the loop waits
 // behind the prefetch.  No workgroup barrier: a wave without a signal leaves.
@@ -1169,7 +1171,8 @@ const Variant kT64[] = {{"tensor:16x4/8 per wave", TF, &launch_t64g2, 32},
 const Variant kT128[] = {{"tensor:16x8/4 per wave", TF, &launch_t128g2, 16},
                          {"tensor:16x8/4 per wave/2-product", TF, &launch_t128g2h, 16},
                          {"tensor:16x8/2 per wave", TF, &launch_t128, 8},
-                         {"tensor:16x8/8 per wave", TF, &launch_t128g4, 32}};
+                         {"tensor:16x8/8 per wave", TF, &launch_t128g4, 32},
+                         {"tensor:16x8/2 per wave/persistent/1 wave", TF, &launch_tensor_persistent<Unit16xM<8>, 1, 1>, 2}};
 const Variant kT256[] = {{"tensor:16x16/2 per wave", TF, &launch_t256g2, 8},
                          {"tensor:16x16/2 per wave/2-product", TF, &launch_t256g2h, 8},
                          {"tensor:16x16/4 per wave", TF, &launch_t256g4, 16},

@@ -53,7 +53,8 @@ def test_forward_matches_oracle(path, n):
 LARGE_BATCH_DEFAULTS = [("best", 8, 1 << 18, "radix:8/lanes/S8/64 threads"),
                         ("best", 16, 1 << 17, "radix:16/lanes/S4/64 threads"),
                         ("best", 32, 1 << 16, "radix:32/lanes/S4/64 threads"),
-                        ("tensor", 8, 1 << 18, None), ("tensor", 256, 1 << 14, None),
+                        ("tensor", 8, 1 << 18, None), ("tensor", 128, 1 << 15, None),
+                        ("tensor", 256, 1 << 14, None),
                         ("tensor", 512, 1 << 14, None)]
 
 
@@ -259,7 +260,8 @@ def test_autotune_and_tuning_roundtrip(tmp_path, c64):
     assert np.max(oracle.rel_l2(got, oracle.reference(x.astype(np.complex128)), axis=-1)) <= tol
 
 
-@pytest.mark.parametrize("n,c64", [(8, False), (64, False), (256, False), (512, False), (4096, False),
+@pytest.mark.parametrize("n,c64", [(8, False), (64, False), (128, False), (256, False), (512, False),
+                                   (4096, False),
                                    (64, True), (128, True), (256, True), (512, True),
                                    (4096, True)])
 def test_workgroups_per_cu_caps_keep_results_bit_for_bit(n, c64):
